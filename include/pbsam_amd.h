@@ -1,0 +1,142 @@
+/*
+ * pbsam_amd: MI355X (gfx950) implementation of PB-SAM's mutual-polarization
+ * solve, with the C ABI that replaces the reference's CMolecule entry points
+ * on that path.  Plain C types only; the library owns all device memory.
+ *
+ * Units follow the reference: lengths in Angstrom, charges in e, energies in
+ * the reference's internal units (e^2/A scaled by 1/eps_in); conversions to
+ * kT or kcal/mol are left to the caller (reference: src/BDnam.h:10,19-20).
+ *
+ * Coefficient blocks use the reference layout (src/expansion.cpp:57-63): a block
+ * of pole order p holds p*p doubles; order n occupies [n*n, n*n+2n] as
+ * [c(n,0), Re c(n,1), Im c(n,1), ..., Re c(n,n), Im c(n,n)].
+ */
+#ifndef PBSAM_AMD_H
+#define PBSAM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference exits or asserts instead; these never exit) */
+#define PBSAM_OK 0
+#define PBSAM_ERR_BADARG 1      /* invalid argument */
+#define PBSAM_ERR_OVERLAP 2     /* two spheres of different molecules overlap
+                                   (reference: computeForces returns false,
+                                   src/molecule.cpp:1090-1097) */
+#define PBSAM_ERR_NOCONV 3      /* round limit reached before the tolerance */
+#define PBSAM_ERR_HIP 4         /* HIP runtime error */
+#define PBSAM_ERR_OOM 5         /* device allocation failed */
+#define PBSAM_ERR_UNSUPPORTED 6 /* input needs a feature this build lacks */
+#define PBSAM_ERR_CHARGE 7      /* a charge lies outside every sphere
+                                   (reference exits, src/molecule.cpp:1019-1020) */
+#define PBSAM_ERR_STATE 8       /* call out of order (e.g. no self-polarization yet) */
+
+#define PBSAM_PMAX 32           /* largest pole order */
+
+typedef struct pbsam_ctx pbsam_ctx;
+typedef struct pbsam_moltype pbsam_moltype;
+typedef struct pbsam_system pbsam_system;
+
+/* Context: one HIP device and stream, the physical constants and recursion tables.
+ * Replaces the process-wide statics set by CSystem::initConstants
+ * (src/system.cpp:16-25) and CMolecule::initConstants (src/molecule.cpp:42-70). */
+int pbsam_ctx_create(double kappa, double eps_out, int device, pbsam_ctx **out);
+void pbsam_ctx_destroy(pbsam_ctx *ctx);
+const char *pbsam_last_error(const pbsam_ctx *ctx);
+/* Debye kappa (1/A) from a salt concentration in mol/L, as src/selfp_makemat.cpp:59-60 */
+double pbsam_kappa_from_salt(double salt_molar, double eps_out, double temp_k);
+
+/* Molecule type: spheres, charges, surface points, per-sphere constants.
+ * Replaces readpqr + generateMolSPX + generateMolTypeIntraPolLists + the CMolecule
+ * self-polarization constructor (src/molecule.cpp:355-415, 501-535, 756-780, 878-1023;
+ * src/expcenter.cpp:375-390, 470-552, 1042-1060). */
+typedef struct pbsam_moltype_desc {
+  int n_sph;
+  const double *cen;     /* 3*n_sph sphere centres (input frame) */
+  const double *rad;     /* n_sph radii */
+  int n_chg;
+  const double *chg_xyz; /* 3*n_chg */
+  const double *chg_q;   /* n_chg, in e */
+  const double *rcen;    /* 3: molecule centre; NULL = origin */
+  int p;                 /* pole order, 1..PBSAM_PMAX */
+  double eps_in;         /* interior dielectric */
+  const int *npts;       /* per-sphere spiral size, NULL = the reference's radius rule */
+  int npts_cap;          /* cap of the radius rule (reference: 10000) */
+  double intra_rcutoff;  /* near/far split (reference: 30 A) */
+  const double *imats;   /* n_sph*p^4 column-major surface-integral matrices, or NULL */
+  int imat_npts;         /* with imats NULL: spiral size used to detect buried points;
+                            spheres with none get the exact identity matrix */
+} pbsam_moltype_desc;
+
+int pbsam_moltype_create(pbsam_ctx *ctx, const pbsam_moltype_desc *desc, pbsam_moltype **out);
+void pbsam_moltype_destroy(pbsam_moltype *mt);
+int pbsam_moltype_info(const pbsam_moltype *mt, int *n_sph, int *p, long long *n_points);
+/* exposed surface points of sphere k (relative to its centre) and the spiral size */
+int pbsam_moltype_points(const pbsam_moltype *mt, int k, double *xyz, int *n_exposed, int *n_spiral);
+
+/* Self-polarization (CMolecule::polarize_self, src/molecule.cpp:1899-1979), run as
+ * Jacobi rounds on the device until the largest per-sphere change is <= tol or
+ * max_rounds is reached.  rounds_out may be NULL.  Stores F/H as the self values. */
+int pbsam_selfpol(pbsam_moltype *mt, double tol, int max_rounds, int *rounds_out);
+/* fixed number of Jacobi rounds from the current state (does not store self values) */
+int pbsam_selfpol_rounds(pbsam_moltype *mt, int rounds, double *maxdev_out);
+/* Inject self-polarized F/H (n_sph*p^2 each), as reading .exp files
+ * (readExpansionsP, src/readutil.cpp:564-666). */
+int pbsam_moltype_set_self(pbsam_moltype *mt, const double *F, const double *H);
+/* Exposed self charges + frozen intra-molecular re-expansions:
+ * generateMolExposedChargesFH + computeMolTypeValues (src/molecule.cpp:545-810). */
+int pbsam_moltype_values(pbsam_moltype *mt);
+/* current self-polarization state of sphere k (any pointer may be NULL);
+ * tq = {TQF, TQH, dev} */
+int pbsam_moltype_get(const pbsam_moltype *mt, int k, double *F, double *H, double *qF, double *qH,
+                      double *tq);
+int pbsam_moltype_get_intra(const pbsam_moltype *mt, int k, double *LF, double *LH, double *LFfar,
+                            double *LHfar);
+
+/* System of rigid molecules; pose = centre position + unit quaternion (w,x,y,z).
+ * Replaces the BD CMolecule constructor + setPos/setOrient (src/molecule.cpp:422-488,
+ * src/molecule.h:325-329) and CMolecule::initMutualConstants (src/molecule.cpp:78-95). */
+int pbsam_system_create(pbsam_ctx *ctx, int n_mol, pbsam_moltype *const *mts, const double *pos,
+                        const double *quat, pbsam_system **out);
+void pbsam_system_destroy(pbsam_system *sys);
+int pbsam_system_set_pose(pbsam_system *sys, int i, const double *pos, const double *quat);
+int pbsam_system_set_cutoffs(pbsam_system *sys, double inter_rcutoff, double interact_rcutoff);
+
+/* Mutual polarization (CMolecule::polarize_mutual phases 0-2, src/molecule.cpp:1989-2075)
+ * preceded by the pair lists (generateInterXFormsForPolarize_LowMemory :1036-1126).
+ * Jacobi rounds until the largest change <= tol.  flags: PBSAM_HROT_REFRESH refreshes the
+ * rotated H used for inter-molecular coupling every round (the reference keeps round-1
+ * values).  Returns PBSAM_ERR_OVERLAP when two molecules' spheres overlap. */
+#define PBSAM_HROT_REFRESH 1
+int pbsam_solve_mutual(pbsam_system *sys, double tol, int max_rounds, unsigned flags, int *rounds_out);
+/* fixed number of Jacobi rounds after the first (round 1 is always from self values) */
+int pbsam_mutual_rounds(pbsam_system *sys, int rounds, unsigned flags, double *maxdev_out);
+int pbsam_system_get(const pbsam_system *sys, int mol, int k, double *F, double *H, double *LS,
+                     double *Hrot);
+/* CMolecule::computePot (src/moldynamics.cpp:509-521) per molecule */
+int pbsam_energy(pbsam_system *sys, double *e_per_mol);
+int pbsam_system_counts(const pbsam_system *sys, int *n_inter_pairs, int *n_polarized);
+
+/* ---- kernel-level entry points (one batch of independent jobs; host arrays in/out) ---- */
+/* CXFormA::reset + xformH (src/xforms.cpp:51-77,123-132) for n pairs.
+ * P: 3n (target - source), a: 2n (source radius, target radius), order: n (<= p),
+ * in/out: n*p*p blocks. */
+int pbsam_xform_h_batch(pbsam_ctx *ctx, int n, int p, const double *P, const double *a, const int *order,
+                        const double *in, double *out);
+/* CXFormAIntra::reset + xformF/xformH (src/xforms.cpp:214-270) */
+int pbsam_xform_intra_batch(pbsam_ctx *ctx, int n, int p, const double *P, const double *a,
+                            const int *orders /* 2n: pH, pF */, const double *H, const double *F,
+                            double *outH, double *outF);
+/* CXFormNIntra::xformFH (src/xforms.cpp:528-550) for one source point set and n targets */
+int pbsam_xform_num_batch(pbsam_ctx *ctx, int n, int p, const double *P, const double *a, int npts,
+                          const double *xyz, const double *qF, const double *qH, double tQF, double tQH,
+                          double *outF, double *outH);
+/* CRotCoeff::reset(CQuat) + rotateWithXi (src/rotcoeff.cpp:181-201,250-350) */
+int pbsam_rotate_quat_batch(pbsam_ctx *ctx, int n, int p, const double *quat, int fwd, const double *in,
+                            double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1708 @@
+// Host side of the MI355X PB-SAM library: context, molecule types, systems and the C ABI
+// declared in include/pbsam_amd.h.  Geometry set-up runs on the host (once per molecule
+// type); every per-round computation runs in the HIP kernels of pbsam_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pbsam_amd.h"
+#include "pbsam_kernels.hip"
+
+using namespace pbsam;
+
+namespace {
+
+// ----------------------------------------------------------------------------
+// error handling
+// ----------------------------------------------------------------------------
+struct Err {
+  int code;
+};
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      fail_hip(#x, e_);                                                            \
+    }                                                                              \
+  } while (0)
+
+thread_local std::string *g_err = nullptr;
+[[noreturn]] void fail(int code, const std::string &msg) {
+  if (g_err) *g_err = msg;
+  throw Err{code};
+}
+[[noreturn]] void fail_hip(const char *what, hipError_t e) {
+  fail(e == hipErrorOutOfMemory ? PBSAM_ERR_OOM : PBSAM_ERR_HIP,
+       std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// device buffer
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    n = count;
+    if (count) HIPCHK(hipMalloc(&p, count * sizeof(T)));
+  }
+  void upload(const std::vector<T> &v, hipStream_t s) {
+    alloc(v.size());
+    if (n) HIPCHK(hipMemcpyAsync(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void zero(hipStream_t s) {
+    if (n) HIPCHK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+  std::vector<T> download(hipStream_t s) const {
+    std::vector<T> v(n);
+    if (n) HIPCHK(hipMemcpyAsync(v.data(), p, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return v;
+  }
+};
+
+inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline void cross3(const double *a, const double *b, double *o) {
+  double x = a[1] * b[2] - a[2] * b[1], y = a[2] * b[0] - a[0] * b[2], z = a[0] * b[1] - a[1] * b[0];
+  o[0] = x; o[1] = y; o[2] = z;
+}
+// q * r for a unit quaternion q = (w,x,y,z)  (util.h:463-468)
+inline void quat_rot(const double *q, const double *r, double *o) {
+  double qr[3], t[3], u[3];
+  cross3(q + 1, r, qr);
+  cross3(q + 1, qr, t);
+  for (int i = 0; i < 3; i++) u[i] = q[0] * qr[i] + t[i];
+  double x = r[0] + 2 * u[0], y = r[1] + 2 * u[1], z = r[2] + 2 * u[2];
+  o[0] = x; o[1] = y; o[2] = z;
+}
+inline void quat_normalize(double *q) {
+  double in = 1.0 / sqrt((q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) + q[0] * q[0]);
+  for (int i = 1; i < 4; i++) q[i] *= in;
+  q[0] *= in;
+}
+// Euler angles of CRotCoeff::reset(CQuat) (rotcoeff.cpp:181-201)
+inline void quat_euler(const double *q, double &theta, double &phi, double &xi) {
+  double w = q[0], x = q[1], y = q[2], z = q[3];
+  theta = acos(1.0 - 2 * x * x - 2 * y * y);
+  if (fabs(theta) > 1e-12) {
+    phi = atan2(y * z + w * x, x * z - w * y);
+    xi = atan2(y * z - w * x, x * z + w * y);
+  } else {
+    theta = 0.0;
+    phi = 0.0;
+    xi = 2 * acos(w);
+  }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// context
+// ----------------------------------------------------------------------------
+struct pbsam_ctx {
+  double kappa, sdiel;
+  int device;
+  hipStream_t stream = nullptr;
+  std::string err;
+  DBuf<double> t_c1, t_c2, t_c3, t_c6, t_c4, t_zeta, t_eta, t_alpha, t_beta, t_gam, t_del, t_Q;
+  Tabs tabs;
+  size_t lds_max = 65536;
+};
+
+static void build_tables(pbsam_ctx *c) {
+  const int NL = NLT, PM = PMAX;
+  std::vector<double> c1(NL * NL), c2(NL * NL), c3(NL * NL), c6(NL), c4(NL);
+  // expansion.cpp:46-96
+  std::vector<double> temp(2 * NL);
+  temp[0] = 1.0;
+  for (int i = 1; i < 2 * NL; i++) temp[i] = temp[i - 1] * sqrt((double)i);
+  for (int n = 0; n < NL; n++)
+    for (int m = 0; m <= n; m++) {
+      c1[n * NL + m] = (2 * n - 1) / (double)(n - m);
+      c2[n * NL + m] = (n + m - 1) / (double)(n - m);
+      c3[n * NL + m] = temp[n - m] / temp[n + m];
+    }
+  c6[0] = 1.0;
+  c6[1] = 1.0;
+  for (int n = 2; n < NL; n++) c6[n] = c6[n - 1] * (2 * n - 1);
+  for (int n = 0; n < NL; n++) c4[n] = 1.0 / ((2 * n - 1) * (2 * n - 3));
+  // rotcoeff.cpp:16-33
+  std::vector<double> zeta(NL * NL, 0.0), eta(NL * 2 * NL, 0.0);
+  for (int n = 0; n < NL; n++)
+    for (int m = 0; m <= n; m++) {
+      zeta[n * NL + m] = sqrt((double)(n + m + 1) * (n - m + 1) / ((2 * n + 1) * (2 * n + 3)));
+      eta[n * 2 * NL + NL + m] = sqrt((double)(n - m - 1) * (n - m) / ((2 * n + 1) * (2 * n - 1)));
+      if (m != 0)
+        eta[n * 2 * NL + NL - m] = -sqrt((double)(n + m - 1) * (n + m) / ((2 * n + 1) * (2 * n - 1)));
+    }
+  // transcoeff.cpp:16-38
+  const double kapsqr = (c->kappa == 0 ? 0 : 1.0);
+  std::vector<double> al(NL * PM), be(NL * PM), ga(NL * 2 * PM, 0.0), de(NL * 2 * PM, 0.0);
+  for (int n = 0; n < NL; n++)
+    for (int m = 0; m < PM; m++) {
+      double a = sqrt((double)(n + m + 1) * (n - m + 1));
+      al[n * PM + m] = a;
+      be[n * PM + m] = kapsqr * a / ((2 * n + 1) * (2 * n + 3));
+      double g = sqrt((double)(n - m - 1) * (n - m));
+      ga[n * 2 * PM + PM + m] = g;
+      de[n * 2 * PM + PM + m] = kapsqr * g / ((2 * n + 1) * (2 * n - 1));
+      if (m != 0) {
+        double gn = -sqrt((double)(n + m - 1) * (n + m));
+        ga[n * 2 * PM + PM - m] = gn;
+        de[n * 2 * PM + PM - m] = kapsqr * gn / ((2 * n + 1) * (2 * n - 1));
+      }
+    }
+  std::vector<double> Q((2 * PM - 1) * PM * 2, 0.0);  // reserved for the gradient transform
+  hipStream_t s = c->stream;
+  c->t_c1.upload(c1, s); c->t_c2.upload(c2, s); c->t_c3.upload(c3, s);
+  c->t_c6.upload(c6, s); c->t_c4.upload(c4, s);
+  c->t_zeta.upload(zeta, s); c->t_eta.upload(eta, s);
+  c->t_alpha.upload(al, s); c->t_beta.upload(be, s); c->t_gam.upload(ga, s); c->t_del.upload(de, s);
+  c->t_Q.upload(Q, s);
+  Tabs &t = c->tabs;
+  t.c1 = c->t_c1.p; t.c2 = c->t_c2.p; t.c3 = c->t_c3.p; t.c6 = c->t_c6.p; t.c4 = c->t_c4.p;
+  t.zeta = c->t_zeta.p; t.eta = c->t_eta.p; t.alpha = c->t_alpha.p; t.beta = c->t_beta.p;
+  t.gam = c->t_gam.p; t.del = c->t_del.p; t.Q = c->t_Q.p;
+  t.kappa = c->kappa;
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+// ----------------------------------------------------------------------------
+// kernel launch helpers
+// ----------------------------------------------------------------------------
+static size_t xform_lds(int p) {
+  const int NR = 2 * p - 1;
+  return (size_t)2 * NR * NR * sizeof(cd) + (size_t)2 * p * p * sizeof(double) + (size_t)4 * p * sizeof(cd);
+}
+static int p2l_tile(int p) {
+  int t = (int)(60000 / (8 * (p * p + 2 * p + 3)));
+  return std::max(1, std::min(64, t));
+}
+static size_t p2l_lds(int p) {
+  int T = p2l_tile(p);
+  return (size_t)8 * ((size_t)T * p * p + 2 * T * p + 3 * T);
+}
+static void set_lds(const void *k, size_t bytes) {
+  if (bytes > 65536) HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+static void launch_trans(pbsam_ctx *c, int njobs, const DBuf<TJob> &jobs, int p, double *out) {
+  if (!njobs) return;
+  size_t lds = sizeof(double) * ((size_t)(2 * p) * p + 2 * p + p);
+  hipLaunchKernelGGL(k_trans, dim3(njobs), dim3(64), lds, c->stream, c->tabs, jobs.p, p, out);
+  HIPCHK(hipGetLastError());
+}
+static void launch_xform(pbsam_ctx *c, int njobs, const XArgs &a) {
+  if (!njobs) return;
+  size_t lds = xform_lds(a.p);
+  set_lds((const void *)k_xform, lds);
+  hipLaunchKernelGGL(k_xform, dim3(njobs), dim3(XT), lds, c->stream, c->tabs, a);
+  HIPCHK(hipGetLastError());
+}
+static void launch_p2l(pbsam_ctx *c, int njobs, PArgs a) {
+  if (!njobs) return;
+  a.tile = p2l_tile(a.p);
+  size_t lds = p2l_lds(a.p);
+  hipLaunchKernelGGL(k_p2l, dim3(njobs), dim3(PT), lds, c->stream, c->tabs, a);
+  HIPCHK(hipGetLastError());
+}
+static void launch_solve(pbsam_ctx *c, int n, const SArgs &a) {
+  if (!n) return;
+  size_t lds = sizeof(double) * (3 * (size_t)a.p * a.p + 8);
+  hipLaunchKernelGGL(k_solve, dim3(n), dim3(ST), lds, c->stream, a);
+  HIPCHK(hipGetLastError());
+}
+static void launch_surface(pbsam_ctx *c, int n, const QArgs &a) {
+  if (!n) return;
+  size_t lds = sizeof(double) * (2 * (size_t)a.p * a.p + 8);
+  hipLaunchKernelGGL(k_surface, dim3(n), dim3(ST), lds, c->stream, c->tabs, a);
+  HIPCHK(hipGetLastError());
+}
+static void launch_rotcoeff(pbsam_ctx *c, int n, RJob *jobs, int p, double *out) {
+  if (!n) return;
+  const int NR = 2 * p - 1;
+  size_t lds = (size_t)2 * NR * NR * sizeof(cd);
+  set_lds((const void *)k_rotcoeff, lds);
+  hipLaunchKernelGGL(k_rotcoeff, dim3(n), dim3(XT), lds, c->stream, c->tabs, jobs, p, out);
+  HIPCHK(hipGetLastError());
+}
+static void launch_rotapply(pbsam_ctx *c, int n, const RAJob *jobs, const RJob *rj, const double *Rst,
+                            const double *in, double *out, int p) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_rotapply, dim3(n), dim3(256), 0, c->stream, jobs, rj, Rst, in, out, p);
+  HIPCHK(hipGetLastError());
+}
+static void launch_sumslots(pbsam_ctx *c, int n, const LArgs &a) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_sumslots, dim3(n), dim3(256), 0, c->stream, a);
+  HIPCHK(hipGetLastError());
+}
+
+// ----------------------------------------------------------------------------
+// molecule types
+// ----------------------------------------------------------------------------
+struct SphHost {
+  double cen[3];  // relative to rcen
+  double rad;
+  int nexp, nspx, pt_off;
+  std::vector<int> neigh, near, far;
+};
+
+// per-sphere constants of CSolExpCenter::initMyConstants (expcenter.cpp:470-505)
+static void sphere_consts(double kappa, double sdiel, double rad, double eps_in, int p, double *out) {
+  std::vector<double> K(p + 1, 1.0), I(p + 1, 1.0);
+  double val = kappa * rad, valsqr = val * val, exp_val = exp(-1.0 * val), eps = eps_in / sdiel;
+  if (val != 0.0 && p + 1 > 1) {  // CLExpan::BESSEL
+    K[1] += val;
+    for (int n = 2; n < p + 1; n++) K[n] = K[n - 1] + valsqr * K[n - 2] * (1.0 / ((2 * n - 1) * (2 * n - 3)));
+  }
+  if (val != 0.0) {  // CMExpan::BESSEL
+    double z = 0.5 * val * val;
+    for (int n = 0; n < p + 1; n++) {
+      double t = z / (2 * n + 3);
+      for (int j = 1; j <= 20; j++) {
+        I[n] += t;
+        t *= (z / ((j + 1) * (2 * (n + j) + 3)));
+        if (t < 1e-20) break;
+      }
+    }
+  }
+  for (int n = 0; n < p; n++) {
+    double DH = -exp_val * K[n];
+    double VH = exp_val * (n * K[n] - (2 * n + 1) * K[n + 1]);
+    double VF = -eps * n;
+    out[0 * p + n] = VH;                                                          // cH1
+    out[1 * p + n] = VF + (2 * n + 1);                                            // cF1
+    out[2 * p + n] = rad * (n * I[n] + valsqr * I[n + 1] / double(2 * n + 3));    // cLH1
+    out[3 * p + n] = -rad * eps * n;                                              // cLF1
+    out[4 * p + n] = DH + (2 * n + 1) / I[n];                                     // cH2
+    out[5 * p + n] = 1.0;                                                         // cF2
+    out[6 * p + n] = -rad * I[n];                                                 // cLH2
+    out[7 * p + n] = rad;                                                         // cLF2
+    out[8 * p + n] = I[n] * (1 / (double)(2 * n + 1));                            // cInvH
+    out[9 * p + n] = ((2 * n + 1) / (4.0 * M_PI)) / I[n];                         // cChgH
+  }
+}
+
+// reference spiral (CMolecule::spherePts molecule.cpp:2273-2294) scaled to radius r
+static void spiral(int N, double r, std::vector<double> &xyz) {
+  std::vector<double> th(N, 0.0), ph(N, 0.0);
+  for (int i = 1; i <= N; i++) {
+    double h = -1.0 + 2.0 * (i - 1.0) / (N - 1.0);
+    th[i - 1] = acos(h);
+    if (i == 1 || i == N) ph[i - 1] = 0;
+    else ph[i - 1] = (ph[i - 2] + 3.6 / sqrt(N * (1.0 - h * h)));
+    while (ph[i - 1] > 2 * M_PI) ph[i - 1] -= 2 * M_PI;
+    while (ph[i - 1] < -2 * M_PI) ph[i - 1] += 2 * M_PI;
+  }
+  xyz.resize(3 * N);
+  for (int i = 0; i < N; i++) {
+    double x = 1 * sin(th[i]) * cos(ph[i]), y = 1 * sin(th[i]) * sin(ph[i]), z = 1 * cos(th[i]);
+    xyz[3 * i] = r * x;
+    xyz[3 * i + 1] = r * y;
+    xyz[3 * i + 2] = r * z;
+  }
+}
+static int npts_rule(double r, int nin) {  // molecule.cpp:957-966
+  const double cutoff = 1.5, maxrad = 10;
+  const double gradient = nin / ((maxrad - cutoff) * (maxrad - cutoff));
+  if (r < cutoff) return 200;
+  int N = int(200 + gradient * (r - cutoff) * (r - cutoff));
+  return N > nin ? nin : N;
+}
+
+struct pbsam_moltype {
+  pbsam_ctx *c;
+  int p, ns;
+  double eps_in, intra_rc, rcen[3];
+  std::vector<SphHost> sph;
+  std::vector<double> pts;        // exposed points, concatenated
+  std::vector<double> consts;     // ns*10*p
+  long long npts_total = 0;
+  DBuf<double> d_pts, d_consts, d_Dfix, d_Vfix, d_E0, d_imats;
+  std::vector<const double *> imat_ptr;
+  DBuf<const double *> d_imat;
+  DBuf<int> d_pt_off, d_pt_cnt, d_nspx, d_all;
+  // self-polarization state
+  DBuf<double> d_F, d_H, d_LFS, d_LHS, d_dev, d_qF, d_qH, d_TQF, d_TQH, d_md;
+  // self-polarization jobs (near lists)
+  int n_xj = 0, n_pj = 0, n_slots = 0;
+  DBuf<XJob> d_xj;
+  DBuf<PJob> d_pj;
+  DBuf<TJob> d_tj;
+  DBuf<double> d_T, d_slotH, d_slotF;
+  DBuf<int> d_lst_off, d_lst;
+  // self values
+  bool have_self = false, have_values = false;
+  DBuf<double> d_Fself, d_Hself, d_qFself, d_qHself, d_TQFself, d_TQHself;
+  DBuf<double> d_LFintra, d_LHintra, d_LFfar, d_LHfar;
+  // far-pair translation matrices (values)
+  DBuf<double> d_Tfar;
+  // index of the analytical job of (target ki, near entry e), -1 for numerical pairs
+  std::vector<std::vector<int>> ana;
+};
+
+static void moltype_setup(pbsam_moltype *m, const pbsam_moltype_desc *d) {
+  pbsam_ctx *c = m->c;
+  hipStream_t s = c->stream;
+  const int ns = d->n_sph, p = d->p, P2 = p * p;
+  m->sph.resize(ns);
+  std::vector<double> rad2(ns);
+  for (int k = 0; k < ns; k++) rad2[k] = d->rad[k] * d->rad[k];
+  // neighbours, exposed points (generateMolSPX / getXFormSpherePoints, molecule.cpp:756-994)
+  for (int ki = 0; ki < ns; ki++) {
+    SphHost &S = m->sph[ki];
+    S.rad = d->rad[ki];
+    const double *ci = d->cen + 3 * ki;
+    for (int kj = 0; kj < ns; kj++) {
+      if (ki == kj) continue;
+      double sumRad = d->rad[ki] + d->rad[kj];
+      double dd[3] = {ci[0] - d->cen[3 * kj], ci[1] - d->cen[3 * kj + 1], ci[2] - d->cen[3 * kj + 2]};
+      if (dot3(dd, dd) < sumRad * sumRad) S.neigh.push_back(kj);
+    }
+    int N = d->npts ? d->npts[ki] : npts_rule(S.rad, d->npts_cap > 0 ? d->npts_cap : 10000);
+    if (N < 2) fail(PBSAM_ERR_BADARG, "sphere point count must be >= 2");
+    std::vector<double> xyz;
+    spiral(N, S.rad, xyz);
+    S.pt_off = (int)(m->pts.size() / 3);
+    for (int h = 0; h < N; h++) {
+      const double *q = &xyz[3 * h];
+      bool inside = false;
+      for (int j : S.neigh) {
+        const double *cj = d->cen + 3 * j;
+        double t[3] = {q[0] + ci[0], q[1] + ci[1], q[2] + ci[2]};
+        double u[3] = {t[0] - cj[0], t[1] - cj[1], t[2] - cj[2]};
+        if (dot3(u, u) < rad2[j]) { inside = true; break; }
+      }
+      if (!inside) for (int k = 0; k < 3; k++) m->pts.push_back(q[k]);
+    }
+    S.nspx = N;
+    S.nexp = (int)(m->pts.size() / 3) - S.pt_off;
+  }
+  m->npts_total = (long long)m->pts.size() / 3;
+  // intra near/far lists (generateMolTypeIntraPolLists molecule.cpp:501-535)
+  for (int kj = 0; kj < ns; kj++)
+    for (int ki = 0; ki < kj; ki++) {
+      const double *a = d->cen + 3 * ki, *b = d->cen + 3 * kj;
+      double P[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+      double rho = sqrt(dot3(P, P));
+      if (rho < (d->rad[ki] + d->rad[kj] + d->intra_rcutoff)) {
+        m->sph[ki].near.push_back(kj);
+        m->sph[kj].near.push_back(ki);
+      } else {
+        m->sph[ki].far.push_back(kj);
+        m->sph[kj].far.push_back(ki);
+      }
+    }
+  // molecule frame and charge assignment (assignCharges molecule.cpp:1002-1023)
+  std::vector<double> cpos(3 * d->n_chg);
+  for (int k = 0; k < ns; k++)
+    for (int x = 0; x < 3; x++) m->sph[k].cen[x] = d->cen[3 * k + x] - m->rcen[x];
+  for (int i = 0; i < d->n_chg; i++)
+    for (int x = 0; x < 3; x++) cpos[3 * i + x] = d->chg_xyz[3 * i + x] - m->rcen[x];
+  std::vector<int> clabel(d->n_chg, -1);
+  for (int i = 0; i < d->n_chg; i++) {
+    for (int ki = 0; ki < ns; ki++) {
+      const double *cc = m->sph[ki].cen;
+      double u[3] = {cpos[3 * i] - cc[0], cpos[3 * i + 1] - cc[1], cpos[3 * i + 2] - cc[2]};
+      if (sqrt(dot3(u, u)) + 0.0 <= d->rad[ki]) { clabel[i] = ki; break; }
+    }
+    if (clabel[i] < 0) fail(PBSAM_ERR_CHARGE, "charge " + std::to_string(i) + " is outside every sphere");
+  }
+  // constants
+  m->consts.resize((size_t)ns * 10 * p);
+  for (int k = 0; k < ns; k++)
+    sphere_consts(c->kappa, c->sdiel, m->sph[k].rad, d->eps_in, p, &m->consts[(size_t)k * 10 * p]);
+  // fixed-charge expansions on the device: E0 (assigned charges, multipole), Efix (inside,
+  // multipole), Lfix (outside, local)  -- setFixedCharges expcenter.cpp:89-103, 514-552
+  std::vector<double> qxyz, qval;
+  std::vector<PJob> jm, jl;
+  // job slots: [0,ns) E0, [ns,2ns) Efix; local: [0,ns) Lfix
+  for (int pass = 0; pass < 3; pass++)
+    for (int k = 0; k < ns; k++) {
+      const SphHost &S = m->sph[k];
+      PJob J{};
+      J.pt_off = (int)qval.size();
+      J.src = -1;
+      J.a_src = S.rad;
+      J.a_tgt = S.rad;
+      for (int i = 0; i < d->n_chg; i++) {
+        double u[3] = {cpos[3 * i] - S.cen[0], cpos[3 * i + 1] - S.cen[1], cpos[3 * i + 2] - S.cen[2]};
+        bool take;
+        if (pass == 0) take = (clabel[i] == k);
+        else if (pass == 1) take = (dot3(u, u) < S.rad * S.rad);
+        else take = !(dot3(u, u) < S.rad * S.rad);
+        if (!take) continue;
+        for (int x = 0; x < 3; x++) qxyz.push_back(u[x]);
+        qval.push_back(d->chg_q[i] / d->eps_in);
+      }
+      J.npts = (int)qval.size() - J.pt_off;
+      J.slot = (pass < 2) ? pass * ns + k : k;
+      (pass < 2 ? jm : jl).push_back(J);
+    }
+  DBuf<double> dqx, dqv, dmul, dloc;
+  DBuf<PJob> djm, djl;
+  dqx.upload(qxyz, s);
+  dqv.upload(qval, s);
+  djm.upload(jm, s);
+  djl.upload(jl, s);
+  dmul.alloc((size_t)2 * ns * P2);
+  dloc.alloc((size_t)ns * P2);
+  PArgs pa{};
+  pa.xyz = dqx.p;
+  pa.q1 = dqv.p;
+  pa.p = p;
+  pa.jobs = djm.p;
+  pa.out1 = dmul.p;
+  pa.multipole = 1;
+  launch_p2l(c, (int)jm.size(), pa);
+  pa.jobs = djl.p;
+  pa.out1 = dloc.p;
+  pa.multipole = 0;
+  launch_p2l(c, (int)jl.size(), pa);
+  std::vector<double> mul = dmul.download(s), loc = dloc.download(s);
+  // cal_DVFixed expcenter.cpp:1042-1060
+  std::vector<double> Dfix((size_t)ns * P2), Vfix((size_t)ns * P2), E0((size_t)ns * P2);
+  const double eps = d->eps_in / c->sdiel;
+  for (int k = 0; k < ns; k++) {
+    const double rad = m->sph[k].rad;
+    const double *Ef = &mul[(size_t)(ns + k) * P2], *Lf = &loc[(size_t)k * P2];
+    for (int n = 0, i = 0; n < p; n++) {
+      double DE = 1.0, DL = rad, VE = eps * (n + 1), VL = -eps * n * rad;
+      for (int mm = 0; mm < 2 * n + 1; mm++, i++) {
+        Dfix[(size_t)k * P2 + i] = DE * Ef[i] + DL * Lf[i];
+        Vfix[(size_t)k * P2 + i] = VE * Ef[i] + VL * Lf[i];
+      }
+    }
+    memcpy(&E0[(size_t)k * P2], &mul[(size_t)k * P2], sizeof(double) * P2);
+  }
+  m->d_Dfix.upload(Dfix, s);
+  m->d_Vfix.upload(Vfix, s);
+  m->d_E0.upload(E0, s);
+  m->d_consts.upload(m->consts, s);
+  m->d_pts.upload(m->pts, s);
+  std::vector<int> off(ns), cnt(ns), nspx(ns), all(ns);
+  for (int k = 0; k < ns; k++) {
+    off[k] = m->sph[k].pt_off;
+    cnt[k] = m->sph[k].nexp;
+    nspx[k] = m->sph[k].nspx;
+    all[k] = k;
+  }
+  m->d_pt_off.upload(off, s);
+  m->d_pt_cnt.upload(cnt, s);
+  m->d_nspx.upload(nspx, s);
+  m->d_all.upload(all, s);
+  // surface-integral matrices: supplied, or exact identity for spheres without neighbours
+  m->imat_ptr.assign(ns, nullptr);
+  if (d->imats) {
+    std::vector<double> im(d->imats, d->imats + (size_t)ns * P2 * P2);
+    m->d_imats.upload(im, s);
+    for (int k = 0; k < ns; k++) m->imat_ptr[k] = m->d_imats.p + (size_t)k * P2 * P2;
+  } else {
+    for (int k = 0; k < ns; k++)
+      if (!m->sph[k].neigh.empty())
+        fail(PBSAM_ERR_UNSUPPORTED, "sphere " + std::to_string(k) +
+                                        " overlaps a neighbour: its surface-integral matrix must be supplied");
+  }
+  m->d_imat.upload(m->imat_ptr, s);
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+// near-list jobs of a molecule type (reexpandIntra_Near_LowMemory molecule.cpp:1414-1473)
+static void moltype_jobs(pbsam_moltype *m) {
+  pbsam_ctx *c = m->c;
+  hipStream_t s = c->stream;
+  const int ns = m->ns, p = m->p, P2 = p * p;
+  std::vector<XJob> xj;
+  std::vector<PJob> pj;
+  std::vector<TJob> tj;
+  std::vector<int> lst_off(ns + 1, 0), lst;
+  int slot = 0;
+  const long long TS = tmat_size(p);
+  m->ana.assign(ns, {});
+  for (int ki = 0; ki < ns; ki++) {
+    const SphHost &Si = m->sph[ki];
+    for (int kj : Si.near) {
+      const SphHost &Sj = m->sph[kj];
+      double P[3] = {Si.cen[0] - Sj.cen[0], Si.cen[1] - Sj.cen[1], Si.cen[2] - Sj.cen[2]};
+      double rho = sqrt(dot3(P, P));
+      double sep = rho - Si.rad - Sj.rad;
+      m->ana[ki].push_back((sep > 0 && !(sep < 5.0)) ? (int)xj.size() : -1);
+      if (sep > 0 && !(sep < 5.0)) {
+        XJob J{};
+        J.src = kj;
+        J.tgt = ki;
+        J.slot = slot;
+        J.kind = 1;
+        for (int x = 0; x < 3; x++) J.P[x] = P[x];
+        J.a_src = Sj.rad;
+        J.a_tgt = Si.rad;
+        TJob T{};
+        for (int x = 0; x < 3; x++) T.P[x] = P[x];
+        T.a_src = Sj.rad;
+        T.a_tgt = Si.rad;
+        T.kappa_seed = c->kappa;
+        T.off = (long long)tj.size() * TS;
+        tj.push_back(T);
+        T.kappa_seed = 0.0;
+        T.off = (long long)tj.size() * TS;
+        tj.push_back(T);
+        xj.push_back(J);
+      } else {
+        PJob J{};
+        J.pt_off = Sj.pt_off;
+        J.npts = Sj.nexp;
+        J.slot = slot;
+        J.src = kj;
+        for (int x = 0; x < 3; x++) J.P[x] = P[x];
+        J.a_src = Sj.rad;
+        J.a_tgt = Si.rad;
+        pj.push_back(J);
+      }
+      lst.push_back(slot++);
+    }
+    lst_off[ki + 1] = (int)lst.size();
+  }
+  m->n_xj = (int)xj.size();
+  m->n_pj = (int)pj.size();
+  m->n_slots = slot;
+  m->d_T.alloc((size_t)tj.size() * TS);
+  for (size_t i = 0; i < xj.size(); i++) {
+    xj[i].TH = m->d_T.p + (size_t)(2 * i) * TS;
+    xj[i].TF = m->d_T.p + (size_t)(2 * i + 1) * TS;
+  }
+  m->d_xj.upload(xj, s);
+  m->d_pj.upload(pj, s);
+  m->d_tj.upload(tj, s);
+  std::vector<int> lst_safe = lst;
+  if (lst_safe.empty()) lst_safe.push_back(0);
+  m->d_lst.upload(lst_safe, s);
+  m->d_lst_off.upload(lst_off, s);
+  m->d_slotH.alloc((size_t)std::max(slot, 1) * P2);
+  m->d_slotF.alloc((size_t)std::max(slot, 1) * P2);
+  launch_trans(c, (int)tj.size(), m->d_tj, p, m->d_T.p);
+  // state: H = rE, F = 0, charges from them (initSurfaceCharges expcenter.cpp:577-587)
+  m->d_F.alloc((size_t)ns * P2);
+  m->d_F.zero(s);
+  m->d_H.alloc((size_t)ns * P2);
+  HIPCHK(hipMemcpyAsync(m->d_H.p, m->d_E0.p, sizeof(double) * ns * P2, hipMemcpyDeviceToDevice, s));
+  m->d_LFS.alloc((size_t)ns * P2);
+  m->d_LHS.alloc((size_t)ns * P2);
+  m->d_LFS.zero(s);
+  m->d_LHS.zero(s);
+  m->d_dev.alloc(ns);
+  m->d_dev.zero(s);
+  m->d_qF.alloc(std::max<long long>(m->npts_total, 1));
+  m->d_qH.alloc(std::max<long long>(m->npts_total, 1));
+  m->d_TQF.alloc(ns);
+  m->d_TQH.alloc(ns);
+  m->d_md.alloc(2);
+  QArgs qa{m->d_all.p, m->d_pts.p, m->d_pt_off.p, m->d_pt_cnt.p, m->d_nspx.p, m->d_consts.p,
+           m->d_F.p, m->d_H.p, m->d_qF.p, m->d_qH.p, m->d_TQF.p, m->d_TQH.p, p};
+  launch_surface(c, ns, qa);
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+// one Jacobi self-polarization round (recompute_LowMemory(ki) for every sphere)
+static double moltype_round(pbsam_moltype *m) {
+  pbsam_ctx *c = m->c;
+  const int ns = m->ns, p = m->p;
+  XArgs xa{};
+  xa.jobs = m->d_xj.p;
+  xa.H = m->d_H.p;
+  xa.F = m->d_F.p;
+  xa.TQH = m->d_TQH.p;
+  xa.TQF = m->d_TQF.p;
+  xa.slotH = m->d_slotH.p;
+  xa.slotF = m->d_slotF.p;
+  xa.p = p;
+  launch_xform(c, m->n_xj, xa);
+  PArgs pa{};
+  pa.jobs = m->d_pj.p;
+  pa.xyz = m->d_pts.p;
+  pa.q1 = m->d_qF.p;
+  pa.q2 = m->d_qH.p;
+  pa.TQ1 = m->d_TQF.p;
+  pa.TQ2 = m->d_TQH.p;
+  pa.out1 = m->d_slotF.p;
+  pa.out2 = m->d_slotH.p;
+  pa.kap1 = 0;
+  pa.kap2 = 1;
+  pa.multipole = 0;
+  pa.p = p;
+  launch_p2l(c, m->n_pj, pa);
+  SArgs sa{};
+  sa.sph = m->d_all.p;
+  sa.lst_off = m->d_lst_off.p;
+  sa.lst = m->d_lst.p;
+  sa.slotH = m->d_slotH.p;
+  sa.slotF = m->d_slotF.p;
+  sa.consts = m->d_consts.p;
+  sa.Dfix = m->d_Dfix.p;
+  sa.Vfix = m->d_Vfix.p;
+  sa.imat = m->d_imat.p;
+  sa.F = m->d_F.p;
+  sa.H = m->d_H.p;
+  sa.LFS = m->d_LFS.p;
+  sa.LHS = m->d_LHS.p;
+  sa.dev = m->d_dev.p;
+  sa.p = p;
+  launch_solve(c, ns, sa);
+  QArgs qa{m->d_all.p, m->d_pts.p, m->d_pt_off.p, m->d_pt_cnt.p, m->d_nspx.p, m->d_consts.p,
+           m->d_F.p, m->d_H.p, m->d_qF.p, m->d_qH.p, m->d_TQF.p, m->d_TQH.p, p};
+  launch_surface(c, ns, qa);
+  hipLaunchKernelGGL(k_maxdev, dim3(1), dim3(256), 0, c->stream, m->d_all.p, ns, m->d_dev.p, m->d_md.p);
+  HIPCHK(hipGetLastError());
+  std::vector<double> md = m->d_md.download(c->stream);
+  return md[0];
+}
+
+static void dcopy(pbsam_ctx *c, DBuf<double> &dst, const DBuf<double> &src) {
+  dst.alloc(src.n);
+  if (src.n) HIPCHK(hipMemcpyAsync(dst.p, src.p, sizeof(double) * src.n, hipMemcpyDeviceToDevice, c->stream));
+}
+
+static void moltype_store_self(pbsam_moltype *m) {
+  dcopy(m->c, m->d_Fself, m->d_F);
+  dcopy(m->c, m->d_Hself, m->d_H);
+  m->have_self = true;
+  m->have_values = false;
+}
+
+// generateMolExposedChargesFH + computeMolTypeValues (molecule.cpp:545-810)
+static void moltype_values(pbsam_moltype *m) {
+  pbsam_ctx *c = m->c;
+  hipStream_t s = c->stream;
+  const int ns = m->ns, p = m->p, P2 = p * p;
+  if (!m->have_self) fail(PBSAM_ERR_STATE, "no self-polarized expansions");
+  m->d_qFself.alloc(std::max<long long>(m->npts_total, 1));
+  m->d_qHself.alloc(std::max<long long>(m->npts_total, 1));
+  m->d_TQFself.alloc(ns);
+  m->d_TQHself.alloc(ns);
+  QArgs qa{m->d_all.p, m->d_pts.p, m->d_pt_off.p, m->d_pt_cnt.p, m->d_nspx.p, m->d_consts.p,
+           m->d_Fself.p, m->d_Hself.p, m->d_qFself.p, m->d_qHself.p, m->d_TQFself.p, m->d_TQHself.p, p};
+  launch_surface(c, ns, qa);
+  // far pairs: analytical with full orders
+  std::vector<XJob> fj;
+  std::vector<TJob> tj;
+  std::vector<int> foff(ns + 1, 0), flst;
+  const long long TS = tmat_size(p);
+  for (int ki = 0; ki < ns; ki++) {
+    const SphHost &Si = m->sph[ki];
+    for (int kj : Si.far) {
+      const SphHost &Sj = m->sph[kj];
+      XJob J{};
+      J.src = kj;
+      J.tgt = ki;
+      J.slot = (int)fj.size();
+      J.kind = 1;
+      for (int x = 0; x < 3; x++) J.P[x] = Si.cen[x] - Sj.cen[x];
+      J.a_src = Sj.rad;
+      J.a_tgt = Si.rad;
+      TJob T{};
+      for (int x = 0; x < 3; x++) T.P[x] = J.P[x];
+      T.a_src = Sj.rad;
+      T.a_tgt = Si.rad;
+      T.kappa_seed = c->kappa;
+      T.off = (long long)tj.size() * TS;
+      tj.push_back(T);
+      T.kappa_seed = 0.0;
+      T.off = (long long)tj.size() * TS;
+      tj.push_back(T);
+      flst.push_back(J.slot);
+      fj.push_back(J);
+    }
+    foff[ki + 1] = (int)flst.size();
+  }
+  m->d_Tfar.alloc((size_t)tj.size() * TS);
+  for (size_t i = 0; i < fj.size(); i++) {
+    fj[i].TH = m->d_Tfar.p + (size_t)(2 * i) * TS;
+    fj[i].TF = m->d_Tfar.p + (size_t)(2 * i + 1) * TS;
+  }
+  DBuf<XJob> dfj;
+  DBuf<TJob> dtj;
+  DBuf<int> dfoff, dflst;
+  DBuf<double> fsH, fsF;
+  dfj.upload(fj, s);
+  dtj.upload(tj, s);
+  dfoff.upload(foff, s);
+  if (flst.empty()) flst.push_back(0);
+  dflst.upload(flst, s);
+  fsH.alloc((size_t)std::max<size_t>(fj.size(), 1) * P2);
+  fsF.alloc((size_t)std::max<size_t>(fj.size(), 1) * P2);
+  launch_trans(c, (int)tj.size(), dtj, p, m->d_Tfar.p);
+  XArgs xa{};
+  xa.jobs = dfj.p;
+  xa.H = m->d_Hself.p;
+  xa.F = m->d_Fself.p;
+  xa.TQH = m->d_TQHself.p;
+  xa.TQF = m->d_TQFself.p;
+  xa.slotH = fsH.p;
+  xa.slotF = fsF.p;
+  xa.p = p;
+  xa.full = 1;
+  launch_xform(c, (int)fj.size(), xa);
+  m->d_LFfar.alloc((size_t)ns * P2);
+  m->d_LHfar.alloc((size_t)ns * P2);
+  LArgs la{m->d_all.p, dfoff.p, dflst.p, fsH.p, nullptr, m->d_LHfar.p, p};
+  launch_sumslots(c, ns, la);
+  la.slot = fsF.p;
+  la.out = m->d_LFfar.p;
+  launch_sumslots(c, ns, la);
+  // near pairs: analytical with full orders, numerical with self totals
+  xa.jobs = m->d_xj.p;
+  xa.slotH = m->d_slotH.p;
+  xa.slotF = m->d_slotF.p;
+  launch_xform(c, m->n_xj, xa);
+  PArgs pa{};
+  pa.jobs = m->d_pj.p;
+  pa.xyz = m->d_pts.p;
+  pa.q1 = m->d_qFself.p;
+  pa.q2 = m->d_qHself.p;
+  pa.TQ1 = m->d_TQFself.p;
+  pa.TQ2 = m->d_TQHself.p;
+  pa.out1 = m->d_slotF.p;
+  pa.out2 = m->d_slotH.p;
+  pa.kap1 = 0;
+  pa.kap2 = 1;
+  pa.p = p;
+  launch_p2l(c, m->n_pj, pa);
+  m->d_LFintra.alloc((size_t)ns * P2);
+  m->d_LHintra.alloc((size_t)ns * P2);
+  LArgs lb{m->d_all.p, m->d_lst_off.p, m->d_lst.p, m->d_slotH.p, m->d_LHfar.p, m->d_LHintra.p, p};
+  launch_sumslots(c, ns, lb);
+  lb.slot = m->d_slotF.p;
+  lb.base = m->d_LFfar.p;
+  lb.out = m->d_LFintra.p;
+  launch_sumslots(c, ns, lb);
+  HIPCHK(hipStreamSynchronize(s));
+  m->have_values = true;
+}
+
+// ----------------------------------------------------------------------------
+// systems
+// ----------------------------------------------------------------------------
+struct SphRef {
+  int i, k;
+};
+struct MolInst {
+  pbsam_moltype *mt;
+  double rcen[3], q[4];
+  std::vector<double> cenRot;
+  int base;       // global sphere index of sphere 0
+  int pbase;      // global point index of this molecule's points
+  double maxR;
+  bool bInterXForm = false;
+  std::vector<std::vector<SphRef>> interPol, interact;
+};
+
+struct pbsam_system {
+  pbsam_ctx *c;
+  int p, nmol, nsph = 0;
+  long long npts = 0;
+  std::vector<MolInst> mols;
+  double interRc = 1e300, interactRc = 1e300;
+  // static per-sphere data (copied from the molecule types)
+  DBuf<double> d_pts, d_consts, d_Dfix, d_Vfix;
+  DBuf<const double *> d_imat;
+  DBuf<int> d_pt_off, d_pt_cnt, d_nspx;
+  DBuf<double> d_Fself, d_Hself, d_qFself, d_qHself, d_TQFself, d_TQHself;
+  DBuf<double> d_LFintra, d_LHintra, d_LFfar, d_LHfar;
+  // state
+  DBuf<double> d_F, d_H, d_Hrot, d_HselfRot, d_LS, d_LSrot, d_LFS, d_LHS, d_dev, d_qF, d_qH, d_TQF, d_TQH,
+      d_md;
+  // jobs
+  std::vector<int> pol;  // polarized spheres (global index)
+  DBuf<int> d_pol, d_inter_off, d_inter_lst, d_intra_off, d_intra_lst, d_ones_off;
+  int n_inter = 0, n_xintra = 0, n_pintra = 0, n_intra_slots = 0;
+  DBuf<XJob> d_xinter, d_xintra;
+  DBuf<PJob> d_pintra;
+  DBuf<TJob> d_tinter;
+  DBuf<double> d_Tinter, d_slotI, d_slotH, d_slotF;
+  // molecule rotations
+  DBuf<RJob> d_rj;
+  DBuf<double> d_Rst;
+  DBuf<RAJob> d_ra_self, d_ra_ls, d_ra_h;
+  int n_ra_self = 0;
+  bool lists_ok = false;
+};
+
+static void system_static(pbsam_system *S) {
+  pbsam_ctx *c = S->c;
+  hipStream_t s = c->stream;
+  const int p = S->p, P2 = p * p;
+  int g = 0;
+  long long pt = 0;
+  for (MolInst &M : S->mols) {
+    M.base = g;
+    M.pbase = (int)pt;
+    g += M.mt->ns;
+    pt += M.mt->npts_total;
+  }
+  S->nsph = g;
+  S->npts = pt;
+  auto cat = [&](DBuf<double> &dst, size_t per, auto getter) {
+    dst.alloc(std::max<size_t>(per * (size_t)g, 1));
+    size_t o = 0;
+    for (MolInst &M : S->mols) {
+      const DBuf<double> &src = getter(M.mt);
+      size_t n = per * (size_t)M.mt->ns;
+      if (n) HIPCHK(hipMemcpyAsync(dst.p + o, src.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      o += n;
+    }
+  };
+  cat(S->d_consts, (size_t)10 * p, [](pbsam_moltype *m) -> const DBuf<double> & { return m->d_consts; });
+  cat(S->d_Dfix, P2, [](pbsam_moltype *m) -> const DBuf<double> & { return m->d_Dfix; });
+  cat(S->d_Vfix, P2, [](pbsam_moltype *m) -> const DBuf<double> & { return m->d_Vfix; });
+  cat(S->d_Fself, P2, [](pbsam_moltype *m) -> const DBuf<double> & { return m->d_Fself; });
+  cat(S->d_Hself, P2, [](pbsam_moltype *m) -> const DBuf<double> & { return m->d_Hself; });
+  cat(S->d_LFintra, P2, [](pbsam_moltype *m) -> const DBuf<double> & { return m->d_LFintra; });
+  cat(S->d_LHintra, P2, [](pbsam_moltype *m) -> const DBuf<double> & { return m->d_LHintra; });
+  cat(S->d_LFfar, P2, [](pbsam_moltype *m) -> const DBuf<double> & { return m->d_LFfar; });
+  cat(S->d_LHfar, P2, [](pbsam_moltype *m) -> const DBuf<double> & { return m->d_LHfar; });
+  cat(S->d_TQFself, 1, [](pbsam_moltype *m) -> const DBuf<double> & { return m->d_TQFself; });
+  cat(S->d_TQHself, 1, [](pbsam_moltype *m) -> const DBuf<double> & { return m->d_TQHself; });
+  // points and per-point self charges
+  S->d_pts.alloc(std::max<long long>(3 * pt, 1));
+  S->d_qFself.alloc(std::max<long long>(pt, 1));
+  S->d_qHself.alloc(std::max<long long>(pt, 1));
+  std::vector<int> off(g), cnt(g), nspx(g);
+  std::vector<const double *> im(g);
+  for (MolInst &M : S->mols) {
+    pbsam_moltype *m = M.mt;
+    long long n = m->npts_total;
+    if (n) {
+      HIPCHK(hipMemcpyAsync(S->d_pts.p + 3 * (size_t)M.pbase, m->d_pts.p, sizeof(double) * 3 * n,
+                            hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipMemcpyAsync(S->d_qFself.p + M.pbase, m->d_qFself.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipMemcpyAsync(S->d_qHself.p + M.pbase, m->d_qHself.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    }
+    for (int k = 0; k < m->ns; k++) {
+      off[M.base + k] = M.pbase + m->sph[k].pt_off;
+      cnt[M.base + k] = m->sph[k].nexp;
+      nspx[M.base + k] = m->sph[k].nspx;
+      im[M.base + k] = m->imat_ptr[k];
+    }
+  }
+  S->d_pt_off.upload(off, s);
+  S->d_pt_cnt.upload(cnt, s);
+  S->d_nspx.upload(nspx, s);
+  S->d_imat.upload(im, s);
+  // state buffers
+  for (DBuf<double> *b : {&S->d_F, &S->d_H, &S->d_Hrot, &S->d_HselfRot, &S->d_LS, &S->d_LSrot, &S->d_LFS,
+                          &S->d_LHS})
+    b->alloc((size_t)g * P2);
+  S->d_dev.alloc(g);
+  S->d_qF.alloc(std::max<long long>(pt, 1));
+  S->d_qH.alloc(std::max<long long>(pt, 1));
+  S->d_TQF.alloc(g);
+  S->d_TQH.alloc(g);
+  S->d_md.alloc(2);
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+static void mol_pose(MolInst &M, const double *pos, const double *quat) {
+  for (int x = 0; x < 3; x++) M.rcen[x] = pos[x];
+  double q[4] = {quat[0], quat[1], quat[2], quat[3]};
+  quat_normalize(q);
+  for (int x = 0; x < 4; x++) M.q[x] = q[x];
+  M.cenRot.assign(3 * M.mt->ns, 0.0);
+  for (int k = 0; k < M.mt->ns; k++) quat_rot(M.q, M.mt->sph[k].cen, &M.cenRot[3 * k]);
+}
+
+// reset to the BD-constructor state (self values) -- expcenter.cpp:398-425
+static void system_reset_state(pbsam_system *S) {
+  pbsam_ctx *c = S->c;
+  hipStream_t s = c->stream;
+  dcopy(c, S->d_F, S->d_Fself);
+  dcopy(c, S->d_H, S->d_Hself);
+  dcopy(c, S->d_qF, S->d_qFself);
+  dcopy(c, S->d_qH, S->d_qHself);
+  dcopy(c, S->d_TQF, S->d_TQFself);
+  dcopy(c, S->d_TQH, S->d_TQHself);
+  dcopy(c, S->d_Hrot, S->d_Hself);
+  S->d_LS.zero(s);
+  S->d_LFS.zero(s);
+  S->d_LHS.zero(s);
+  S->d_dev.zero(s);
+}
+
+// generateInterXFormsForPolarize_LowMemory (molecule.cpp:1036-1126) and the job lists
+static int system_lists(pbsam_system *S) {
+  pbsam_ctx *c = S->c;
+  hipStream_t s = c->stream;
+  const int nmol = S->nmol, p = S->p, P2 = p * p;
+  system_reset_state(S);
+  for (MolInst &M : S->mols) {
+    M.bInterXForm = false;
+    M.interPol.assign(M.mt->ns, {});
+    M.interact.assign(M.mt->ns, {});
+  }
+  S->lists_ok = false;
+  for (int j = 1; j < nmol; j++) {
+    MolInst &Mj = S->mols[j];
+    std::vector<int> neigh;
+    for (int i = 0; i < j; i++) {
+      MolInst &Mi = S->mols[i];
+      double d[3] = {Mi.rcen[0] - Mj.rcen[0], Mi.rcen[1] - Mj.rcen[1], Mi.rcen[2] - Mj.rcen[2]};
+      if (sqrt(dot3(d, d)) - Mi.maxR - Mj.maxR < S->interactRc) neigh.push_back(i);
+    }
+    for (int kj = 0; kj < Mj.mt->ns; kj++) {
+      const double *mcj = &Mj.cenRot[3 * kj];
+      double rj = Mj.mt->sph[kj].rad;
+      for (int i : neigh) {
+        MolInst &Mi = S->mols[i];
+        for (int ki = 0; ki < Mi.mt->ns; ki++) {
+          const double *mci = &Mi.cenRot[3 * ki];
+          double ri = Mi.mt->sph[ki].rad;
+          double a[3] = {Mi.rcen[0] + mci[0], Mi.rcen[1] + mci[1], Mi.rcen[2] + mci[2]};
+          double b[3] = {Mj.rcen[0] + mcj[0], Mj.rcen[1] + mcj[1], Mj.rcen[2] + mcj[2]};
+          double P[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+          double sep = sqrt(dot3(P, P)) - ri - rj;
+          if (sep <= 0) return PBSAM_ERR_OVERLAP;
+          if (sep < S->interactRc) {
+            if (sep > S->interRc) {
+              Mi.interact[ki].push_back({j, kj});
+              Mj.interact[kj].push_back({i, ki});
+            } else {
+              Mi.interPol[ki].push_back({j, kj});
+              Mj.interPol[kj].push_back({i, ki});
+            }
+            Mi.bInterXForm = true;
+            Mj.bInterXForm = true;
+          }
+        }
+      }
+    }
+  }
+  // polarized spheres and their jobs
+  S->pol.clear();
+  std::vector<XJob> xi, xa;
+  std::vector<PJob> pa;
+  std::vector<TJob> ti;
+  std::vector<int> ioff(1, 0), ilst, aoff(1, 0), alst;
+  const long long TS = tmat_size(p);
+  int islot = 0, aslot = 0;
+  for (int i = 0; i < nmol; i++) {
+    MolInst &Mi = S->mols[i];
+    pbsam_moltype *mt = Mi.mt;
+    if (!Mi.bInterXForm) continue;
+    for (int ki = 0; ki < mt->ns; ki++) {
+      if (Mi.interPol[ki].empty()) continue;
+      const int g = Mi.base + ki;
+      S->pol.push_back(g);
+      const double *mci = &Mi.cenRot[3 * ki];
+      double ci[3] = {Mi.rcen[0] + mci[0], Mi.rcen[1] + mci[1], Mi.rcen[2] + mci[2]};
+      for (const SphRef &e : Mi.interPol[ki]) {
+        MolInst &Mj = S->mols[e.i];
+        const double *mcj = &Mj.cenRot[3 * e.k];
+        double cj[3] = {Mj.rcen[0] + mcj[0], Mj.rcen[1] + mcj[1], Mj.rcen[2] + mcj[2]};
+        XJob J{};
+        J.src = Mj.base + e.k;
+        J.tgt = g;
+        J.slot = islot;
+        J.kind = 0;
+        for (int x = 0; x < 3; x++) J.P[x] = ci[x] - cj[x];
+        J.a_src = Mj.mt->sph[e.k].rad;
+        J.a_tgt = mt->sph[ki].rad;
+        TJob T{};
+        for (int x = 0; x < 3; x++) T.P[x] = J.P[x];
+        T.a_src = J.a_src;
+        T.a_tgt = J.a_tgt;
+        T.kappa_seed = c->kappa;
+        T.off = (long long)ti.size() * TS;
+        ti.push_back(T);
+        xi.push_back(J);
+        ilst.push_back(islot++);
+      }
+      ioff.push_back((int)ilst.size());
+      // intra near pairs of this sphere, sharing the molecule type's translation matrices
+      const SphHost &Si = mt->sph[ki];
+      for (size_t e = 0; e < Si.near.size(); e++) {
+        const int kj = Si.near[e];
+        const SphHost &Sj = mt->sph[kj];
+        double P[3] = {Si.cen[0] - Sj.cen[0], Si.cen[1] - Sj.cen[1], Si.cen[2] - Sj.cen[2]};
+        double rho = sqrt(dot3(P, P));
+        double sep = rho - Si.rad - Sj.rad;
+        if (sep > 0 && !(sep < 5.0)) {
+          XJob J{};
+          J.src = Mi.base + kj;
+          J.tgt = g;
+          J.slot = aslot;
+          J.kind = 1;
+          for (int x = 0; x < 3; x++) J.P[x] = P[x];
+          J.a_src = Sj.rad;
+          J.a_tgt = Si.rad;
+          const int idx = mt->ana[ki][e];  // shares the molecule type's translation matrices
+          J.TH = mt->d_T.p + (size_t)(2 * idx) * TS;
+          J.TF = mt->d_T.p + (size_t)(2 * idx + 1) * TS;
+          xa.push_back(J);
+        } else {
+          PJob J{};
+          J.pt_off = Mi.pbase + Sj.pt_off;
+          J.npts = Sj.nexp;
+          J.slot = aslot;
+          J.src = Mi.base + kj;
+          for (int x = 0; x < 3; x++) J.P[x] = P[x];
+          J.a_src = Sj.rad;
+          J.a_tgt = Si.rad;
+          pa.push_back(J);
+        }
+        alst.push_back(aslot++);
+      }
+      aoff.push_back((int)alst.size());
+    }
+  }
+  S->n_inter = (int)xi.size();
+  S->n_xintra = (int)xa.size();
+  S->n_pintra = (int)pa.size();
+  S->n_intra_slots = aslot;
+  S->d_Tinter.alloc(std::max<size_t>(ti.size() * TS, 1));
+  for (size_t k = 0; k < xi.size(); k++) xi[k].TH = S->d_Tinter.p + (size_t)k * TS;
+  S->d_xinter.upload(xi, s);
+  S->d_xintra.upload(xa, s);
+  S->d_pintra.upload(pa, s);
+  S->d_tinter.upload(ti, s);
+  auto up = [&](DBuf<int> &d, std::vector<int> v) {
+    if (v.empty()) v.push_back(0);
+    d.upload(v, s);
+  };
+  up(S->d_pol, S->pol);
+  up(S->d_inter_off, ioff);
+  up(S->d_inter_lst, ilst);
+  up(S->d_intra_off, aoff);
+  up(S->d_intra_lst, alst);
+  std::vector<int> zoff(S->pol.size() + 1, 0);
+  up(S->d_ones_off, zoff);
+  S->d_slotI.alloc((size_t)std::max(islot, 1) * P2);
+  S->d_slotH.alloc((size_t)std::max(aslot, 1) * P2);
+  S->d_slotF.alloc((size_t)std::max(aslot, 1) * P2);
+  launch_trans(c, (int)ti.size(), S->d_tinter, p, S->d_Tinter.p);
+  // molecule rotations (rotateRotCoeff moldynamics.cpp:682-685)
+  std::vector<RJob> rj(nmol);
+  const long long RS = 2 * rot_off(p);
+  for (int i = 0; i < nmol; i++) {
+    quat_euler(S->mols[i].q, rj[i].theta, rj[i].phi, rj[i].xi);
+    rj[i].off = (long long)i * RS;
+    rj[i].sing = 0;
+  }
+  S->d_rj.upload(rj, s);
+  S->d_Rst.alloc((size_t)nmol * RS);
+  launch_rotcoeff(c, nmol, S->d_rj.p, p, S->d_Rst.p);
+  // rotation jobs: Hself -> HselfRot for spheres with any interaction; LS and H for polarized
+  std::vector<RAJob> rs, rl, rh;
+  for (int i = 0; i < nmol; i++) {
+    MolInst &M = S->mols[i];
+    if (!M.bInterXForm) continue;
+    for (int k = 0; k < M.mt->ns; k++)
+      if (!(M.interPol[k].empty() && M.interact[k].empty())) rs.push_back({M.base + k, M.base + k, i, 1});
+  }
+  for (int g : S->pol) {
+    int mi = 0;
+    while (mi + 1 < nmol && S->mols[mi + 1].base <= g) mi++;
+    rl.push_back({g, g, mi, 0});
+    rh.push_back({g, g, mi, 1});
+  }
+  S->n_ra_self = (int)rs.size();
+  S->d_ra_self.upload(rs, s);
+  S->d_ra_ls.upload(rl, s);
+  S->d_ra_h.upload(rh, s);
+  HIPCHK(hipStreamSynchronize(s));
+  S->lists_ok = true;
+  return PBSAM_OK;
+}
+
+// one mutual round: round1 uses self values (recomputeFromSelfVal_LowMemory)
+static double system_round(pbsam_system *S, bool round1, bool refresh) {
+  pbsam_ctx *c = S->c;
+  const int p = S->p, np = (int)S->pol.size();
+  // inter-molecular re-expansions and LS = R^T sum
+  XArgs xa{};
+  xa.jobs = S->d_xinter.p;
+  xa.H = round1 ? S->d_HselfRot.p : S->d_Hrot.p;
+  xa.TQH = round1 ? S->d_TQHself.p : S->d_TQH.p;
+  xa.slotH = S->d_slotI.p;
+  xa.p = p;
+  launch_xform(c, S->n_inter, xa);
+  LArgs la{S->d_pol.p, S->d_inter_off.p, S->d_inter_lst.p, S->d_slotI.p, nullptr, S->d_LSrot.p, p};
+  launch_sumslots(c, np, la);
+  launch_rotapply(c, np, S->d_ra_ls.p, S->d_rj.p, S->d_Rst.p, S->d_LSrot.p, S->d_LS.p, p);
+  SArgs sa{};
+  sa.sph = S->d_pol.p;
+  sa.slotH = S->d_slotH.p;
+  sa.slotF = S->d_slotF.p;
+  sa.LS = S->d_LS.p;
+  sa.consts = S->d_consts.p;
+  sa.Dfix = S->d_Dfix.p;
+  sa.Vfix = S->d_Vfix.p;
+  sa.imat = S->d_imat.p;
+  sa.F = S->d_F.p;
+  sa.H = S->d_H.p;
+  sa.LFS = S->d_LFS.p;
+  sa.LHS = S->d_LHS.p;
+  sa.dev = S->d_dev.p;
+  sa.p = p;
+  if (round1) {
+    sa.lst_off = S->d_ones_off.p;
+    sa.lst = S->d_intra_lst.p;
+    sa.farH = S->d_LHintra.p;
+    sa.farF = S->d_LFintra.p;
+  } else {
+    XArgs xb{};
+    xb.jobs = S->d_xintra.p;
+    xb.H = S->d_H.p;
+    xb.F = S->d_F.p;
+    xb.TQH = S->d_TQH.p;
+    xb.TQF = S->d_TQF.p;
+    xb.slotH = S->d_slotH.p;
+    xb.slotF = S->d_slotF.p;
+    xb.p = p;
+    launch_xform(c, S->n_xintra, xb);
+    PArgs pa{};
+    pa.jobs = S->d_pintra.p;
+    pa.xyz = S->d_pts.p;
+    pa.q1 = S->d_qF.p;
+    pa.q2 = S->d_qH.p;
+    pa.TQ1 = S->d_TQF.p;
+    pa.TQ2 = S->d_TQH.p;
+    pa.out1 = S->d_slotF.p;
+    pa.out2 = S->d_slotH.p;
+    pa.kap1 = 0;
+    pa.kap2 = 1;
+    pa.p = p;
+    launch_p2l(c, S->n_pintra, pa);
+    sa.lst_off = S->d_intra_off.p;
+    sa.lst = S->d_intra_lst.p;
+    sa.farH = S->d_LHfar.p;
+    sa.farF = S->d_LFfar.p;
+  }
+  launch_solve(c, np, sa);
+  if (round1 || refresh) launch_rotapply(c, np, S->d_ra_h.p, S->d_rj.p, S->d_Rst.p, S->d_H.p, S->d_Hrot.p, p);
+  QArgs qa{S->d_pol.p, S->d_pts.p, S->d_pt_off.p, S->d_pt_cnt.p, S->d_nspx.p, S->d_consts.p,
+           S->d_F.p, S->d_H.p, S->d_qF.p, S->d_qH.p, S->d_TQF.p, S->d_TQH.p, p};
+  launch_surface(c, np, qa);
+  hipLaunchKernelGGL(k_maxdev, dim3(1), dim3(256), 0, c->stream, S->d_pol.p, np, S->d_dev.p, S->d_md.p);
+  HIPCHK(hipGetLastError());
+  std::vector<double> md = S->d_md.download(c->stream);
+  return md[0];
+}
+
+// phase 0 of polarize_mutual: molecule rotations and HselfRot
+static void system_phase0(pbsam_system *S) {
+  launch_rotapply(S->c, S->n_ra_self, S->d_ra_self.p, S->d_rj.p, S->d_Rst.p, S->d_Hself.p, S->d_HselfRot.p, S->p);
+  dcopy(S->c, S->d_Hrot, S->d_HselfRot);
+}
+
+// ----------------------------------------------------------------------------
+// C ABI
+// ----------------------------------------------------------------------------
+template <class F>
+static int guarded(std::string *err, F f) {
+  std::string *old = g_err;
+  g_err = err;
+  int rc = PBSAM_OK;
+  try {
+    rc = f();
+  } catch (const Err &e) {
+    rc = e.code;
+  } catch (const std::bad_alloc &) {
+    if (err) *err = "host allocation failed";
+    rc = PBSAM_ERR_OOM;
+  }
+  g_err = old;
+  return rc;
+}
+
+extern "C" {
+
+double pbsam_kappa_from_salt(double salt_molar, double eps_out, double temp_k) {
+  // selfp_makemat.cpp:14-21, 59-60
+  const double AVOGADRO_NUM = 6.0221415e23, ELECT_CHG = 1.60217646e-19, PERMITTIVITY_VAC = 8.854187817e-12,
+               KB = 1.3806503e-23, ANGSTROM = 1e-10, LITRE = 1e-3;
+  return ANGSTROM * sqrt((2 * salt_molar * AVOGADRO_NUM / LITRE * ELECT_CHG * ELECT_CHG) /
+                         (eps_out * PERMITTIVITY_VAC * KB * temp_k));
+}
+
+int pbsam_ctx_create(double kappa, double eps_out, int device, pbsam_ctx **out) {
+  if (!out || kappa < 0 || eps_out <= 0) return PBSAM_ERR_BADARG;
+  *out = nullptr;
+  std::unique_ptr<pbsam_ctx> c(new pbsam_ctx());
+  c->kappa = kappa;
+  c->sdiel = eps_out;
+  c->device = device;
+  std::string err;
+  int rc = guarded(&err, [&]() {
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) fail(PBSAM_ERR_BADARG, "no such HIP device");
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    int lds = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    c->lds_max = (size_t)lds;
+    build_tables(c.get());
+    return PBSAM_OK;
+  });
+  if (rc != PBSAM_OK) {
+    fprintf(stderr, "pbsam_ctx_create: %s\n", err.c_str());
+    return rc;
+  }
+  *out = c.release();
+  return PBSAM_OK;
+}
+
+void pbsam_ctx_destroy(pbsam_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *pbsam_last_error(const pbsam_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int pbsam_moltype_create(pbsam_ctx *c, const pbsam_moltype_desc *d, pbsam_moltype **out) {
+  if (!c || !d || !out) return PBSAM_ERR_BADARG;
+  *out = nullptr;
+  if (d->n_sph <= 0 || !d->cen || !d->rad || d->p < 1 || d->p > PBSAM_PMAX || d->eps_in <= 0 ||
+      (d->n_chg > 0 && (!d->chg_xyz || !d->chg_q))) {
+    c->err = "invalid molecule description";
+    return PBSAM_ERR_BADARG;
+  }
+  std::unique_ptr<pbsam_moltype> m(new pbsam_moltype());
+  m->c = c;
+  m->p = d->p;
+  m->ns = d->n_sph;
+  m->eps_in = d->eps_in;
+  m->intra_rc = d->intra_rcutoff;
+  for (int x = 0; x < 3; x++) m->rcen[x] = d->rcen ? d->rcen[x] : 0.0;
+  int rc = guarded(&c->err, [&]() {
+    HIPCHK(hipSetDevice(c->device));
+    moltype_setup(m.get(), d);
+    moltype_jobs(m.get());
+    return PBSAM_OK;
+  });
+  if (rc == PBSAM_OK) *out = m.release();
+  return rc;
+}
+
+void pbsam_moltype_destroy(pbsam_moltype *m) { delete m; }
+
+int pbsam_moltype_info(const pbsam_moltype *m, int *n_sph, int *p, long long *n_points) {
+  if (!m) return PBSAM_ERR_BADARG;
+  if (n_sph) *n_sph = m->ns;
+  if (p) *p = m->p;
+  if (n_points) *n_points = m->npts_total;
+  return PBSAM_OK;
+}
+
+int pbsam_moltype_points(const pbsam_moltype *m, int k, double *xyz, int *n_exposed, int *n_spiral) {
+  if (!m || k < 0 || k >= m->ns) return PBSAM_ERR_BADARG;
+  const SphHost &S = m->sph[k];
+  if (xyz) memcpy(xyz, &m->pts[3 * (size_t)S.pt_off], sizeof(double) * 3 * S.nexp);
+  if (n_exposed) *n_exposed = S.nexp;
+  if (n_spiral) *n_spiral = S.nspx;
+  return PBSAM_OK;
+}
+
+int pbsam_selfpol_rounds(pbsam_moltype *m, int rounds, double *maxdev_out) {
+  if (!m || rounds < 0) return PBSAM_ERR_BADARG;
+  return guarded(&m->c->err, [&]() {
+    double md = 0;
+    for (int r = 0; r < rounds; r++) md = moltype_round(m);
+    if (maxdev_out) *maxdev_out = md;
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_selfpol(pbsam_moltype *m, double tol, int max_rounds, int *rounds_out) {
+  if (!m || max_rounds < 1) return PBSAM_ERR_BADARG;
+  return guarded(&m->c->err, [&]() {
+    double md = 1e300;
+    int r = 0;
+    while (r < max_rounds && md > tol) {
+      md = moltype_round(m);
+      r++;
+    }
+    if (rounds_out) *rounds_out = r;
+    moltype_store_self(m);
+    return md > tol ? PBSAM_ERR_NOCONV : PBSAM_OK;
+  });
+}
+
+int pbsam_moltype_set_self(pbsam_moltype *m, const double *F, const double *H) {
+  if (!m || !F || !H) return PBSAM_ERR_BADARG;
+  return guarded(&m->c->err, [&]() {
+    size_t n = (size_t)m->ns * m->p * m->p;
+    std::vector<double> f(F, F + n), h(H, H + n);
+    m->d_F.upload(f, m->c->stream);
+    m->d_H.upload(h, m->c->stream);
+    moltype_store_self(m);
+    HIPCHK(hipStreamSynchronize(m->c->stream));
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_moltype_values(pbsam_moltype *m) {
+  if (!m) return PBSAM_ERR_BADARG;
+  return guarded(&m->c->err, [&]() {
+    moltype_values(m);
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_moltype_get(const pbsam_moltype *m, int k, double *F, double *H, double *qF, double *qH, double *tq) {
+  if (!m || k < 0 || k >= m->ns) return PBSAM_ERR_BADARG;
+  return guarded(&m->c->err, [&]() {
+    hipStream_t s = m->c->stream;
+    const int P2 = m->p * m->p;
+    const SphHost &S = m->sph[k];
+    if (F) HIPCHK(hipMemcpyAsync(F, m->d_F.p + (size_t)k * P2, sizeof(double) * P2, hipMemcpyDeviceToHost, s));
+    if (H) HIPCHK(hipMemcpyAsync(H, m->d_H.p + (size_t)k * P2, sizeof(double) * P2, hipMemcpyDeviceToHost, s));
+    if (qF && S.nexp)
+      HIPCHK(hipMemcpyAsync(qF, m->d_qF.p + S.pt_off, sizeof(double) * S.nexp, hipMemcpyDeviceToHost, s));
+    if (qH && S.nexp)
+      HIPCHK(hipMemcpyAsync(qH, m->d_qH.p + S.pt_off, sizeof(double) * S.nexp, hipMemcpyDeviceToHost, s));
+    if (tq) {
+      HIPCHK(hipMemcpyAsync(tq, m->d_TQF.p + k, sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(tq + 1, m->d_TQH.p + k, sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(tq + 2, m->d_dev.p + k, sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_moltype_get_intra(const pbsam_moltype *m, int k, double *LF, double *LH, double *LFfar, double *LHfar) {
+  if (!m || k < 0 || k >= m->ns) return PBSAM_ERR_BADARG;
+  if (!m->have_values) return PBSAM_ERR_STATE;
+  return guarded(&m->c->err, [&]() {
+    hipStream_t s = m->c->stream;
+    const int P2 = m->p * m->p;
+    const DBuf<double> *src[4] = {&m->d_LFintra, &m->d_LHintra, &m->d_LFfar, &m->d_LHfar};
+    double *dst[4] = {LF, LH, LFfar, LHfar};
+    for (int i = 0; i < 4; i++)
+      if (dst[i])
+        HIPCHK(hipMemcpyAsync(dst[i], src[i]->p + (size_t)k * P2, sizeof(double) * P2, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_system_create(pbsam_ctx *c, int n_mol, pbsam_moltype *const *mts, const double *pos, const double *quat,
+                        pbsam_system **out) {
+  if (!c || n_mol < 1 || !mts || !pos || !quat || !out) return PBSAM_ERR_BADARG;
+  *out = nullptr;
+  std::unique_ptr<pbsam_system> S(new pbsam_system());
+  S->c = c;
+  S->nmol = n_mol;
+  S->p = mts[0]->p;
+  int rc = guarded(&c->err, [&]() {
+    HIPCHK(hipSetDevice(c->device));
+    S->mols.resize(n_mol);
+    for (int i = 0; i < n_mol; i++) {
+      if (!mts[i] || mts[i]->c != c) fail(PBSAM_ERR_BADARG, "molecule type from another context");
+      if (mts[i]->p != S->p) fail(PBSAM_ERR_BADARG, "all molecules of a system must share the pole order");
+      if (!mts[i]->have_self) fail(PBSAM_ERR_STATE, "molecule type has no self-polarized expansions");
+      if (!mts[i]->have_values) moltype_values(mts[i]);
+      MolInst &M = S->mols[i];
+      M.mt = mts[i];
+      mol_pose(M, pos + 3 * i, quat + 4 * i);
+      double maxR = 0;  // computeMaxRadius moldynamics.cpp:329-338
+      for (int k = 0; k < M.mt->ns; k++) {
+        double dist = sqrt(dot3(M.mt->sph[k].cen, M.mt->sph[k].cen)) + M.mt->sph[k].rad;
+        if (dist > maxR) maxR = dist;
+      }
+      M.maxR = maxR;
+    }
+    system_static(S.get());
+    return PBSAM_OK;
+  });
+  if (rc == PBSAM_OK) *out = S.release();
+  return rc;
+}
+
+void pbsam_system_destroy(pbsam_system *S) { delete S; }
+
+int pbsam_system_set_pose(pbsam_system *S, int i, const double *pos, const double *quat) {
+  if (!S || i < 0 || i >= S->nmol || !pos || !quat) return PBSAM_ERR_BADARG;
+  mol_pose(S->mols[i], pos, quat);
+  S->lists_ok = false;
+  return PBSAM_OK;
+}
+
+int pbsam_system_set_cutoffs(pbsam_system *S, double inter_rc, double interact_rc) {
+  if (!S) return PBSAM_ERR_BADARG;
+  S->interRc = inter_rc;
+  S->interactRc = interact_rc;
+  S->lists_ok = false;
+  return PBSAM_OK;
+}
+
+int pbsam_mutual_rounds(pbsam_system *S, int rounds, unsigned flags, double *maxdev_out) {
+  if (!S || rounds < 0) return PBSAM_ERR_BADARG;
+  return guarded(&S->c->err, [&]() {
+    int rc = system_lists(S);
+    if (rc != PBSAM_OK) return rc;
+    if (S->nmol == 1) return PBSAM_OK;
+    system_phase0(S);
+    double md = system_round(S, true, (flags & PBSAM_HROT_REFRESH) != 0);
+    for (int r = 0; r < rounds; r++) md = system_round(S, false, (flags & PBSAM_HROT_REFRESH) != 0);
+    if (maxdev_out) *maxdev_out = md;
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_solve_mutual(pbsam_system *S, double tol, int max_rounds, unsigned flags, int *rounds_out) {
+  if (!S || max_rounds < 1) return PBSAM_ERR_BADARG;
+  return guarded(&S->c->err, [&]() {
+    int rc = system_lists(S);
+    if (rc != PBSAM_OK) return rc;
+    if (S->nmol == 1 || S->pol.empty()) {
+      if (rounds_out) *rounds_out = 0;
+      return PBSAM_OK;
+    }
+    system_phase0(S);
+    bool refresh = (flags & PBSAM_HROT_REFRESH) != 0;
+    double md = system_round(S, true, refresh);
+    int r = 1;
+    while (md > tol && r < max_rounds) {
+      md = system_round(S, false, refresh);
+      r++;
+    }
+    if (rounds_out) *rounds_out = r;
+    return md > tol ? PBSAM_ERR_NOCONV : PBSAM_OK;
+  });
+}
+
+int pbsam_system_get(const pbsam_system *S, int i, int k, double *F, double *H, double *LS, double *Hrot) {
+  if (!S || i < 0 || i >= S->nmol || k < 0 || k >= S->mols[i].mt->ns) return PBSAM_ERR_BADARG;
+  return guarded(&S->c->err, [&]() {
+    hipStream_t s = S->c->stream;
+    const int P2 = S->p * S->p;
+    const size_t g = (size_t)S->mols[i].base + k;
+    const DBuf<double> *src[4] = {&S->d_F, &S->d_H, &S->d_LS, &S->d_Hrot};
+    double *dst[4] = {F, H, LS, Hrot};
+    for (int a = 0; a < 4; a++)
+      if (dst[a]) HIPCHK(hipMemcpyAsync(dst[a], src[a]->p + g * P2, sizeof(double) * P2, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_energy(pbsam_system *S, double *e) {
+  if (!S || !e) return PBSAM_ERR_BADARG;
+  return guarded(&S->c->err, [&]() {
+    std::vector<double> H = S->d_H.download(S->c->stream), LS = S->d_LS.download(S->c->stream);
+    const int p = S->p, P2 = p * p;
+    for (int i = 0; i < S->nmol; i++) {
+      double pot = 0.0;
+      const MolInst &M = S->mols[i];
+      for (int k = 0; k < M.mt->ns; k++) {
+        const double *h = &H[(size_t)(M.base + k) * P2], *l = &LS[(size_t)(M.base + k) * P2];
+        double sum = 0.0;
+        for (int n = 0; n < p; n++) {
+          double t = 0.0;
+          for (int mm = 1; mm < 2 * n + 1; mm++) t += h[n * n + mm] * l[n * n + mm];
+          t *= 2.0;
+          t += h[n * n] * l[n * n];
+          sum += 1.0 * t;
+        }
+        pot += sum;
+      }
+      e[i] = pot;
+    }
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_system_counts(const pbsam_system *S, int *n_inter_pairs, int *n_polarized) {
+  if (!S) return PBSAM_ERR_BADARG;
+  if (n_inter_pairs) *n_inter_pairs = S->n_inter;
+  if (n_polarized) *n_polarized = (int)S->pol.size();
+  return PBSAM_OK;
+}
+
+// ---- kernel-level batches ----
+static int xform_batch(pbsam_ctx *c, int n, int p, const double *P, const double *a, const int *orders, bool intra,
+                       const double *H, const double *F, double *outH, double *outF) {
+  if (!c || n < 1 || p < 1 || p > PBSAM_PMAX || !P || !a || !orders || !H || !outH || (intra && (!F || !outF)))
+    return PBSAM_ERR_BADARG;
+  for (int i = 0; i < n * (intra ? 2 : 1); i++)
+    if (orders[i] < 1 || orders[i] > p) return PBSAM_ERR_BADARG;
+  return guarded(&c->err, [&]() {
+    hipStream_t s = c->stream;
+    const int P2 = p * p;
+    const long long TS = tmat_size(p);
+    std::vector<XJob> xj(n);
+    std::vector<TJob> tj;
+    std::vector<int> oH(n), oF(n);
+    for (int i = 0; i < n; i++) {
+      XJob &J = xj[i];
+      J.src = i;
+      J.tgt = i;
+      J.slot = i;
+      J.kind = intra ? 1 : 0;
+      for (int x = 0; x < 3; x++) J.P[x] = P[3 * i + x];
+      J.a_src = a[2 * i];
+      J.a_tgt = a[2 * i + 1];
+      TJob T{};
+      for (int x = 0; x < 3; x++) T.P[x] = J.P[x];
+      T.a_src = J.a_src;
+      T.a_tgt = J.a_tgt;
+      T.kappa_seed = c->kappa;
+      T.off = (long long)tj.size() * TS;
+      tj.push_back(T);
+      if (intra) {
+        T.kappa_seed = 0.0;
+        T.off = (long long)tj.size() * TS;
+        tj.push_back(T);
+      }
+      oH[i] = intra ? orders[2 * i] : orders[i];
+      oF[i] = intra ? orders[2 * i + 1] : 0;
+    }
+    DBuf<double> dT, dH, dF, dsH, dsF;
+    dT.alloc(tj.size() * TS);
+    for (int i = 0; i < n; i++) {
+      xj[i].TH = dT.p + (size_t)(intra ? 2 * i : i) * TS;
+      xj[i].TF = intra ? dT.p + (size_t)(2 * i + 1) * TS : nullptr;
+    }
+    DBuf<XJob> dxj;
+    DBuf<TJob> dtj;
+    DBuf<int> doH, doF;
+    dxj.upload(xj, s);
+    dtj.upload(tj, s);
+    doH.upload(oH, s);
+    doF.upload(oF, s);
+    dH.upload(std::vector<double>(H, H + (size_t)n * P2), s);
+    if (intra) dF.upload(std::vector<double>(F, F + (size_t)n * P2), s);
+    dsH.alloc((size_t)n * P2);
+    dsF.alloc((size_t)n * P2);
+    launch_trans(c, (int)tj.size(), dtj, p, dT.p);
+    XArgs xa{};
+    xa.jobs = dxj.p;
+    xa.H = dH.p;
+    xa.F = intra ? dF.p : dH.p;
+    xa.slotH = dsH.p;
+    xa.slotF = dsF.p;
+    xa.ordH = doH.p;
+    xa.ordF = doF.p;
+    xa.p = p;
+    launch_xform(c, n, xa);
+    std::vector<double> rH = dsH.download(s);
+    memcpy(outH, rH.data(), sizeof(double) * n * P2);
+    if (intra) {
+      std::vector<double> rF = dsF.download(s);
+      memcpy(outF, rF.data(), sizeof(double) * n * P2);
+    }
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_xform_h_batch(pbsam_ctx *c, int n, int p, const double *P, const double *a, const int *order,
+                        const double *in, double *out) {
+  return xform_batch(c, n, p, P, a, order, false, in, nullptr, out, nullptr);
+}
+
+int pbsam_xform_intra_batch(pbsam_ctx *c, int n, int p, const double *P, const double *a, const int *orders,
+                            const double *H, const double *F, double *outH, double *outF) {
+  return xform_batch(c, n, p, P, a, orders, true, H, F, outH, outF);
+}
+
+int pbsam_xform_num_batch(pbsam_ctx *c, int n, int p, const double *P, const double *a, int npts, const double *xyz,
+                          const double *qF, const double *qH, double tQF, double tQH, double *outF, double *outH) {
+  if (!c || n < 1 || p < 1 || p > PBSAM_PMAX || npts < 1 || !P || !a || !xyz || !qF || !qH || !outF || !outH)
+    return PBSAM_ERR_BADARG;
+  return guarded(&c->err, [&]() {
+    hipStream_t s = c->stream;
+    const int P2 = p * p;
+    std::vector<PJob> pj(n);
+    for (int i = 0; i < n; i++) {
+      pj[i] = PJob{};
+      pj[i].pt_off = 0;
+      pj[i].npts = npts;
+      pj[i].slot = i;
+      pj[i].src = 0;
+      for (int x = 0; x < 3; x++) pj[i].P[x] = P[3 * i + x];
+      pj[i].a_src = a[2 * i];
+      pj[i].a_tgt = a[2 * i + 1];
+    }
+    DBuf<PJob> dpj;
+    DBuf<double> dx, dqF, dqH, dt1, dt2, o1, o2;
+    dpj.upload(pj, s);
+    dx.upload(std::vector<double>(xyz, xyz + 3 * (size_t)npts), s);
+    dqF.upload(std::vector<double>(qF, qF + npts), s);
+    dqH.upload(std::vector<double>(qH, qH + npts), s);
+    dt1.upload(std::vector<double>(1, tQF), s);
+    dt2.upload(std::vector<double>(1, tQH), s);
+    o1.alloc((size_t)n * P2);
+    o2.alloc((size_t)n * P2);
+    PArgs pa{};
+    pa.jobs = dpj.p;
+    pa.xyz = dx.p;
+    pa.q1 = dqF.p;
+    pa.q2 = dqH.p;
+    pa.TQ1 = dt1.p;
+    pa.TQ2 = dt2.p;
+    pa.out1 = o1.p;
+    pa.out2 = o2.p;
+    pa.kap1 = 0;
+    pa.kap2 = 1;
+    pa.p = p;
+    launch_p2l(c, n, pa);
+    std::vector<double> r1 = o1.download(s), r2 = o2.download(s);
+    memcpy(outF, r1.data(), sizeof(double) * n * P2);
+    memcpy(outH, r2.data(), sizeof(double) * n * P2);
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_rotate_quat_batch(pbsam_ctx *c, int n, int p, const double *quat, int fwd, const double *in, double *out) {
+  if (!c || n < 1 || p < 1 || p > PBSAM_PMAX || !quat || !in || !out) return PBSAM_ERR_BADARG;
+  return guarded(&c->err, [&]() {
+    hipStream_t s = c->stream;
+    const int P2 = p * p;
+    const long long RS = 2 * rot_off(p);
+    std::vector<RJob> rj(n);
+    std::vector<RAJob> ra(n);
+    for (int i = 0; i < n; i++) {
+      double q[4] = {quat[4 * i], quat[4 * i + 1], quat[4 * i + 2], quat[4 * i + 3]};
+      quat_normalize(q);
+      quat_euler(q, rj[i].theta, rj[i].phi, rj[i].xi);
+      rj[i].off = (long long)i * RS;
+      rj[i].sing = 0;
+      ra[i] = {i, i, i, fwd ? 1 : 0};
+    }
+    DBuf<RJob> drj;
+    DBuf<RAJob> dra;
+    DBuf<double> dR, din, dout;
+    drj.upload(rj, s);
+    dra.upload(ra, s);
+    dR.alloc((size_t)n * RS);
+    din.upload(std::vector<double>(in, in + (size_t)n * P2), s);
+    dout.alloc((size_t)n * P2);
+    launch_rotcoeff(c, n, drj.p, p, dR.p);
+    launch_rotapply(c, n, dra.p, drj.p, dR.p, din.p, dout.p, p);
+    std::vector<double> r = dout.download(s);
+    memcpy(out, r.data(), sizeof(double) * n * P2);
+    return PBSAM_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,991 @@
+// HIP kernels for the PB-SAM mutual-polarization path (gfx950, FP64).
+//
+// Kernel map (reference functions restated, paths relative to the reference src/):
+//   k_trans        CTransCoeff::reset/computeCoeff/computeCoeff_   transcoeff.cpp:151-203,372-449
+//   k_xform        CXFormA/CXFormAIntra::reset+xformH/xformF      xforms.cpp:51-77,123-132,214-270
+//                  (CRotCoeff::reset/computeCoeff/rotate rotcoeff.cpp:155-171,357-441,780-810,
+//                   CTransCoeff::translate transcoeff.cpp:211-286) -- fused per pair
+//   k_p2l          CXFormNIntra::xformFH / CLocalExpan / CMulExpan xforms.cpp:528-550,
+//                  expansion.cpp:203-340
+//   k_rotcoeff     CRotCoeff::reset(CQuat) rotcoeff.cpp:181-201 (coefficients stored)
+//   k_rotapply     CRotCoeff::rotateWithXi rotcoeff.cpp:250-350
+//   k_gather       LS / LHS / LFS assembly molecule.cpp:1214-1297,1414-1473
+//   k_solve        CSolExpCenter::solveFH expcenter.cpp:1080-1370
+//   k_surface      computeExposedSurfaceChargesFH expcenter.cpp:1420-1440
+#include "pbsam_device.h"
+
+namespace pbsam {
+
+// ----------------------------------------------------------------------------
+// translation matrices: one wavefront per pair, recursion over (l, n) per level m
+// Output per pair: for each m, dense M_m[n-m][l-m] (n,l in [m,p)) holding the factor the
+// forward translate multiplies Min(l,m) by:  l<=n: FACT(n-l)*(-1)^(n+l)*T(n,l,m), l>n: T(l,n,m)
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_trans(Tabs tb, const TJob *jobs, int p, double *Tout) {
+  extern __shared__ double sh[];
+  const int L = 2 * p - 1;  // recursion rows l in [0, 2p-2]
+  double *U = sh;           // [(L+1)][p]   U[l*p+n] for the current level (columns n>=m)
+  double *Ks = U + (L + 1) * p;   // [2p]
+  double *fact = Ks + 2 * p;      // [p]
+  const TJob J = jobs[blockIdx.x];
+  const int tid = threadIdx.x;
+  double rho, th, ph;
+  cart_to_sph(J.P[0], J.P[1], J.P[2], rho, th, ph);
+  const double scalei = J.a_src, scaleo = J.a_tgt;
+  double *out = Tout + J.off;
+  for (int pass = 0; pass < (J.grad ? 2 : 1); pass++) {
+    // seeds T(l,0,0) (transcoeff.cpp:374-391): sequential product chain on lane 0
+    if (tid == 0) {
+      double d = rho, kd = J.kappa_seed * d, ir = 0, exkid = 0;
+      if (d > 0.0) {
+        ir = 1.0 / d;
+        exkid = exp(-kd) * ir;
+      }
+      for (int i = 0; i < 2 * p; i++) Ks[i] = 1.0;
+      if (kd != 0.0 && 2 * p > 1) {
+        Ks[1] += kd;
+        double vs = kd * kd;
+        for (int n = 2; n < 2 * p; n++) Ks[n] = bessel_k_next(tb, n, Ks[n - 1], Ks[n - 2], vs);
+      }
+      if (pass == 0) U[0] = exkid * Ks[0];
+      exkid *= ir;
+      if (pass == 1) U[0] = -exkid * Ks[1];
+      exkid *= scalei;
+      for (int l = 1; l < 2 * p - 1; l++) {
+        if (pass == 0) U[l * p] = exkid * Ks[l];
+        exkid *= ir;
+        if (pass == 1) U[l * p] = exkid * (l * Ks[l] - (2 * l + 1) * Ks[l + 1]);
+        exkid *= scalei;
+      }
+      fact[0] = 1.0;
+      for (int n = 1; n < p; n++) fact[n] = fact[n - 1] * (scaleo / scalei);
+    }
+    __syncthreads();
+    const double kappasqr = tb.kappa * tb.kappa;
+    const double kpio = kappasqr * scalei * scaleo;
+    const double kpoo = kappasqr * scaleo * scaleo;
+    const double o_i = scaleo / scalei;
+    double *o = out + pass * tmat_size(p);
+    for (int m = 0; m < p; m++) {
+      if (m == 0) {
+        for (int l = 1 + tid; l < 2 * p - 2; l += 64)
+          U[l * p + 1] = -(kpio * BETA(tb, l - 1, 0) * U[(l - 1) * p + 0] +
+                           o_i * ALPHA(tb, l, 0) * U[(l + 1) * p + 0]) / ALPHA(tb, 0, 0);
+        __syncthreads();
+        for (int n = 1; n < p - 1; n++) {
+          for (int l = n + 1 + tid; l < 2 * p - 2 - n; l += 64)
+            U[l * p + n + 1] = -(kpio * BETA(tb, l - 1, 0) * U[(l - 1) * p + n] +
+                                 kpoo * BETA(tb, n - 1, 0) * U[l * p + n - 1] +
+                                 o_i * ALPHA(tb, l, 0) * U[(l + 1) * p + n]) / ALPHA(tb, n, 0);
+          __syncthreads();
+        }
+      } else {
+        // diagonal U(l,m,m) from column m-1 of the previous level
+        for (int l = m + tid; l < 2 * p - 1 - m; l += 64)
+          U[l * p + m] = -(kpio * DELTA(tb, l, -m) * U[(l - 1) * p + m - 1] +
+                           o_i * GAMMA(tb, l + 1, m - 1) * U[(l + 1) * p + m - 1]) / GAMMA(tb, m, -m);
+        __syncthreads();
+        {
+          int n = m;
+          for (int l = n + 1 + tid; l < 2 * p - 2 - n; l += 64)
+            U[l * p + n + 1] = -(kpio * BETA(tb, l - 1, m) * U[(l - 1) * p + n] +
+                                 o_i * ALPHA(tb, l, m) * U[(l + 1) * p + n]) / ALPHA(tb, n, m);
+          __syncthreads();
+        }
+        for (int n = m + 1; n < p - 1; n++) {
+          for (int l = n + 1 + tid; l < 2 * p - 2 - n; l += 64)
+            U[l * p + n + 1] = -(kpio * BETA(tb, l - 1, m) * U[(l - 1) * p + n] +
+                                 kpoo * BETA(tb, n - 1, m) * U[l * p + n - 1] +
+                                 o_i * ALPHA(tb, l, m) * U[(l + 1) * p + n]) / ALPHA(tb, n, m);
+          __syncthreads();
+        }
+      }
+      // emit M_m: rows n, cols l in [m,p)
+      const int w = p - m;
+      double *Mm = o + tmat_off(p, m);
+      for (int e = tid; e < w * w; e += 64) {
+        int n = m + e / w, l = m + e % w;
+        double v;
+        if (l <= n) {
+          double T = U[n * p + l];
+          T = ((n + l) % 2 == 0) ? T : -T;
+          v = fact[n - l] * T;
+        } else {
+          v = U[l * p + n];
+        }
+        Mm[e] = v;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// fused rotate -> translate -> rotate-back, one workgroup per pair.
+// LDS: two rotation levels (2q-1)^2 complex each, source blocks, per-level vectors.
+// ----------------------------------------------------------------------------
+constexpr int XT = 512;  // threads per workgroup
+constexpr int XG = 16;   // lanes per dot product in the forward rotate / translate
+
+__device__ __forceinline__ double seg_reduce(double v) {
+  for (int o = XG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, XG);
+  return v;
+}
+
+struct XArgs {
+  const XJob *jobs;
+  const double *H;     // source H blocks (Hrot for inter pairs)
+  const double *F;     // source F blocks
+  const double *TQH;   // source totals driving the truncation order
+  const double *TQF;
+  double *slotH, *slotF;
+  const int *ordH, *ordF;  // explicit orders per job (null: adaptive from the totals)
+  int p;
+  int full;            // 1: use order p for both blocks (no adaptive truncation)
+};
+
+__global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
+  extern __shared__ double sh[];
+  const XJob J = A.jobs[blockIdx.x];
+  const int p = A.p, P2 = p * p;
+  const int tid = threadIdx.x;
+  const bool intra = (J.kind == 1);
+  double rho, th, ph;
+  cart_to_sph(J.P[0], J.P[1], J.P[2], rho, th, ph);
+  int pH, pF = 0;
+  if (A.ordH) {
+    pH = A.ordH[blockIdx.x];
+    if (intra) pF = A.ordF[blockIdx.x];
+  } else {
+    pH = A.full ? p : compute_order(rho, A.TQH[J.src], J.a_src, p);
+    if (intra) pF = A.full ? p : compute_order(rho, A.TQF[J.src], J.a_src, p);
+  }
+  const int q = pH > pF ? pH : pF;
+  const int NR = 2 * q - 1;  // rotation rows n < 2q-1
+  // LDS carve
+  cd *cur = reinterpret_cast<cd *>(sh);
+  cd *nxt = cur + NR * NR;
+  double *hs = reinterpret_cast<double *>(nxt + NR * NR);  // source H (q^2)
+  double *fs = hs + q * q;                                   // source F
+  cd *aH = reinterpret_cast<cd *>(fs + q * q);               // forward-rotated, per level [q]
+  cd *aF = aH + q;
+  cd *bH = aF + q;                                           // translated
+  cd *bF = bH + q;
+  for (int i = tid; i < q * q; i += XT) {
+    hs[i] = (i < pH * pH) ? A.H[(size_t)J.src * P2 + i] : 0.0;
+    fs[i] = (intra && i < pF * pF) ? A.F[(size_t)J.src * P2 + i] : 0.0;
+  }
+  // rotation parameters (CRotCoeff::initParams rotcoeff.cpp:209-244), xi = pi
+  double sint = sin(th), cost;
+  int sing = 0;
+  cd exphi, exxi;
+  if (sint < 1e-12) {
+    sing = (th < M_PI / 2) ? 1 : -1;
+    sint = 0.0;
+    cost = (sing == 1) ? 1.0 : -1.0;
+    exphi = {1.0, 0.0};
+  } else {
+    cost = cos(th);
+    double s_, c_;
+    sincos(ph, &s_, &c_);
+    exphi = {c_, s_};
+  }
+  const cd exiphi = cconj(exphi);
+  {
+    double s_, c_;
+    sincos(M_PI, &s_, &c_);
+    exxi = {c_, s_};
+  }
+  // back-rotated accumulators: each thread owns up to two (n,s) outputs
+  const int nout = q * (q + 1) / 2;
+  int on[2], os[2];
+  cd accH[2], accF[2];
+  for (int k = 0; k < 2; k++) {
+    int e = tid + k * XT;
+    on[k] = -1;
+    os[k] = 0;
+    accH[k] = {0.0, 0.0};
+    accF[k] = {0.0, 0.0};
+    if (e < nout) {
+      int n = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+      while (n * (n + 1) / 2 > e) n--;
+      while ((n + 1) * (n + 2) / 2 <= e) n++;
+      on[k] = n;
+      os[k] = e - n * (n + 1) / 2;
+    }
+  }
+  if (sing != 0) {
+    // singular rotation: rotate() reduces to identity / parity maps (rotcoeff.cpp:368-393)
+    __syncthreads();
+    for (int m = 0; m < q; m++) {
+      // forward "rotation" value a(n,m) = sign * c(n,m)
+      for (int n = m + tid; n < q; n += XT) {
+        double sg = 1.0, sgi = 1.0;
+        if (sing == -1) {
+          // Mout(n,mm) = s*Min(n,mm), s = (-1)^n at mm = 0 and mm = 1, alternating over mm
+          int s0 = (n % 2 == 0) ? 1 : -1;
+          sg = s0;                      // mm = 0 and odd mm (real parts)
+          if (m > 0) sgi = -s0;         // even mm >= 2 (imaginary parts)
+        }
+        if (n < pH) aH[n] = (m == 0) ? cd{sg * hs[n * n], 0.0}
+                                     : cd{sg * hs[n * n + 2 * m - 1], sgi * hs[n * n + 2 * m]};
+        else aH[n] = {0.0, 0.0};
+        if (intra && n < pF) aF[n] = (m == 0) ? cd{sg * fs[n * n], 0.0}
+                                              : cd{sg * fs[n * n + 2 * m - 1], sgi * fs[n * n + 2 * m]};
+        else aF[n] = {0.0, 0.0};
+      }
+      __syncthreads();
+      for (int n = m + tid; n < q; n += XT) {
+        const int w = p - m;
+        cd sH = {0, 0}, sF = {0, 0};
+        if (n < pH) {
+          const double *Mm = J.TH + tmat_off(p, m) + (size_t)(n - m) * w;
+          for (int l = m; l < pH; l++) sH = cadd(sH, cscale(Mm[l - m], aH[l]));
+        }
+        if (intra && n < pF) {
+          const double *Mm = J.TF + tmat_off(p, m) + (size_t)(n - m) * w;
+          for (int l = m; l < pF; l++) sF = cadd(sF, cscale(Mm[l - m], aF[l]));
+        }
+        bH[n] = sH;
+        bF[n] = sF;
+      }
+      __syncthreads();
+      for (int k = 0; k < 2; k++) {
+        int n = on[k], s = os[k];
+        if (n < 0 || s != m) continue;
+        double sg = 1.0, sgi = 1.0;
+        if (sing == -1) {
+          int s0 = (n % 2 == 0) ? 1 : -1;
+          sg = s0;
+          if (m > 0) sgi = -s0;
+        }
+        accH[k] = (m == 0) ? cd{sg * bH[n].x, 0.0} : cd{sg * bH[n].x, sgi * bH[n].y};
+        accF[k] = (m == 0) ? cd{sg * bF[n].x, 0.0} : cd{sg * bF[n].x, sgi * bF[n].y};
+      }
+      __syncthreads();
+    }
+  } else {
+    // level 0 seeds: ROT(n,s,0) = conj-ordered SH of order 2q-1 (rotcoeff.cpp:785-795)
+    for (int m = tid; m < NR; m += XT) {
+      sh_column(tb, NR, m, cost, sint, ph, [&](int n, double re, double im) {
+        if (m == 0) {
+          cur[n * n + n] = {re, 0.0};
+        } else {
+          cur[n * n + n + m] = {re, -im};  // ROT(n, m,0) = comp(n,-m)
+          cur[n * n + n - m] = {re, im};   // ROT(n,-m,0) = comp(n, m)
+        }
+      });
+    }
+    const double r1 = -0.5 * (1 + cost), r2 = 0.5 * (1 - cost), r3 = -sint;
+    __syncthreads();
+    for (int m = 0; m < q; m++) {
+      // (a) forward rotation of this level: a(n) for n in [m, q)
+      {
+        int g = tid / XG, lane = tid % XG;
+        for (int n = m + g; n < q; n += XT / XG) {
+          const cd *row = cur + n * n + n;  // row[s] = ROT(n,s,m)
+          cd sH = {0, 0}, sF = {0, 0};
+          for (int s = lane; s <= n; s += XG) {
+            if (s == 0) {
+              if (n < pH) sH = cadd(sH, cmul(row[0], cd{hs[n * n], 0.0}));
+              if (intra && n < pF) sF = cadd(sF, cmul(row[0], cd{fs[n * n], 0.0}));
+            } else {
+              if (n < pH) {
+                cd c = {hs[n * n + 2 * s - 1], hs[n * n + 2 * s]};
+                sH = cadd(sH, cadd(cmul(row[s], c), cmul(row[-s], cconj(c))));
+              }
+              if (intra && n < pF) {
+                cd c = {fs[n * n + 2 * s - 1], fs[n * n + 2 * s]};
+                sF = cadd(sF, cadd(cmul(row[s], c), cmul(row[-s], cconj(c))));
+              }
+            }
+          }
+          sH.x = seg_reduce(sH.x);
+          sH.y = seg_reduce(sH.y);
+          sF.x = seg_reduce(sF.x);
+          sF.y = seg_reduce(sF.y);
+          if (lane == 0) {
+            if (m == 0) { sH.y = 0.0; sF.y = 0.0; }  // Mout(n,0) keeps the real part
+            aH[n] = sH;
+            aF[n] = sF;
+          }
+        }
+      }
+      // (b) recursion to level m+1 (rotcoeff.cpp:801-809), rows n-1 for n in [m+2, 2q-m-2]
+      if (m < q - 1) {
+        const int base = (m + 1) * (m + 1);
+        const int total = (2 * q - m - 2) * (2 * q - m - 2) - (m + 1) * (m + 1);
+        for (int e = tid; e < total; e += XT) {
+          int t = e + base;
+          int n = (int)sqrt((double)t);
+          while (n * n > t) n--;
+          while ((n + 1) * (n + 1) <= t) n++;
+          n += 1;                         // element t belongs to row n: start (n-1)^2
+          int s = t - (n - 1) * (n - 1) - (n - 1);
+          const cd *row = cur + n * n + n;
+          cd d1 = cscale(r1 * ETA(tb, n, s - 1), row[s - 1]);
+          cd d2 = cscale(r2 * ETA(tb, n, -s - 1), row[s + 1]);
+          cd d3 = cscale(r3 * ZETA(tb, n - 1, abs(s)), row[s]);
+          cd v = cadd(cadd(cmul(d1, exiphi), cmul(d2, exphi)), d3);
+          v = cmul(exxi, v);
+          double em = ETA(tb, n, m);
+          nxt[(n - 1) * (n - 1) + (n - 1) + s] = {v.x / em, v.y / em};
+        }
+      }
+      __syncthreads();
+      // (c) translate this level: b(n) = sum_l M_m[n][l] a(l)
+      {
+        int g = tid / XG, lane = tid % XG;
+        const int w = p - m;
+        for (int n = m + g; n < q; n += XT / XG) {
+          cd sH = {0, 0}, sF = {0, 0};
+          if (n < pH) {
+            const double *Mm = J.TH + tmat_off(p, m) + (size_t)(n - m) * w;
+            for (int l = m + lane; l < pH; l += XG) sH = cadd(sH, cscale(Mm[l - m], aH[l]));
+          }
+          if (intra && n < pF) {
+            const double *Mm = J.TF + tmat_off(p, m) + (size_t)(n - m) * w;
+            for (int l = m + lane; l < pF; l += XG) sF = cadd(sF, cscale(Mm[l - m], aF[l]));
+          }
+          sH.x = seg_reduce(sH.x);
+          sH.y = seg_reduce(sH.y);
+          sF.x = seg_reduce(sF.x);
+          sF.y = seg_reduce(sF.y);
+          if (lane == 0) {
+            if (m == 0) { sH.y = 0.0; sF.y = 0.0; }
+            bH[n] = sH;
+            bF[n] = sF;
+          }
+        }
+      }
+      __syncthreads();
+      // (d) back-rotation contribution of level m (rotcoeff.cpp:420-438)
+      for (int k = 0; k < 2; k++) {
+        int n = on[k], s = os[k];
+        if (n < m) continue;
+        const cd *row = cur + n * n + n;
+        if (n < pH) {
+          cd b = bH[n];
+          if (m == 0) accH[k] = cadd(accH[k], cmul(cscale(1.0, row[-s]), cd{b.x, 0.0}));
+          else accH[k] = cadd(accH[k], cadd(cmul(cconj(row[s]), b), cmul(row[-s], cconj(b))));
+        }
+        if (intra && n < pF) {
+          cd b = bF[n];
+          if (m == 0) accF[k] = cadd(accF[k], cmul(cscale(1.0, row[-s]), cd{b.x, 0.0}));
+          else accF[k] = cadd(accF[k], cadd(cmul(cconj(row[s]), b), cmul(row[-s], cconj(b))));
+        }
+      }
+      __syncthreads();
+      cd *tmp = cur;
+      cur = nxt;
+      nxt = tmp;
+    }
+  }
+  // write slots (orders beyond the truncation are zero)
+  double *oH = A.slotH + (size_t)J.slot * P2;
+  double *oF = intra ? A.slotF + (size_t)J.slot * P2 : nullptr;
+  for (int i = tid; i < P2; i += XT) {
+    oH[i] = 0.0;
+    if (oF) oF[i] = 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < 2; k++) {
+    int n = on[k], s = os[k];
+    if (n < 0) continue;
+    if (n < pH) {
+      if (s == 0) oH[n * n] = accH[k].x;
+      else { oH[n * n + 2 * s - 1] = accH[k].x; oH[n * n + 2 * s] = accH[k].y; }
+    }
+    if (oF && n < pF) {
+      if (s == 0) oF[n * n] = accF[k].x;
+      else { oF[n * n + 2 * s - 1] = accF[k].x; oF[n * n + 2 * s] = accF[k].y; }
+    }
+  }
+}
+
+
+// ----------------------------------------------------------------------------
+// point-to-expansion sums: CLocalExpan / CMulExpan of point charges
+// (expansion.cpp:203-340), used for the numerical re-expansion of surface charges
+// (CXFormNIntra::xformFH, xforms.cpp:528-550) and for fixed-charge expansions.
+// One workgroup per job; points are processed in tiles held in LDS, one thread per
+// coefficient accumulates over the points in their input order.
+// ----------------------------------------------------------------------------
+constexpr int PT = 256;
+
+struct PJob {
+  int pt_off, npts;  // point range (xyz and charges share the index)
+  int slot;          // output slot
+  int src;           // source sphere for the truncation order (-1: full order)
+  double P[3];       // target centre - source centre (points are relative to the source)
+  double a_src, a_tgt;
+};
+struct PArgs {
+  const PJob *jobs;
+  const double *xyz;
+  const double *q1, *q2;     // two charge sets on the same points (q2 may be null)
+  const double *TQ1, *TQ2;   // source totals for the orders (null: full order)
+  double *out1, *out2;
+  int kap1, kap2;            // screened (k-hat with kappa) for each set
+  int multipole;             // 0: local (CLExpan), 1: multipole (CMExpan)
+  int tile;                  // points per tile
+  int p;
+};
+
+__global__ void __launch_bounds__(PT) k_p2l(Tabs tb, PArgs A) {
+  extern __shared__ double sh[];
+  const PJob J = A.jobs[blockIdx.x];
+  const int p = A.p, P2 = p * p, tid = threadIdx.x;
+  const bool two = (A.q2 != nullptr);
+  double rho0 = sqrt(J.P[0] * J.P[0] + J.P[1] * J.P[1] + J.P[2] * J.P[2]);
+  int p1 = p, p2 = two ? p : 0;
+  if (J.src >= 0 && A.TQ1) p1 = compute_order(rho0, A.TQ1[J.src], J.a_src, p);
+  if (two && J.src >= 0 && A.TQ2) p2 = compute_order(rho0, A.TQ2[J.src], J.a_src, p);
+  const int q = p1 > p2 ? p1 : p2;
+  const int T = A.tile;
+  double *Y = sh;                    // [T][q*q]
+  double *w1 = Y + (size_t)T * q * q; // [T][q]
+  double *w2 = w1 + T * q;           // [T][q]
+  double *geo = w2 + T * q;          // [T][3]: cost, sint, phi
+  double acc1[4] = {0, 0, 0, 0}, acc2[4] = {0, 0, 0, 0};
+  const int q2 = q * q;
+  for (int t0 = 0; t0 < J.npts; t0 += T) {
+    const int nt = min(T, J.npts - t0);
+    // phase 0: geometry and radial factors of each point
+    for (int t = tid; t < nt; t += PT) {
+      const double *x = A.xyz + 3 * (size_t)(J.pt_off + t0 + t);
+      double dx = x[0] - J.P[0], dy = x[1] - J.P[1], dz = x[2] - J.P[2];
+      double rho, th, ph;
+      cart_to_sph(dx, dy, dz, rho, th, ph);
+      geo[3 * t] = cos(th);
+      geo[3 * t + 1] = sin(th);
+      geo[3 * t + 2] = ph;
+      for (int set = 0; set < (two ? 2 : 1); set++) {
+        const double ch = (set == 0 ? A.q1 : A.q2)[J.pt_off + t0 + t];
+        const int kap = (set == 0 ? A.kap1 : A.kap2);
+        const int qq = (set == 0 ? p1 : p2);
+        double *w = (set == 0 ? w1 : w2) + t * q;
+        if (ch == 0.0) {
+          for (int n = 0; n < q; n++) w[n] = 0.0;
+          continue;
+        }
+        const double val = kap ? rho * tb.kappa : 0.0;
+        if (!A.multipole) {
+          // CLExpan::init + BESSEL + applyRadialComponent (expansion.cpp:289-316,422-436)
+          double mr = 1.0 / rho;
+          double mk = ch;
+          mk *= mr * (kap ? exp(-val) : 1.0);
+          mr *= J.a_tgt;
+          double km2 = 1.0, km1 = 1.0 + val, vs = val * val;
+          for (int n = 0; n < qq; n++) {
+            double K;
+            if (val == 0.0) K = 1.0;
+            else if (n == 0) K = 1.0;
+            else if (n == 1) K = km1;
+            else {
+              K = bessel_k_next(tb, n, km1, km2, vs);
+              km2 = km1;
+              km1 = K;
+            }
+            w[n] = K * mk;
+            mk *= mr;
+          }
+        } else {
+          // CMExpan::init + BESSEL + applyRadialComponent (expansion.cpp:323-365)
+          double mscale = 1.0 / J.a_tgt;
+          double mr = rho * mscale;
+          double mk = ch;
+          double z = 0.5 * val * val;
+          for (int n = 0; n < qq; n++) {
+            double I = 1.0;
+            if (val != 0.0) {
+              double tt = z / (2 * n + 3);
+              for (int j = 1; j <= 20; j++) {
+                I += tt;
+                tt *= (z / ((j + 1) * (2 * (n + j) + 3)));
+                if (tt < 1e-20) break;
+              }
+            }
+            w[n] = I * mk;
+            mk *= mr;
+          }
+        }
+        for (int n = qq; n < q; n++) w[n] = 0.0;
+      }
+    }
+    __syncthreads();
+    // phase A: spherical harmonics of the tile, one (point, column) per task
+    for (int task = tid; task < nt * q; task += PT) {
+      const int t = task / q, m = task % q;
+      double *Yt = Y + (size_t)t * q2;
+      sh_column(tb, q, m, geo[3 * t], geo[3 * t + 1], geo[3 * t + 2], [&](int n, double re, double im) {
+        if (m == 0) Yt[n * n] = re;
+        else {
+          Yt[n * n + 2 * m - 1] = re;
+          Yt[n * n + 2 * m] = im;
+        }
+      });
+    }
+    __syncthreads();
+    // phase B: accumulate in point order
+    for (int k = 0; k < 4; k++) {
+      const int c = tid + k * PT;
+      if (c >= q2) break;
+      int n = (int)sqrt((double)c);
+      while (n * n > c) n--;
+      while ((n + 1) * (n + 1) <= c) n++;
+      for (int t = 0; t < nt; t++) {
+        double y = Y[(size_t)t * q2 + c];
+        if (n < p1) acc1[k] += y * w1[t * q + n];
+        if (two && n < p2) acc2[k] += y * w2[t * q + n];
+      }
+    }
+    __syncthreads();
+  }
+  double *o1 = A.out1 + (size_t)J.slot * P2;
+  double *o2 = two ? A.out2 + (size_t)J.slot * P2 : nullptr;
+  for (int i = tid; i < P2; i += PT) {
+    o1[i] = 0.0;
+    if (o2) o2[i] = 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < 4; k++) {
+    const int c = tid + k * PT;
+    if (c >= q2) break;
+    int n = (int)sqrt((double)c);
+    while (n * n > c) n--;
+    while ((n + 1) * (n + 1) <= c) n++;
+    if (n < p1) o1[c] = acc1[k];
+    if (o2 && n < p2) o2[c] = acc2[k];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// rotation coefficients stored to HBM: molecule orientations (CRotCoeff::reset
+// rotcoeff.cpp:155-171,780-810).  Stores ROT(n,s,m) for n < p.
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(XT) k_rotcoeff(Tabs tb, RJob *jobs, int p, double *out) {
+  extern __shared__ double sh[];
+  RJob &J = jobs[blockIdx.x];
+  const int tid = threadIdx.x, q = p, NR = 2 * q - 1;
+  cd *cur = reinterpret_cast<cd *>(sh);
+  cd *nxt = cur + NR * NR;
+  double th = J.theta, ph = J.phi;
+  double sint = sin(th), cost;
+  int sing = 0;
+  cd exphi, exxi;
+  if (sint < 1e-12) {
+    sing = (th < M_PI / 2) ? 1 : -1;
+    sint = 0.0;
+    cost = (sing == 1) ? 1.0 : -1.0;
+    exphi = {1.0, 0.0};
+  } else {
+    cost = cos(th);
+    double s_, c_;
+    sincos(J.phi, &s_, &c_);
+    exphi = {c_, s_};
+  }
+  const cd exiphi = cconj(exphi);
+  {
+    double s_, c_;
+    sincos(J.xi, &s_, &c_);
+    exxi = {c_, s_};
+  }
+  if (tid == 0) J.sing = sing;
+  if (sing != 0) return;
+  cd *R = reinterpret_cast<cd *>(out + J.off);
+  for (int m = tid; m < NR; m += XT) {
+    sh_column(tb, NR, m, cost, sint, ph, [&](int n, double re, double im) {
+      if (m == 0) cur[n * n + n] = {re, 0.0};
+      else {
+        cur[n * n + n + m] = {re, -im};
+        cur[n * n + n - m] = {re, im};
+      }
+    });
+  }
+  const double r1 = -0.5 * (1 + cost), r2 = 0.5 * (1 - cost), r3 = -sint;
+  __syncthreads();
+  for (int m = 0; m < q; m++) {
+    // store level m rows n in [m, q)
+    for (int n = m; n < q; n++)
+      for (int s = -n + tid; s <= n; s += XT) R[rot_off(n) + m * (2 * n + 1) + s + n] = cur[n * n + n + s];
+    if (m < q - 1) {
+      const int base = (m + 1) * (m + 1);
+      const int total = (2 * q - m - 2) * (2 * q - m - 2) - (m + 1) * (m + 1);
+      for (int e = tid; e < total; e += XT) {
+        int t = e + base;
+        int n = (int)sqrt((double)t);
+        while (n * n > t) n--;
+        while ((n + 1) * (n + 1) <= t) n++;
+        n += 1;
+        int s = t - (n - 1) * (n - 1) - (n - 1);
+        const cd *row = cur + n * n + n;
+        cd d1 = cscale(r1 * ETA(tb, n, s - 1), row[s - 1]);
+        cd d2 = cscale(r2 * ETA(tb, n, -s - 1), row[s + 1]);
+        cd d3 = cscale(r3 * ZETA(tb, n - 1, abs(s)), row[s]);
+        cd v = cadd(cadd(cmul(d1, exiphi), cmul(d2, exphi)), d3);
+        v = cmul(exxi, v);
+        double em = ETA(tb, n, m);
+        nxt[(n - 1) * (n - 1) + (n - 1) + s] = {v.x / em, v.y / em};
+      }
+    }
+    __syncthreads();
+    cd *tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+}
+
+// rotateWithXi (rotcoeff.cpp:250-350) of whole blocks with stored coefficients.
+struct RAJob {
+  int in, out;       // block indices into the in / out arrays
+  int rot;           // rotation job index
+  int fwd;
+};
+__global__ void __launch_bounds__(256) k_rotapply(const RAJob *jobs, const RJob *rj, const double *Rst,
+                                                  const double *in, double *out, int p) {
+  const RAJob J = jobs[blockIdx.x];
+  const RJob R = rj[J.rot];
+  const int P2 = p * p;
+  const double *Min = in + (size_t)J.in * P2;
+  double *Mout = out + (size_t)J.out * P2;
+  const cd *C = reinterpret_cast<const cd *>(Rst + R.off);
+  const int nout = p * (p + 1) / 2;
+  for (int e = threadIdx.x; e < nout; e += 256) {
+    int n = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    while (n * (n + 1) / 2 > e) n--;
+    while ((n + 1) * (n + 1 + 1) / 2 <= e) n++;
+    const int k = e - n * (n + 1) / 2;  // m (forward) or s (transposed)
+    const double *mi = Min + n * n;
+    if (R.sing != 0) {
+      double xi = J.fwd ? R.xi : -R.xi;
+      if (k == 0) {
+        Mout[n * n] = (R.sing == -1 && (n % 2)) ? -mi[0] : mi[0];
+      } else {
+        cd inc = {mi[2 * k - 1], mi[2 * k]};
+        double sx, cx;
+        sincos(k * xi, &sx, &cx);
+        cd o = cmul(cd{cx, sx}, inc);
+        if (R.sing == -1) {
+          int s = (n % 2 == 0) ? 1 : -1;
+          Mout[n * n + 2 * k - 1] = s * o.x;
+          Mout[n * n + 2 * k] = -s * o.y;
+        } else {
+          Mout[n * n + 2 * k - 1] = o.x;
+          Mout[n * n + 2 * k] = o.y;
+        }
+      }
+      continue;
+    }
+    const cd *row = C + rot_off(n);  // row[m*(2n+1) + s + n]
+    const int W = 2 * n + 1;
+    cd o;
+    if (J.fwd) {
+      const int m = k;
+      o = cmul(cscale(1.0, row[m * W + n]), cd{mi[0], 0.0});
+      for (int s = 1; s <= n; s++) {
+        cd inc = {mi[2 * s - 1], mi[2 * s]};
+        o = cadd(o, cadd(cmul(row[m * W + n + s], inc), cmul(row[m * W + n - s], cconj(inc))));
+      }
+    } else {
+      const int s = k;
+      o = cmul(cscale(1.0, row[0 * W + n - s]), cd{mi[0], 0.0});
+      for (int m = 1; m <= n; m++) {
+        cd inc = {mi[2 * m - 1], mi[2 * m]};
+        o = cadd(o, cadd(cmul(cconj(row[m * W + n + s]), inc), cmul(row[m * W + n - s], cconj(inc))));
+      }
+    }
+    if (k == 0) Mout[n * n] = o.x * 1.0;
+    else {
+      Mout[n * n + 2 * k - 1] = o.x;
+      Mout[n * n + 2 * k] = o.y;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// per-sphere local-field assembly + inner F/H iteration (solveFH)
+// ----------------------------------------------------------------------------
+constexpr int ST = 256;
+
+struct SArgs {
+  const int *sph;          // sphere index per workgroup
+  const int *lst_off;      // per workgroup: CSR offsets into lst (intra slots)
+  const int *lst;          // slot ids
+  const double *slotH, *slotF;
+  const double *LS;        // [sphere] inter-molecular local expansion, or null
+  const double *farH, *farF;  // [sphere] frozen local expansions, or null
+  const double *consts;    // [sphere][10][p]
+  const double *Dfix, *Vfix;  // [sphere][p^2]
+  const double *const *imat;  // [sphere] column-major p^2 x p^2 matrix, null: identity
+  double *F, *H;           // [sphere][p^2] in/out
+  double *LFS, *LHS;       // [sphere][p^2] out
+  double *dev;             // [sphere] out: computeDev2(H_in, H_out)
+  int *sweeps;             // [sphere] out (may be null)
+  int p;
+};
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < (int)(blockDim.x / 64); i++) s += red[i];
+  return s;
+}
+
+__global__ void __launch_bounds__(ST) k_solve(SArgs A) {
+  extern __shared__ double sh[];
+  const int p = A.p, P2 = p * p, tid = threadIdx.x;
+  const int g = A.sph[blockIdx.x];
+  double *F = sh, *H = F + P2, *X = H + P2, *red = X + P2;
+  const double *cs = A.consts + (size_t)g * 10 * p;
+  const double *cH1 = cs, *cF1 = cs + p, *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cH2 = cs + 4 * p,
+               *cF2 = cs + 5 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p, *cInvH = cs + 8 * p;
+  const int l0 = A.lst_off[blockIdx.x], l1 = A.lst_off[blockIdx.x + 1];
+  const double *IM = A.imat[g];
+  constexpr int KPT = 4;  // up to 1024 coefficients per sphere
+  double fb[KPT], hb[KPT], oldH[KPT], outerH[KPT];
+  int nn[KPT];
+  for (int k = 0; k < KPT; k++) {
+    const int i = tid + k * ST;
+    nn[k] = -1;
+    if (i >= P2) continue;
+    int n = (int)sqrt((double)i);
+    while (n * n > i) n--;
+    while ((n + 1) * (n + 1) <= i) n++;
+    nn[k] = n;
+    double LH = A.LS ? A.LS[(size_t)g * P2 + i] : 0.0;
+    double LF = 0.0;
+    if (A.LS) LH = 0.0 + LH;
+    for (int e = l0; e < l1; e++) {
+      LF += A.slotF[(size_t)A.lst[e] * P2 + i];
+      LH += A.slotH[(size_t)A.lst[e] * P2 + i];
+    }
+    if (A.farH) {
+      LH += A.farH[(size_t)g * P2 + i];
+      LF += A.farF[(size_t)g * P2 + i];
+    }
+    A.LFS[(size_t)g * P2 + i] = LF;
+    A.LHS[(size_t)g * P2 + i] = LH;
+    fb[k] = A.Vfix[(size_t)g * P2 + i] + cLH1[n] * LH + cLF1[n] * LF;
+    hb[k] = A.Dfix[(size_t)g * P2 + i] + cLH2[n] * LH + cLF2[n] * LF;
+    F[i] = A.F[(size_t)g * P2 + i];
+    H[i] = A.H[(size_t)g * P2 + i];
+    oldH[k] = H[i];
+    outerH[k] = H[i];
+  }
+  __syncthreads();
+  int ct = 0;
+  double dev = 1e300;
+  while (ct == 0 || (dev > 1e-8 && ct < 100)) {
+    // F <- diag(1/(2n+1)) IMat (fbase + cH1 H + cF1 F)
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * ST;
+      if (nn[k] < 0) continue;
+      X[i] = fb[k] + cH1[nn[k]] * H[i] + cF1[nn[k]] * F[i];
+    }
+    __syncthreads();
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * ST;
+      if (nn[k] < 0) continue;
+      double y;
+      if (IM) {
+        y = 0.0;
+        for (int c = 0; c < P2; c++) y += IM[i + (size_t)c * P2] * X[c];
+      } else
+        y = X[i];
+      F[i] = y * (1 / (double)(2 * nn[k] + 1));
+    }
+    __syncthreads();
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * ST;
+      if (nn[k] < 0) continue;
+      X[i] = hb[k] + cH2[nn[k]] * H[i] + cF2[nn[k]] * F[i];
+    }
+    __syncthreads();
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * ST;
+      if (nn[k] < 0) continue;
+      double y;
+      if (IM) {
+        y = 0.0;
+        for (int c = 0; c < P2; c++) y += IM[i + (size_t)c * P2] * X[c];
+      } else
+        y = X[i];
+      H[i] = y * cInvH[nn[k]];
+    }
+    __syncthreads();
+    // computeDev(oldH, H) over (n,m) pairs (expcenter.cpp:1270-1328); oldH staged in X
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * ST;
+      if (nn[k] < 0) continue;
+      X[i] = oldH[k];
+    }
+    __syncthreads();
+    double sum = 0.0;
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * ST;
+      if (nn[k] < 0) continue;
+      const int n = nn[k], mm = i - n * n;
+      if (mm > 0 && (mm % 2) == 0) continue;
+      double ar = X[i], ai = (mm > 0) ? X[i + 1] : 0.0;
+      double br = H[i], bi = (mm > 0) ? H[i + 1] : 0.0;
+      double s;
+      const double PL = 1e-30;
+      if (fabs(ar) < PL && fabs(ai) < PL) {
+        if (fabs(br) < PL && fabs(bi) < PL) continue;
+        s = 1.0;
+      } else if (fabs(br) < PL && fabs(bi) < PL) {
+        s = 1.0;
+      } else {
+        double dr = ar - br, di = ai - bi;
+        double top = dr * dr + di * di;
+        double bot = ar * ar + ai * ai + br * br + bi * bi;
+        s = (mm != 0) ? 2 * top / bot : top / bot;
+      }
+      sum += s;
+    }
+    sum = block_sum(sum, red);
+    dev = sum / (4 * p * p);
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * ST;
+      if (nn[k] < 0) continue;
+      oldH[k] = H[i];
+    }
+    ct++;
+    __syncthreads();
+  }
+  // computeDev2(outerOldH, H) (expcenter.cpp:1336-1370)
+  double td = 0.0, ts = 0.0;
+  for (int k = 0; k < KPT; k++) {
+    const int i = tid + k * ST;
+    if (nn[k] < 0) continue;
+    const int mm = i - nn[k] * nn[k];
+    double a = outerH[k], b = H[i];
+    double w = (mm == 0) ? 1.0 : 2.0;
+    td += w * ((a - b) * (a - b));
+    ts += w * (a * a + b * b);
+  }
+  td = block_sum(td, red);
+  ts = block_sum(ts, red);
+  for (int k = 0; k < KPT; k++) {
+    const int i = tid + k * ST;
+    if (nn[k] < 0) continue;
+    A.F[(size_t)g * P2 + i] = F[i];
+    A.H[(size_t)g * P2 + i] = H[i];
+  }
+  if (tid == 0) {
+    A.dev[g] = td / (4 * ts) / 1.0;
+    if (A.sweeps) A.sweeps[g] = ct;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// exposed surface charges (computeExposedSurfaceChargesFH expcenter.cpp:1420-1440)
+// ----------------------------------------------------------------------------
+struct QArgs {
+  const int *sph;
+  const double *xyz;            // exposed points, relative to the sphere centre
+  const int *pt_off, *pt_cnt, *nspx;
+  const double *consts;         // [sphere][10][p] (cChgH at row 9)
+  const double *F, *H;
+  double *qF, *qH, *TQF, *TQH;
+  int p;
+};
+__global__ void __launch_bounds__(ST) k_surface(Tabs tb, QArgs A) {
+  extern __shared__ double sh[];
+  const int p = A.p, P2 = p * p, tid = threadIdx.x;
+  const int g = A.sph[blockIdx.x];
+  double *Fs = sh, *Hs = Fs + P2, *red = Hs + P2;
+  const double *cChgH = A.consts + (size_t)g * 10 * p + 9 * p;
+  for (int i = tid; i < P2; i += ST) {
+    int n = (int)sqrt((double)i);
+    while (n * n > i) n--;
+    while ((n + 1) * (n + 1) <= i) n++;
+    Fs[i] = A.F[(size_t)g * P2 + i];
+    Hs[i] = A.H[(size_t)g * P2 + i];
+  }
+  __syncthreads();
+  const int off = A.pt_off[g], cnt = A.pt_cnt[g];
+  const double dA = 4 * M_PI / double(A.nspx[g]);
+  double tF = 0.0, tH = 0.0;
+  for (int h = tid; h < cnt; h += ST) {
+    const double *x = A.xyz + 3 * (size_t)(off + h);
+    double rho, th, ph;
+    cart_to_sph(x[0], x[1], x[2], rho, th, ph);
+    const double cost = cos(th), sint = sin(th);
+    double sF = 0.0, sH = 0.0;
+    for (int m = 0; m < p; m++) {
+      sh_column(tb, p, m, cost, sint, ph, [&](int n, double re, double im) {
+        const double c2 = (2 * n + 1) / (4.0 * M_PI), ch = cChgH[n];
+        if (m == 0) {
+          sF += Fs[n * n] * (re * c2);
+          sH += Hs[n * n] * (re * ch);
+        } else {
+          const int i = n * n + 2 * m - 1;
+          sF += 2.0 * (Fs[i] * (re * c2) + Fs[i + 1] * (im * c2));
+          sH += 2.0 * (Hs[i] * (re * ch) + Hs[i + 1] * (im * ch));
+        }
+      });
+    }
+    const double qf = sF * dA, qh = sH * dA;
+    A.qF[off + h] = qf;
+    A.qH[off + h] = qh;
+    tF += fabs(qf);
+    tH += fabs(qh);
+  }
+  tF = block_sum(tF, red);
+  tH = block_sum(tH, red);
+  if (tid == 0) {
+    A.TQF[g] = tF;
+    A.TQH[g] = tH;
+  }
+}
+
+// max over listed spheres of dev; also returns the sum
+__global__ void __launch_bounds__(256) k_maxdev(const int *sph, int n, const double *dev, double *out) {
+  __shared__ double red[8];
+  double mx = 0.0, sm = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double d = dev[sph[i]];
+    mx = fmax(mx, d);
+    sm += d;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, o, 64));
+    sm += __shfl_xor(sm, o, 64);
+  }
+  if (threadIdx.x % 64 == 0) {
+    red[threadIdx.x / 64] = mx;
+    red[4 + threadIdx.x / 64] = sm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    out[1] = red[4] + red[5] + red[6] + red[7];
+  }
+}
+
+// inter-molecular local expansion: LS = R^T (sum of slots in list order)
+struct LArgs {
+  const int *sph;        // target sphere per workgroup
+  const int *lst_off, *lst;
+  const double *slot;
+  const double *base;    // [sphere] starting value, or null (zero)
+  double *out;           // [sphere][p^2]
+  int p;
+};
+__global__ void __launch_bounds__(256) k_sumslots(LArgs A) {
+  const int p = A.p, P2 = p * p, g = A.sph[blockIdx.x];
+  const int l0 = A.lst_off[blockIdx.x], l1 = A.lst_off[blockIdx.x + 1];
+  for (int i = threadIdx.x; i < P2; i += 256) {
+    double s = A.base ? A.base[(size_t)g * P2 + i] : 0.0;
+    for (int e = l0; e < l1; e++) s += A.slot[(size_t)A.lst[e] * P2 + i];
+    A.out[(size_t)g * P2 + i] = s;
+  }
+}
+
+}  // namespace pbsam

@@ -1,0 +1,300 @@
+"""ctypes mirror of the pbsam_amd C ABI (include/pbsam_amd.h).
+
+The classes follow the reference's objects on this path: MolType ~ a molecule type
+set up by `pbsam spol` (CMolecule self-pol constructor + polarize_self,
+src/selfp_makemat.cpp:37-118), System ~ the BD drivers' vector<CMolecule*>
+(CMolecule::computeForces / polarize_mutual, src/moldynamics.cpp:554-587).
+All compute runs in libpbsam_amd.so; importing this module fails loudly if the
+library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(os.path.dirname(HERE), "libpbsam_amd.so")
+
+OK, ERR_BADARG, ERR_OVERLAP, ERR_NOCONV, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED, ERR_CHARGE, ERR_STATE = range(9)
+HROT_REFRESH = 1
+PMAX = 32
+
+D = ctypes.c_double
+I = ctypes.c_int
+P = ctypes.c_void_p
+
+SYMBOLS = [
+    "pbsam_ctx_create", "pbsam_ctx_destroy", "pbsam_last_error", "pbsam_kappa_from_salt",
+    "pbsam_moltype_create", "pbsam_moltype_destroy", "pbsam_moltype_info", "pbsam_moltype_points",
+    "pbsam_selfpol", "pbsam_selfpol_rounds", "pbsam_moltype_set_self", "pbsam_moltype_values",
+    "pbsam_moltype_get", "pbsam_moltype_get_intra", "pbsam_system_create", "pbsam_system_destroy",
+    "pbsam_system_set_pose", "pbsam_system_set_cutoffs", "pbsam_solve_mutual", "pbsam_mutual_rounds",
+    "pbsam_system_get", "pbsam_energy", "pbsam_system_counts", "pbsam_xform_h_batch",
+    "pbsam_xform_intra_batch", "pbsam_xform_num_batch", "pbsam_rotate_quat_batch",
+]
+
+
+class PbsamError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pbsam error {code}: {msg}")
+        self.code = code
+
+
+class MolDesc(ctypes.Structure):
+    _fields_ = [("n_sph", I), ("cen", P), ("rad", P), ("n_chg", I), ("chg_xyz", P),
+                ("chg_q", P), ("rcen", P), ("p", I), ("eps_in", D), ("npts", P),
+                ("npts_cap", I), ("intra_rcutoff", D), ("imats", P), ("imat_npts", I)]
+
+
+_LIB = None
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def load():
+    """Load libpbsam_amd.so (built by __graft_entry__.build / `make -C pb-sam_amd`)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(f"{LIBPATH} is missing: build it with `make -C pb-sam_amd`")
+    L = ctypes.CDLL(LIBPATH)
+    sig = {
+        "pbsam_ctx_create": ([D, D, I, P], I),
+        "pbsam_ctx_destroy": ([P], None),
+        "pbsam_last_error": ([P], ctypes.c_char_p),
+        "pbsam_kappa_from_salt": ([D, D, D], D),
+        "pbsam_moltype_create": ([P, ctypes.POINTER(MolDesc), P], I),
+        "pbsam_moltype_destroy": ([P], None),
+        "pbsam_moltype_info": ([P, P, P, P], I),
+        "pbsam_moltype_points": ([P, I, P, P, P], I),
+        "pbsam_selfpol": ([P, D, I, P], I),
+        "pbsam_selfpol_rounds": ([P, I, P], I),
+        "pbsam_moltype_set_self": ([P, P, P], I),
+        "pbsam_moltype_values": ([P], I),
+        "pbsam_moltype_get": ([P, I, P, P, P, P, P], I),
+        "pbsam_moltype_get_intra": ([P, I, P, P, P, P], I),
+        "pbsam_system_create": ([P, I, P, P, P, P], I),
+        "pbsam_system_destroy": ([P], None),
+        "pbsam_system_set_pose": ([P, I, P, P], I),
+        "pbsam_system_set_cutoffs": ([P, D, D], I),
+        "pbsam_solve_mutual": ([P, D, I, ctypes.c_uint, P], I),
+        "pbsam_mutual_rounds": ([P, I, ctypes.c_uint, P], I),
+        "pbsam_system_get": ([P, I, I, P, P, P, P], I),
+        "pbsam_energy": ([P, P], I),
+        "pbsam_system_counts": ([P, P, P], I),
+        "pbsam_xform_h_batch": ([P, I, I, P, P, P, P, P], I),
+        "pbsam_xform_intra_batch": ([P, I, I, P, P, P, P, P, P, P], I),
+        "pbsam_xform_num_batch": ([P, I, I, P, P, I, P, P, P, D, D, P, P], I),
+        "pbsam_rotate_quat_batch": ([P, I, I, P, I, P, P], I),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _LIB = L
+    return L
+
+
+def kappa_from_salt(salt_molar, eps_out=78.0, temp=298.15):
+    return load().pbsam_kappa_from_salt(salt_molar, eps_out, temp)
+
+
+class Context:
+    """One HIP device + stream with the physical constants (CSystem::initConstants)."""
+
+    def __init__(self, kappa, eps_out=78.0, device=0):
+        self.L = load()
+        h = ctypes.c_void_p()
+        rc = self.L.pbsam_ctx_create(kappa, eps_out, device, ctypes.byref(h))
+        if rc != OK:
+            raise PbsamError(rc, "context creation failed (no HIP device?)")
+        self.h = h.value
+        self.kappa = kappa
+
+    def check(self, rc):
+        if rc not in (OK,):
+            raise PbsamError(rc, self.L.pbsam_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pbsam_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # kernel-level batches
+    def xform_h(self, P3, a, order, blocks, p):
+        P3, a, blocks = f64(P3), f64(a), f64(blocks)
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        n = len(order)
+        out = np.zeros((n, p * p))
+        self.check(self.L.pbsam_xform_h_batch(self.h, n, p, _p(P3), _p(a), _p(order), _p(blocks), _p(out)))
+        return out
+
+    def xform_intra(self, P3, a, orders, H, F, p):
+        P3, a, H, F = f64(P3), f64(a), f64(H), f64(F)
+        orders = np.ascontiguousarray(orders, dtype=np.int32)
+        n = len(orders) // 2
+        oH, oF = np.zeros((n, p * p)), np.zeros((n, p * p))
+        self.check(self.L.pbsam_xform_intra_batch(self.h, n, p, _p(P3), _p(a), _p(orders), _p(H), _p(F),
+                                                  _p(oH), _p(oF)))
+        return oH, oF
+
+    def xform_num(self, P3, a, xyz, qF, qH, tQF, tQH, p):
+        P3, a, xyz, qF, qH = f64(P3), f64(a), f64(xyz), f64(qF), f64(qH)
+        n = len(P3) // 3
+        oF, oH = np.zeros((n, p * p)), np.zeros((n, p * p))
+        self.check(self.L.pbsam_xform_num_batch(self.h, n, p, _p(P3), _p(a), len(qF), _p(xyz), _p(qF), _p(qH),
+                                                tQF, tQH, _p(oF), _p(oH)))
+        return oF, oH
+
+    def rotate_quat(self, quats, fwd, blocks, p):
+        quats, blocks = f64(quats), f64(blocks)
+        n = len(quats) // 4
+        out = np.zeros((n, p * p))
+        self.check(self.L.pbsam_rotate_quat_batch(self.h, n, p, _p(quats), int(fwd), _p(blocks), _p(out)))
+        return out
+
+
+class MolType:
+    """Molecule type on the device: spheres, charges, surface points, self-pol state."""
+
+    def __init__(self, ctx, cen, rad, chg_xyz, chg_q, p, eps_in=4.0, rcen=None, npts=None, npts_cap=10000,
+                 intra_rcutoff=30.0, imats=None, imat_npts=0):
+        self.ctx = ctx
+        L = ctx.L
+        keep = [f64(cen).reshape(-1), f64(rad), f64(chg_xyz).reshape(-1), f64(chg_q),
+                f64(np.zeros(3) if rcen is None else rcen)]
+        cen_a, rad_a, cx, cq, rc = keep
+        npts_a = None if npts is None else np.ascontiguousarray(npts, dtype=np.int32)
+        im = None if imats is None else f64(imats)
+        self._keep = keep + [npts_a, im]
+        d = MolDesc(len(rad_a), _p(cen_a), _p(rad_a), len(cq), _p(cx), _p(cq), _p(rc), p, eps_in, _p(npts_a),
+                    npts_cap, intra_rcutoff, _p(im), imat_npts)
+        h = ctypes.c_void_p()
+        ctx.check(L.pbsam_moltype_create(ctx.h, ctypes.byref(d), ctypes.byref(h)))
+        self.h = h.value
+        self.p = p
+        self.ns = len(rad_a)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.pbsam_moltype_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def points(self, k):
+        ne, nt = ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(self.ctx.L.pbsam_moltype_points(self.h, k, None, ctypes.byref(ne), ctypes.byref(nt)))
+        out = np.zeros(3 * ne.value)
+        self.ctx.check(self.ctx.L.pbsam_moltype_points(self.h, k, _p(out), None, None))
+        return out.reshape(-1, 3), nt.value
+
+    def selfpol(self, tol=1e-24, max_rounds=400):
+        r = ctypes.c_int()
+        rc = self.ctx.L.pbsam_selfpol(self.h, tol, max_rounds, ctypes.byref(r))
+        if rc not in (OK, ERR_NOCONV):
+            self.ctx.check(rc)
+        return r.value, rc
+
+    def rounds(self, n):
+        md = ctypes.c_double()
+        self.ctx.check(self.ctx.L.pbsam_selfpol_rounds(self.h, n, ctypes.byref(md)))
+        return md.value
+
+    def set_self(self, F, H):
+        F, H = f64(F), f64(H)
+        self.ctx.check(self.ctx.L.pbsam_moltype_set_self(self.h, _p(F), _p(H)))
+
+    def values(self):
+        self.ctx.check(self.ctx.L.pbsam_moltype_values(self.h))
+
+    def get(self, k):
+        p = self.p
+        pts, _ = self.points(k)
+        n = len(pts)
+        F, H = np.zeros(p * p), np.zeros(p * p)
+        qF, qH, tq = np.zeros(max(n, 1)), np.zeros(max(n, 1)), np.zeros(3)
+        self.ctx.check(self.ctx.L.pbsam_moltype_get(self.h, k, _p(F), _p(H), _p(qF), _p(qH), _p(tq)))
+        return dict(F=F, H=H, qF=qF[:n], qH=qH[:n], TQF=tq[0], TQH=tq[1], dev=tq[2])
+
+    def FH(self):
+        g = [self.get(k) for k in range(self.ns)]
+        return np.stack([x["F"] for x in g]), np.stack([x["H"] for x in g])
+
+    def intra(self, k):
+        a = [np.zeros(self.p * self.p) for _ in range(4)]
+        self.ctx.check(self.ctx.L.pbsam_moltype_get_intra(self.h, k, *[_p(x) for x in a]))
+        return a
+
+
+class System:
+    """Rigid molecules with poses; mutual polarization and energies on the device."""
+
+    def __init__(self, ctx, moltypes, pos, quat):
+        self.ctx = ctx
+        self.mts = list(moltypes)
+        arr = (ctypes.c_void_p * len(moltypes))(*[m.h for m in moltypes])
+        self._arr = arr
+        pos, quat = f64(pos).reshape(-1), f64(quat).reshape(-1)
+        h = ctypes.c_void_p()
+        ctx.check(ctx.L.pbsam_system_create(ctx.h, len(moltypes), ctypes.cast(arr, P), _p(pos), _p(quat),
+                                            ctypes.byref(h)))
+        self.h = h.value
+        self.n = len(moltypes)
+        self.p = moltypes[0].p
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.pbsam_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_cutoffs(self, inter, interact):
+        self.ctx.check(self.ctx.L.pbsam_system_set_cutoffs(self.h, inter, interact))
+
+    def set_pose(self, i, pos, quat):
+        pos, quat = f64(pos), f64(quat)
+        self.ctx.check(self.ctx.L.pbsam_system_set_pose(self.h, i, _p(pos), _p(quat)))
+
+    def solve(self, tol=1e-24, max_rounds=400, refresh=False):
+        r = ctypes.c_int()
+        rc = self.ctx.L.pbsam_solve_mutual(self.h, tol, max_rounds, HROT_REFRESH if refresh else 0,
+                                           ctypes.byref(r))
+        if rc not in (OK, ERR_NOCONV):
+            self.ctx.check(rc)
+        return r.value, rc
+
+    def rounds(self, n, refresh=False):
+        md = ctypes.c_double()
+        rc = self.ctx.L.pbsam_mutual_rounds(self.h, n, HROT_REFRESH if refresh else 0, ctypes.byref(md))
+        self.ctx.check(rc)
+        return md.value
+
+    def get(self, i, k):
+        a = [np.zeros(self.p * self.p) for _ in range(4)]
+        self.ctx.check(self.ctx.L.pbsam_system_get(self.h, i, k, *[_p(x) for x in a]))
+        return dict(F=a[0], H=a[1], LS=a[2], Hrot=a[3])
+
+    def energy(self):
+        e = np.zeros(self.n)
+        self.ctx.check(self.ctx.L.pbsam_energy(self.h, _p(e)))
+        return e
+
+    def counts(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(self.ctx.L.pbsam_system_counts(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
