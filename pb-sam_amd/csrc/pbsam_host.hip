@@ -244,9 +244,9 @@ static void launch_rotcoeff(pbsam_ctx *c, int n, RJob *jobs, int p, double *out)
   HIPCHK(hipGetLastError());
 }
 static void launch_rotapply(pbsam_ctx *c, int n, const RAJob *jobs, const RJob *rj, const double *Rst,
-                            const double *in, double *out, int p) {
+                            const double *in, double *out, int p, const int *sel = nullptr) {
   if (!n) return;
-  hipLaunchKernelGGL(k_rotapply, dim3(n), dim3(256), 0, c->stream, jobs, rj, Rst, in, out, p);
+  hipLaunchKernelGGL(k_rotapply, dim3(n), dim3(256), 0, c->stream, jobs, sel, rj, Rst, in, out, p);
   HIPCHK(hipGetLastError());
 }
 static void launch_sumslots(pbsam_ctx *c, int n, const LArgs &a) {
@@ -357,6 +357,13 @@ struct pbsam_moltype {
   DBuf<double> d_Tfar;
   // index of the analytical job of (target ki, near entry e), -1 for numerical pairs
   std::vector<std::vector<int>> ana;
+  // near entries per target (source, analytical job or numerical job) and by source
+  struct Entry {
+    int src, xj, pj;
+  };
+  std::vector<std::vector<Entry>> ent;
+  std::vector<std::vector<std::pair<int, int>>> bysrc;  // source -> (target, entry index)
+  DBuf<int> d_selx, d_selp;
 };
 
 static void moltype_setup(pbsam_moltype *m, const pbsam_moltype_desc *d) {
@@ -537,6 +544,8 @@ static void moltype_jobs(pbsam_moltype *m) {
   int slot = 0;
   const long long TS = tmat_size(p);
   m->ana.assign(ns, {});
+  m->ent.assign(ns, {});
+  m->bysrc.assign(ns, {});
   for (int ki = 0; ki < ns; ki++) {
     const SphHost &Si = m->sph[ki];
     for (int kj : Si.near) {
@@ -545,6 +554,9 @@ static void moltype_jobs(pbsam_moltype *m) {
       double rho = sqrt(dot3(P, P));
       double sep = rho - Si.rad - Sj.rad;
       m->ana[ki].push_back((sep > 0 && !(sep < 5.0)) ? (int)xj.size() : -1);
+      if (sep > 0 && !(sep < 5.0)) m->ent[ki].push_back({kj, (int)xj.size(), -1});
+      else m->ent[ki].push_back({kj, -1, (int)pj.size()});
+      m->bysrc[kj].push_back({ki, (int)m->ent[ki].size() - 1});
       if (sep > 0 && !(sep < 5.0)) {
         XJob J{};
         J.src = kj;
@@ -674,9 +686,141 @@ static double moltype_round(pbsam_moltype *m) {
   return md[0];
 }
 
+// One Gauss-Seidel pass over the recompute set S (ascending), in the reference's order
+// (recompute_LowMemory for ki = 0..N-1, molecule.cpp:1946-1958 / 2048-2067): a target reads
+// sources updated earlier in the pass, and the round-start value of every other source.
+// Round-start transforms run in one batch; after each sphere's update the transforms from
+// it to the later targets of the pass run as one batch.
+struct GSJobs {
+  std::vector<int> selx, selp;        // pre batch first, then one segment per step
+  std::vector<int> offx, offp;        // segment offsets: [0] pre, [1+i] after step i
+};
+template <class EntFn, class BySrcFn>
+static GSJobs gs_plan(const std::vector<int> &S, int nsph_ids, EntFn ent_of, BySrcFn bysrc_of) {
+  GSJobs J;
+  std::vector<int> pos(nsph_ids, -1);
+  for (size_t i = 0; i < S.size(); i++) pos[S[i]] = (int)i;
+  for (size_t i = 0; i < S.size(); i++)
+    for (auto &e : ent_of(S[i]))
+      if (!(pos[e.src] >= 0 && pos[e.src] < (int)i)) {
+        if (e.xj >= 0) J.selx.push_back(e.xj);
+        else J.selp.push_back(e.pj);
+      }
+  J.offx.push_back(0);
+  J.offp.push_back(0);
+  J.offx.push_back((int)J.selx.size());
+  J.offp.push_back((int)J.selp.size());
+  for (size_t i = 0; i < S.size(); i++) {
+    for (auto &te : bysrc_of(S[i])) {
+      int u = te.first;
+      if (pos[u] > (int)i) {
+        auto &e = ent_of(u)[te.second];
+        if (e.xj >= 0) J.selx.push_back(e.xj);
+        else J.selp.push_back(e.pj);
+      }
+    }
+    J.offx.push_back((int)J.selx.size());
+    J.offp.push_back((int)J.selp.size());
+  }
+  return J;
+}
+
 static void dcopy(pbsam_ctx *c, DBuf<double> &dst, const DBuf<double> &src) {
   dst.alloc(src.n);
   if (src.n) HIPCHK(hipMemcpyAsync(dst.p, src.p, sizeof(double) * src.n, hipMemcpyDeviceToDevice, c->stream));
+}
+
+// self-polarization round in the reference order over the recompute set S
+static void moltype_gs_pass(pbsam_moltype *m, const std::vector<int> &S) {
+  pbsam_ctx *c = m->c;
+  hipStream_t st = c->stream;
+  const int p = m->p;
+  GSJobs J = gs_plan(S, m->ns, [&](int t) -> const std::vector<pbsam_moltype::Entry> & { return m->ent[t]; },
+                     [&](int t) -> const std::vector<std::pair<int, int>> & { return m->bysrc[t]; });
+  std::vector<int> sx = J.selx, sp = J.selp;
+  if (sx.empty()) sx.push_back(0);
+  if (sp.empty()) sp.push_back(0);
+  m->d_selx.upload(sx, st);
+  m->d_selp.upload(sp, st);
+  XArgs xa{};
+  xa.jobs = m->d_xj.p;
+  xa.H = m->d_H.p;
+  xa.F = m->d_F.p;
+  xa.TQH = m->d_TQH.p;
+  xa.TQF = m->d_TQF.p;
+  xa.slotH = m->d_slotH.p;
+  xa.slotF = m->d_slotF.p;
+  xa.p = p;
+  PArgs pa{};
+  pa.jobs = m->d_pj.p;
+  pa.xyz = m->d_pts.p;
+  pa.q1 = m->d_qF.p;
+  pa.q2 = m->d_qH.p;
+  pa.TQ1 = m->d_TQF.p;
+  pa.TQ2 = m->d_TQH.p;
+  pa.out1 = m->d_slotF.p;
+  pa.out2 = m->d_slotH.p;
+  pa.kap1 = 0;
+  pa.kap2 = 1;
+  pa.p = p;
+  SArgs sa{};
+  sa.lst = m->d_lst.p;
+  sa.slotH = m->d_slotH.p;
+  sa.slotF = m->d_slotF.p;
+  sa.consts = m->d_consts.p;
+  sa.Dfix = m->d_Dfix.p;
+  sa.Vfix = m->d_Vfix.p;
+  sa.imat = m->d_imat.p;
+  sa.F = m->d_F.p;
+  sa.H = m->d_H.p;
+  sa.LFS = m->d_LFS.p;
+  sa.LHS = m->d_LHS.p;
+  sa.dev = m->d_dev.p;
+  sa.p = p;
+  QArgs qa{m->d_all.p, m->d_pts.p, m->d_pt_off.p, m->d_pt_cnt.p, m->d_nspx.p, m->d_consts.p,
+           m->d_F.p, m->d_H.p, m->d_qF.p, m->d_qH.p, m->d_TQF.p, m->d_TQH.p, p};
+  for (size_t seg = 0; seg <= S.size(); seg++) {
+    if (seg > 0) {
+      const int t = S[seg - 1];
+      sa.sph = m->d_all.p + t;
+      sa.lst_off = m->d_lst_off.p + t;
+      launch_solve(c, 1, sa);
+      qa.sph = m->d_all.p + t;
+      launch_surface(c, 1, qa);
+    }
+    xa.sel = m->d_selx.p + J.offx[seg];
+    launch_xform(c, J.offx[seg + 1] - J.offx[seg], xa);
+    pa.sel = m->d_selp.p + J.offp[seg];
+    launch_p2l(c, J.offp[seg + 1] - J.offp[seg], pa);
+  }
+}
+
+// CMolecule::polarize_self (molecule.cpp:1899-1979): reference order, selective recompute
+// (dev > 0.1 maxdev or every 5th round), stop when maxdev <= tol (stock: 1e-4).
+static int moltype_selfpol_ref(pbsam_moltype *m, double tol, int max_rounds, double *maxdev_out) {
+  const int ns = m->ns;
+  std::vector<int> all(ns);
+  for (int k = 0; k < ns; k++) all[k] = k;
+  if (tol <= 0) tol = (ns == 1) ? 1e-2 : 1e-2 * 1e-2;
+  if (max_rounds <= 0) max_rounds = 1700;
+  moltype_gs_pass(m, all);
+  std::vector<double> dev = m->d_dev.download(m->c->stream);
+  double maxdev = 0.0;
+  for (double d : dev) maxdev = std::max(maxdev, d);
+  int ct = 1;
+  while (maxdev > tol && ct < max_rounds) {
+    std::vector<int> S;
+    for (int k = 0; k < ns; k++)
+      if (ns == 1 || dev[k] > 0.1 * maxdev || ct % 5 == 0) S.push_back(k);
+    moltype_gs_pass(m, S);
+    dev = m->d_dev.download(m->c->stream);
+    double mnew = 0.0;
+    for (double d : dev) mnew = std::max(mnew, d);
+    maxdev = mnew;
+    ct++;
+  }
+  if (maxdev_out) *maxdev_out = maxdev;
+  return ct;
 }
 
 static void moltype_store_self(pbsam_moltype *m) {
@@ -761,7 +905,7 @@ static void moltype_values(pbsam_moltype *m) {
   launch_xform(c, (int)fj.size(), xa);
   m->d_LFfar.alloc((size_t)ns * P2);
   m->d_LHfar.alloc((size_t)ns * P2);
-  LArgs la{m->d_all.p, dfoff.p, dflst.p, fsH.p, nullptr, m->d_LHfar.p, p};
+  LArgs la{m->d_all.p, dfoff.p, dflst.p, fsH.p, nullptr, m->d_LHfar.p, nullptr, p};
   launch_sumslots(c, ns, la);
   la.slot = fsF.p;
   la.out = m->d_LFfar.p;
@@ -786,7 +930,7 @@ static void moltype_values(pbsam_moltype *m) {
   launch_p2l(c, m->n_pj, pa);
   m->d_LFintra.alloc((size_t)ns * P2);
   m->d_LHintra.alloc((size_t)ns * P2);
-  LArgs lb{m->d_all.p, m->d_lst_off.p, m->d_lst.p, m->d_slotH.p, m->d_LHfar.p, m->d_LHintra.p, p};
+  LArgs lb{m->d_all.p, m->d_lst_off.p, m->d_lst.p, m->d_slotH.p, m->d_LHfar.p, m->d_LHintra.p, nullptr, p};
   launch_sumslots(c, ns, lb);
   lb.slot = m->d_slotF.p;
   lb.base = m->d_LFfar.p;
@@ -842,6 +986,14 @@ struct pbsam_system {
   DBuf<RAJob> d_ra_self, d_ra_ls, d_ra_h;
   int n_ra_self = 0;
   bool lists_ok = false;
+  // reference-order schedule: near entries per polarized sphere (global ids), by source
+  std::vector<std::vector<pbsam_moltype::Entry>> ent;        // [global sphere]
+  std::vector<std::vector<std::pair<int, int>>> bysrc;        // [global source] -> (target, entry)
+  std::vector<int> pidx;                                      // global sphere -> pol index (-1)
+  std::vector<int> ioff;                                      // inter jobs of pol index k
+  std::vector<int> mol_pol_begin;                             // pol index range per molecule
+  DBuf<int> d_iota, d_selx, d_selp, d_seli, d_selk;
+  double itot = 0.0;
 };
 
 static void system_static(pbsam_system *S) {
@@ -994,6 +1146,10 @@ static int system_lists(pbsam_system *S) {
   }
   // polarized spheres and their jobs
   S->pol.clear();
+  S->ent.assign(S->nsph, {});
+  S->bysrc.assign(S->nsph, {});
+  S->pidx.assign(S->nsph, -1);
+  S->mol_pol_begin.assign(nmol + 1, 0);
   std::vector<XJob> xi, xa;
   std::vector<PJob> pa;
   std::vector<TJob> ti;
@@ -1003,10 +1159,12 @@ static int system_lists(pbsam_system *S) {
   for (int i = 0; i < nmol; i++) {
     MolInst &Mi = S->mols[i];
     pbsam_moltype *mt = Mi.mt;
+    S->mol_pol_begin[i] = (int)S->pol.size();
     if (!Mi.bInterXForm) continue;
     for (int ki = 0; ki < mt->ns; ki++) {
       if (Mi.interPol[ki].empty()) continue;
       const int g = Mi.base + ki;
+      S->pidx[g] = (int)S->pol.size();
       S->pol.push_back(g);
       const double *mci = &Mi.cenRot[3 * ki];
       double ci[3] = {Mi.rcen[0] + mci[0], Mi.rcen[1] + mci[1], Mi.rcen[2] + mci[2]};
@@ -1053,6 +1211,7 @@ static int system_lists(pbsam_system *S) {
           const int idx = mt->ana[ki][e];  // shares the molecule type's translation matrices
           J.TH = mt->d_T.p + (size_t)(2 * idx) * TS;
           J.TF = mt->d_T.p + (size_t)(2 * idx + 1) * TS;
+          S->ent[g].push_back({Mi.base + kj, (int)xa.size(), -1});
           xa.push_back(J);
         } else {
           PJob J{};
@@ -1063,12 +1222,22 @@ static int system_lists(pbsam_system *S) {
           for (int x = 0; x < 3; x++) J.P[x] = P[x];
           J.a_src = Sj.rad;
           J.a_tgt = Si.rad;
+          S->ent[g].push_back({Mi.base + kj, -1, (int)pa.size()});
           pa.push_back(J);
         }
+        S->bysrc[Mi.base + kj].push_back({g, (int)S->ent[g].size() - 1});
         alst.push_back(aslot++);
       }
       aoff.push_back((int)alst.size());
     }
+  }
+  S->mol_pol_begin[nmol] = (int)S->pol.size();
+  S->ioff = ioff;
+  S->itot = S->pol.empty() ? 0.0 : 1.0 / (double)S->pol.size();
+  {
+    std::vector<int> iota(std::max<size_t>(std::max<size_t>(S->pol.size(), S->nsph), 1));
+    for (size_t k = 0; k < iota.size(); k++) iota[k] = (int)k;
+    S->d_iota.upload(iota, s);
   }
   S->n_inter = (int)xi.size();
   S->n_xintra = (int)xa.size();
@@ -1141,7 +1310,7 @@ static double system_round(pbsam_system *S, bool round1, bool refresh) {
   xa.slotH = S->d_slotI.p;
   xa.p = p;
   launch_xform(c, S->n_inter, xa);
-  LArgs la{S->d_pol.p, S->d_inter_off.p, S->d_inter_lst.p, S->d_slotI.p, nullptr, S->d_LSrot.p, p};
+  LArgs la{S->d_pol.p, S->d_inter_off.p, S->d_inter_lst.p, S->d_slotI.p, nullptr, S->d_LSrot.p, nullptr, p};
   launch_sumslots(c, np, la);
   launch_rotapply(c, np, S->d_ra_ls.p, S->d_rj.p, S->d_Rst.p, S->d_LSrot.p, S->d_LS.p, p);
   SArgs sa{};
@@ -1202,6 +1371,139 @@ static double system_round(pbsam_system *S, bool round1, bool refresh) {
   HIPCHK(hipGetLastError());
   std::vector<double> md = S->d_md.download(c->stream);
   return md[0];
+}
+
+static void system_phase0(pbsam_system *S);
+static double system_round(pbsam_system *S, bool round1, bool refresh);
+
+// mutual rounds >= 2 in the reference order (molecule.cpp:2042-2075): per molecule, spheres
+// whose last change exceeds 0.1 MAXDEV are recomputed in ascending order.
+static void system_gs_round(pbsam_system *S, const std::vector<double> &dev, double MAXDEV, bool refresh) {
+  pbsam_ctx *c = S->c;
+  hipStream_t st = c->stream;
+  const int p = S->p;
+  for (int i = 0; i < S->nmol; i++) {
+    std::vector<int> Sk, Sg;
+    for (int k = S->mol_pol_begin[i]; k < S->mol_pol_begin[i + 1]; k++)
+      if (dev[S->pol[k]] > 0.1 * MAXDEV) {
+        Sk.push_back(k);
+        Sg.push_back(S->pol[k]);
+      }
+    if (Sk.empty()) continue;
+    // LS from the partners' rotated H (orders from their current totals)
+    std::vector<int> seli;
+    for (int k : Sk)
+      for (int j = S->ioff[k]; j < S->ioff[k + 1]; j++) seli.push_back(j);
+    if (seli.empty()) seli.push_back(0);
+    S->d_seli.upload(seli, st);
+    S->d_selk.upload(Sk, st);
+    XArgs xa{};
+    xa.jobs = S->d_xinter.p;
+    xa.H = S->d_Hrot.p;
+    xa.TQH = S->d_TQH.p;
+    xa.slotH = S->d_slotI.p;
+    xa.sel = S->d_seli.p;
+    xa.p = p;
+    int nsel = 0;
+    for (int k : Sk) nsel += S->ioff[k + 1] - S->ioff[k];
+    launch_xform(c, nsel, xa);
+    LArgs la{S->d_pol.p, S->d_inter_off.p, S->d_inter_lst.p, S->d_slotI.p, nullptr, S->d_LSrot.p, S->d_selk.p, p};
+    launch_sumslots(c, (int)Sk.size(), la);
+    launch_rotapply(c, (int)Sk.size(), S->d_ra_ls.p, S->d_rj.p, S->d_Rst.p, S->d_LSrot.p, S->d_LS.p, p, S->d_selk.p);
+    // intra-molecular chain
+    GSJobs J = gs_plan(Sg, S->nsph, [&](int g) -> const std::vector<pbsam_moltype::Entry> & { return S->ent[g]; },
+                       [&](int g) -> const std::vector<std::pair<int, int>> & { return S->bysrc[g]; });
+    std::vector<int> sx = J.selx, sp = J.selp;
+    if (sx.empty()) sx.push_back(0);
+    if (sp.empty()) sp.push_back(0);
+    S->d_selx.upload(sx, st);
+    S->d_selp.upload(sp, st);
+    XArgs xb{};
+    xb.jobs = S->d_xintra.p;
+    xb.H = S->d_H.p;
+    xb.F = S->d_F.p;
+    xb.TQH = S->d_TQH.p;
+    xb.TQF = S->d_TQF.p;
+    xb.slotH = S->d_slotH.p;
+    xb.slotF = S->d_slotF.p;
+    xb.p = p;
+    PArgs pa{};
+    pa.jobs = S->d_pintra.p;
+    pa.xyz = S->d_pts.p;
+    pa.q1 = S->d_qF.p;
+    pa.q2 = S->d_qH.p;
+    pa.TQ1 = S->d_TQF.p;
+    pa.TQ2 = S->d_TQH.p;
+    pa.out1 = S->d_slotF.p;
+    pa.out2 = S->d_slotH.p;
+    pa.kap1 = 0;
+    pa.kap2 = 1;
+    pa.p = p;
+    SArgs sa{};
+    sa.lst = S->d_intra_lst.p;
+    sa.slotH = S->d_slotH.p;
+    sa.slotF = S->d_slotF.p;
+    sa.LS = S->d_LS.p;
+    sa.farH = S->d_LHfar.p;
+    sa.farF = S->d_LFfar.p;
+    sa.consts = S->d_consts.p;
+    sa.Dfix = S->d_Dfix.p;
+    sa.Vfix = S->d_Vfix.p;
+    sa.imat = S->d_imat.p;
+    sa.F = S->d_F.p;
+    sa.H = S->d_H.p;
+    sa.LFS = S->d_LFS.p;
+    sa.LHS = S->d_LHS.p;
+    sa.dev = S->d_dev.p;
+    sa.p = p;
+    QArgs qa{S->d_pol.p, S->d_pts.p, S->d_pt_off.p, S->d_pt_cnt.p, S->d_nspx.p, S->d_consts.p,
+             S->d_F.p, S->d_H.p, S->d_qF.p, S->d_qH.p, S->d_TQF.p, S->d_TQH.p, p};
+    for (size_t seg = 0; seg <= Sk.size(); seg++) {
+      if (seg > 0) {
+        const int k = Sk[seg - 1];
+        sa.sph = S->d_pol.p + k;
+        sa.lst_off = S->d_intra_off.p + k;
+        launch_solve(c, 1, sa);
+        qa.sph = S->d_pol.p + k;
+        launch_surface(c, 1, qa);
+        if (refresh)
+          launch_rotapply(c, 1, S->d_ra_h.p, S->d_rj.p, S->d_Rst.p, S->d_H.p, S->d_Hrot.p, p, S->d_iota.p + k);
+      }
+      xb.sel = S->d_selx.p + J.offx[seg];
+      launch_xform(c, J.offx[seg + 1] - J.offx[seg], xb);
+      pa.sel = S->d_selp.p + J.offp[seg];
+      launch_p2l(c, J.offp[seg + 1] - J.offp[seg], pa);
+    }
+  }
+}
+
+// CMolecule::polarize_mutual rounds (molecule.cpp:2007-2075) in the reference order;
+// stop when the mean change <= tol (stock 1e-4) or after max_rounds (stock 1000).
+static int system_mutual_ref(pbsam_system *S, double tol, int max_rounds, bool refresh, double *crit_out) {
+  if (tol <= 0) tol = 1e-2 * 1e-2;
+  if (max_rounds <= 0) max_rounds = 1000;
+  system_phase0(S);
+  system_round(S, true, refresh);
+  std::vector<double> dev = S->d_dev.download(S->c->stream);
+  double total = 0.0, MAXDEV = 0.0;
+  for (int g : S->pol) {
+    total += dev[g];
+    MAXDEV = std::max(MAXDEV, dev[g]);
+  }
+  int ct = 1;
+  while (total * S->itot > tol && ct < max_rounds) {
+    system_gs_round(S, dev, MAXDEV, refresh);
+    dev = S->d_dev.download(S->c->stream);
+    total = 0.0;
+    MAXDEV = 0.0;
+    for (int g : S->pol) {
+      total += dev[g];
+      MAXDEV = std::max(MAXDEV, dev[g]);
+    }
+    ct++;
+  }
+  if (crit_out) *crit_out = total * S->itot;
+  return ct;
 }
 
 // phase 0 of polarize_mutual: molecule rotations and HselfRot
@@ -1332,17 +1634,14 @@ int pbsam_selfpol_rounds(pbsam_moltype *m, int rounds, double *maxdev_out) {
 }
 
 int pbsam_selfpol(pbsam_moltype *m, double tol, int max_rounds, int *rounds_out) {
-  if (!m || max_rounds < 1) return PBSAM_ERR_BADARG;
+  if (!m) return PBSAM_ERR_BADARG;
   return guarded(&m->c->err, [&]() {
-    double md = 1e300;
-    int r = 0;
-    while (r < max_rounds && md > tol) {
-      md = moltype_round(m);
-      r++;
-    }
+    double md = 0.0;
+    int r = moltype_selfpol_ref(m, tol, max_rounds, &md);
     if (rounds_out) *rounds_out = r;
     moltype_store_self(m);
-    return md > tol ? PBSAM_ERR_NOCONV : PBSAM_OK;
+    double t = tol > 0 ? tol : (m->ns == 1 ? 1e-2 : 1e-4);
+    return md > t ? PBSAM_ERR_NOCONV : PBSAM_OK;
   });
 }
 
@@ -1470,7 +1769,7 @@ int pbsam_mutual_rounds(pbsam_system *S, int rounds, unsigned flags, double *max
 }
 
 int pbsam_solve_mutual(pbsam_system *S, double tol, int max_rounds, unsigned flags, int *rounds_out) {
-  if (!S || max_rounds < 1) return PBSAM_ERR_BADARG;
+  if (!S) return PBSAM_ERR_BADARG;
   return guarded(&S->c->err, [&]() {
     int rc = system_lists(S);
     if (rc != PBSAM_OK) return rc;
@@ -1478,16 +1777,11 @@ int pbsam_solve_mutual(pbsam_system *S, double tol, int max_rounds, unsigned fla
       if (rounds_out) *rounds_out = 0;
       return PBSAM_OK;
     }
-    system_phase0(S);
     bool refresh = (flags & PBSAM_HROT_REFRESH) != 0;
-    double md = system_round(S, true, refresh);
-    int r = 1;
-    while (md > tol && r < max_rounds) {
-      md = system_round(S, false, refresh);
-      r++;
-    }
+    double crit = 0.0;
+    int r = system_mutual_ref(S, tol, max_rounds, refresh, &crit);
     if (rounds_out) *rounds_out = r;
-    return md > tol ? PBSAM_ERR_NOCONV : PBSAM_OK;
+    return crit > (tol > 0 ? tol : 1e-4) ? PBSAM_ERR_NOCONV : PBSAM_OK;
   });
 }
 

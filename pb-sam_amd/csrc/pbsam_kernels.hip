@@ -140,13 +140,14 @@ struct XArgs {
   const double *TQF;
   double *slotH, *slotF;
   const int *ordH, *ordF;  // explicit orders per job (null: adaptive from the totals)
+  const int *sel;          // job indices to run (null: job = workgroup index)
   int p;
   int full;            // 1: use order p for both blocks (no adaptive truncation)
 };
 
 __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
   extern __shared__ double sh[];
-  const XJob J = A.jobs[blockIdx.x];
+  const XJob J = A.jobs[A.sel ? A.sel[blockIdx.x] : blockIdx.x];
   const int p = A.p, P2 = p * p;
   const int tid = threadIdx.x;
   const bool intra = (J.kind == 1);
@@ -429,12 +430,13 @@ struct PArgs {
   int kap1, kap2;            // screened (k-hat with kappa) for each set
   int multipole;             // 0: local (CLExpan), 1: multipole (CMExpan)
   int tile;                  // points per tile
+  const int *sel;            // job indices to run (null: job = workgroup index)
   int p;
 };
 
 __global__ void __launch_bounds__(PT) k_p2l(Tabs tb, PArgs A) {
   extern __shared__ double sh[];
-  const PJob J = A.jobs[blockIdx.x];
+  const PJob J = A.jobs[A.sel ? A.sel[blockIdx.x] : blockIdx.x];
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
   const bool two = (A.q2 != nullptr);
   double rho0 = sqrt(J.P[0] * J.P[0] + J.P[1] * J.P[1] + J.P[2] * J.P[2]);
@@ -642,9 +644,9 @@ struct RAJob {
   int rot;           // rotation job index
   int fwd;
 };
-__global__ void __launch_bounds__(256) k_rotapply(const RAJob *jobs, const RJob *rj, const double *Rst,
-                                                  const double *in, double *out, int p) {
-  const RAJob J = jobs[blockIdx.x];
+__global__ void __launch_bounds__(256) k_rotapply(const RAJob *jobs, const int *sel, const RJob *rj,
+                                                  const double *Rst, const double *in, double *out, int p) {
+  const RAJob J = jobs[sel ? sel[blockIdx.x] : blockIdx.x];
   const RJob R = rj[J.rot];
   const int P2 = p * p;
   const double *Min = in + (size_t)J.in * P2;
@@ -976,11 +978,13 @@ struct LArgs {
   const double *slot;
   const double *base;    // [sphere] starting value, or null (zero)
   double *out;           // [sphere][p^2]
+  const int *sel;        // entries of sph/lst_off to run (null: workgroup index)
   int p;
 };
 __global__ void __launch_bounds__(256) k_sumslots(LArgs A) {
-  const int p = A.p, P2 = p * p, g = A.sph[blockIdx.x];
-  const int l0 = A.lst_off[blockIdx.x], l1 = A.lst_off[blockIdx.x + 1];
+  const int e0 = A.sel ? A.sel[blockIdx.x] : blockIdx.x;
+  const int p = A.p, P2 = p * p, g = A.sph[e0];
+  const int l0 = A.lst_off[e0], l1 = A.lst_off[e0 + 1];
   for (int i = threadIdx.x; i < P2; i += 256) {
     double s = A.base ? A.base[(size_t)g * P2 + i] : 0.0;
     for (int e = l0; e < l1; e++) s += A.slot[(size_t)A.lst[e] * P2 + i];
