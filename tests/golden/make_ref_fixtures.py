@@ -139,6 +139,16 @@ def main():
         fx["rotquat_%d_in" % i] = np.concatenate([qq, [p, fwd], b])
         cmds.append("rotquat %s %d %d %s" % (fmt(qq), p, fwd, fmt(b)))
         keys.append("rotquat_%d" % i)
+    # singular orientations (theta = 0 or pi after the Euler split), both directions
+    rs = np.random.default_rng(77)
+    for i, qq in enumerate([(0.0, 1.0, 0.0, 0.0), (0.0, 0.6, 0.8, 0.0), (0.6, 0.0, 0.0, 0.8),
+                            (0.0, 0.0, 0.0, 1.0)]):
+        for fwd in (0, 1):
+            p = 9
+            b = rand_block(rs, p)
+            fx["rotqs_%d_%d_in" % (i, fwd)] = np.concatenate([qq, [p, fwd], b])
+            cmds.append("rotquat %s %d %d %s" % (fmt(qq), p, fwd, fmt(b)))
+            keys.append("rotqs_%d_%d" % (i, fwd))
 
     # translations
     for i, (rho, kap, p, si, so, grad) in enumerate([(11.0, KAPPA, 10, 3.0, 4.5, 0), (11.0, 0.0, 10, 3.0, 4.5, 0),

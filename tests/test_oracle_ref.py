@@ -66,6 +66,15 @@ def test_rotate_quaternion(orc, golden):
         assert relerr(orc.rotate_quat(q, p, fwd, a[6:]), golden["out_rotquat_%d" % i]) < TOL, i
 
 
+def test_rotate_quaternion_singular(orc, golden):
+    """orientations whose Euler split has theta = 0 or pi, both directions"""
+    for i in range(4):
+        for fwd in (0, 1):
+            a = golden["rotqs_%d_%d_in" % (i, fwd)]
+            got = orc.rotate_quat(a[:4], int(a[4]), int(a[5]), a[6:])
+            assert relerr(got, golden["out_rotqs_%d_%d" % (i, fwd)]) < TOL, (i, fwd)
+
+
 def test_translate(orc, golden):
     for i in range(4):
         a = golden["trans_%d_in" % i]
