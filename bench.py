@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X mutual-polarization solve (PB-SAM's hot path).
+
+Workload: two synthetic molecules of 60 spheres each (radii U[2,6] A, rejection-packed in
+a 28.8 A ball, 10-40 charges of +-1 e per sphere), pole order p = 30, 1800 spiral points
+per sphere, FP64, centres 50 A apart with random orientations, 0.05 M salt, eps 4 / 78,
+inter-molecular polarization cutoff 100 A (every pair polarized).  Each molecule type is
+self-polarized once (untimed set-up, the reference's `pbsam spol` stage).
+
+A step is one complete mutual polarization of the system from the self values, in the
+reference's Gauss-Seidel order with its stock stopping rule (CMolecule::polarize_mutual,
+src/molecule.cpp:1989-2075).  Inputs stay resident in HBM; the timed region is bracketed
+by stream synchronizations.
+
+Prints one JSON line.  `roofline` is the kernel with the largest share of device time,
+timed with HIP events around each of its launches on the library's stream, with the
+algorithmic flops/bytes its launches processed (counted on the device from the
+reference's loop bounds at the truncation orders actually used).  `cpu_baseline` is the
+CPU oracle (a scalar restatement of the reference, 1 thread) solving the same system.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pb-sam_amd"))
+
+METRIC = "mutual-polarization solves/s & force+torque evals/s, p=30, FP64 rel err ≤1e-8"
+N_SPH, P, NPTS, DIST = 60, 30, 1800, 50.0
+SEEDS, POSE_SEED = (101, 102), 7
+PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix), flop/s
+PEAK_HBM = 8.0e12        # MI355X HBM3E, B/s
+
+
+def build_system(pbsam_amd, synth, ctx):
+    mols = [synth.molecule(N_SPH, seed=s) for s in SEEDS]
+    mts = []
+    for m in mols:
+        mt = pbsam_amd.MolType(ctx, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], P, eps_in=synth.EPS_IN,
+                               rcen=m["rcen"], npts=[NPTS] * N_SPH)
+        mt.selfpol(tol=0, max_rounds=0)
+        mt.values()
+        mts.append(mt)
+    pos, quat = synth.place_pair(mols[0], mols[1], DIST, seed=POSE_SEED)
+    system = pbsam_amd.System(ctx, mts, pos, quat)
+    system.set_cutoffs(100.0, 100.0)
+    return mols, mts, pos, quat, system
+
+
+def cpu_baseline(kappa, mols, mts, pos, quat, gpu_system, budget_s):
+    """The oracle (a scalar restatement of the reference, 1 thread) on the same molecules,
+    self values and pose: complete mutual solves repeated until `budget_s` seconds have
+    passed (at least one).  Also returns the largest relative difference of the device's
+    F and H blocks from the oracle's."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from pbsam_amd import synth
+    orc = pyoracle.Oracle(kappa)
+    omts = []
+    for m, g in zip(mols, mts):
+        o = pyoracle.MolType(orc, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], P, eps_in=synth.EPS_IN,
+                             rcen=m["rcen"], npts=[NPTS] * N_SPH)
+        o.set_self(*g.FH())
+        o.values()
+        omts.append(o)
+    osys = pyoracle.System(orc, omts, pos, quat)
+    osys.set_cutoffs(100.0, 100.0)
+    osys.lists()
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        rounds = osys.mutual(0, 0.0, 0)
+        reps += 1
+        tf = time.perf_counter() - t0
+        if tf >= budget_s:
+            break
+    err = 0.0
+    for i in range(2):
+        for k in range(N_SPH):
+            a, b = gpu_system.get(i, k), osys.get(i, k)
+            for key in ("F", "H"):
+                scale = max(abs(b[key]).max(), 1e-300)
+                err = max(err, abs(a[key] - b[key]).max() / scale)
+    return dict(value=reps / tf, unit="solves/s", cores=1, kind="port",
+                sample="%d complete mutual solves of the same system (%d rounds each) in %.1f s" % (reps, rounds, tf),
+                max_rel_err_device_vs_oracle=err)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of oracle solves to time")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the solve of one system runs on one device; "
+              "independent systems would shard across GPUs with no exchange.  Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import pbsam_amd
+    from pbsam_amd import synth
+    kappa = pbsam_amd.kappa_from_salt(synth.SALT_M, synth.EPS_OUT, synth.TEMP)
+    ctx = pbsam_amd.Context(kappa, synth.EPS_OUT)
+    t_setup = time.perf_counter()
+    mols, mts, pos, quat, system = build_system(pbsam_amd, synth, ctx)
+    t_setup = time.perf_counter() - t_setup
+    print("set-up (self-polarization of both molecule types) %.1f s" % t_setup, file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        system.solve(tol=0, max_rounds=0)
+    ctx.sync()
+    ctx.profile(True)
+    rounds = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r, _ = system.solve(tol=0, max_rounds=0)
+        rounds.append(r)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    prof = {k: ctx.profile_read(k) for k in pbsam_amd.KERNELS}
+    ctx.profile(False)
+    ms_step = 1e3 * elapsed / args.steps
+    value = args.steps / elapsed
+    print("timed %d solves in %.3f s (rounds %s)" % (args.steps, elapsed, rounds), file=sys.stderr, flush=True)
+
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    d = prof[dom]
+    avg_s = d["ms"] * 1e-3 / max(d["launches"], 1)
+    compute_bound = dom != "solve" or d["bytes"] < 16 * d["flops"] / 4
+    if compute_bound:
+        achieved = d["flops"] / max(d["ms"] * 1e-3, 1e-30)
+        roof = dict(bound="mfma", pipe="fp64 (VALU; the FP64 matrix peak is the same 78.6 TFLOP/s)",
+                    achieved=achieved / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s", frac=achieved / PEAK_FP64)
+    else:
+        achieved = d["bytes"] / max(d["ms"] * 1e-3, 1e-30)
+        roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s", frac=achieved / PEAK_HBM)
+    roof.update(kernel=dom, launches=d["launches"], avg_launch_us=avg_s * 1e6,
+                algorithmic_per_launch=(d["flops"] if compute_bound else d["bytes"]) / max(d["launches"], 1),
+                traffic=None)
+    kernels = {k: dict(launches=v["launches"], ms_per_step=v["ms"] / args.steps,
+                       gflop_per_step=v["flops"] / args.steps / 1e9) for k, v in prof.items()}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(kappa, mols, mts, pos, quat, system, args.cpu_budget)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded molecules, no dataset)",
+        "config": {
+            "workload": "mutual polarization, 2 molecules x 60 spheres, p=30, 1800 points/sphere, "
+                        "centres 50 A apart, random orientations, reference Gauss-Seidel schedule",
+            "molecules": 2, "spheres_per_molecule": N_SPH, "p": P, "points_per_sphere": NPTS,
+            "separation_A": DIST, "inter_rcutoff_A": 100.0, "rounds_per_solve": rounds[-1],
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -75,11 +75,15 @@ int pbsam_moltype_info(const pbsam_moltype *mt, int *n_sph, int *p, long long *n
 /* exposed surface points of sphere k (relative to its centre) and the spiral size */
 int pbsam_moltype_points(const pbsam_moltype *mt, int k, double *xyz, int *n_exposed, int *n_spiral);
 
-/* Self-polarization (CMolecule::polarize_self, src/molecule.cpp:1899-1979), run as
- * Jacobi rounds on the device until the largest per-sphere change is <= tol or
- * max_rounds is reached.  rounds_out may be NULL.  Stores F/H as the self values. */
+/* Self-polarization (CMolecule::polarize_self, src/molecule.cpp:1899-1979) in the
+ * reference's Gauss-Seidel order: spheres in ascending index, selective recompute
+ * (dev > 0.1*maxdev or every 5th round), until maxdev <= tol or max_rounds.
+ * tol <= 0 / max_rounds <= 0 select the reference's values (1e-4, or 1e-2 for one sphere;
+ * 1700 rounds).  rounds_out may be NULL.  Stores F/H as the self values.
+ * Returns PBSAM_ERR_NOCONV when the round limit ends the loop. */
 int pbsam_selfpol(pbsam_moltype *mt, double tol, int max_rounds, int *rounds_out);
-/* fixed number of Jacobi rounds from the current state (does not store self values) */
+/* fixed number of simultaneous (Jacobi) rounds from the current state, every sphere
+ * from the previous round's values (does not store self values) */
 int pbsam_selfpol_rounds(pbsam_moltype *mt, int rounds, double *maxdev_out);
 /* Inject self-polarized F/H (n_sph*p^2 each), as reading .exp files
  * (readExpansionsP, src/readutil.cpp:564-666). */
@@ -105,12 +109,17 @@ int pbsam_system_set_cutoffs(pbsam_system *sys, double inter_rcutoff, double int
 
 /* Mutual polarization (CMolecule::polarize_mutual phases 0-2, src/molecule.cpp:1989-2075)
  * preceded by the pair lists (generateInterXFormsForPolarize_LowMemory :1036-1126).
- * Jacobi rounds until the largest change <= tol.  flags: PBSAM_HROT_REFRESH refreshes the
- * rotated H used for inter-molecular coupling every round (the reference keeps round-1
- * values).  Returns PBSAM_ERR_OVERLAP when two molecules' spheres overlap. */
+ * Phase 0 rotates the self values to the lab frame, round 1 updates every polarized
+ * sphere from the self values, later rounds run in the reference's Gauss-Seidel order
+ * (molecules in order, spheres ascending, those with dev > 0.1*MAXDEV) until the mean
+ * change <= tol or max_rounds (tol <= 0 / max_rounds <= 0: the reference's 1e-4 and
+ * 1000).  flags: PBSAM_HROT_REFRESH refreshes the rotated H used for inter-molecular
+ * coupling after each update (the reference keeps round-1 values).
+ * Returns PBSAM_ERR_OVERLAP when two molecules' spheres overlap and PBSAM_ERR_NOCONV
+ * when the round limit ends the loop. */
 #define PBSAM_HROT_REFRESH 1
 int pbsam_solve_mutual(pbsam_system *sys, double tol, int max_rounds, unsigned flags, int *rounds_out);
-/* fixed number of Jacobi rounds after the first (round 1 is always from self values) */
+/* round 1 from self values, then a fixed number of simultaneous (Jacobi) rounds */
 int pbsam_mutual_rounds(pbsam_system *sys, int rounds, unsigned flags, double *maxdev_out);
 int pbsam_system_get(const pbsam_system *sys, int mol, int k, double *F, double *H, double *LS,
                      double *Hrot);
@@ -135,6 +144,23 @@ int pbsam_xform_num_batch(pbsam_ctx *ctx, int n, int p, const double *P, const d
 /* CRotCoeff::reset(CQuat) + rotateWithXi (src/rotcoeff.cpp:181-201,250-350) */
 int pbsam_rotate_quat_batch(pbsam_ctx *ctx, int n, int p, const double *quat, int fwd, const double *in,
                             double *out);
+
+/* ---- instrumentation (no counterpart in the reference) ----
+ * With profiling on, every launch of the kernels below is bracketed by HIP events on the
+ * context's stream, and the kernels add their algorithmic work to device counters:
+ * flops and HBM bytes by the reference's loop bounds for the units they process
+ * (pairs at their truncation orders, points, inner sweeps).  pbsam_profile(ctx, 1)
+ * resets and starts; (ctx, 0) stops.  ms is the summed event time of the launches. */
+#define PBSAM_K_XFORM 0   /* analytical re-expansion R^H T R, one workgroup per pair */
+#define PBSAM_K_P2L 1     /* numerical re-expansion of surface charges */
+#define PBSAM_K_SOLVE 2   /* per-sphere inner F/H iteration */
+#define PBSAM_K_SURFACE 3 /* exposed surface charges */
+#define PBSAM_K_COUNT 4
+int pbsam_profile(pbsam_ctx *ctx, int on);
+int pbsam_profile_read(pbsam_ctx *ctx, int kernel, long long *launches, double *ms, double *flops,
+                       double *bytes);
+/* wait for the context's stream */
+int pbsam_sync(pbsam_ctx *ctx);
 
 #ifdef __cplusplus
 }

@@ -97,7 +97,7 @@ void orc_selfpol_round_jacobi(orc_moltype *m, double *maxdev);
 void orc_moltype_get(const orc_moltype *m, int k, double *F, double *H, double *qF,
                      double *qH, double *tq);
 void orc_moltype_set_self(orc_moltype *m, const double *F, const double *H);
-void orc_moltype_values(orc_moltype *m); /* exposed charges + intra self values */
+int orc_moltype_values(orc_moltype *m); /* exposed charges + intra self values; -1: no self values */
 void orc_moltype_get_intra(const orc_moltype *m, int k, double *LF, double *LH,
                            double *LFfar, double *LHfar);
 double orc_moltype_energy_self(const orc_moltype *m);

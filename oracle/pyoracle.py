@@ -59,7 +59,6 @@ def lib():
             ("orc_selfpol_round_jacobi", [P, P]),
             ("orc_moltype_get", [P, I, P, P, P, P, P]),
             ("orc_moltype_set_self", [P, P, P]),
-            ("orc_moltype_values", [P]),
             ("orc_moltype_get_intra", [P, I, P, P, P, P]),
             ("orc_system_destroy", [P]),
             ("orc_system_set_cutoffs", [P, D, D]),
@@ -73,6 +72,7 @@ def lib():
             f.restype = None
         for name, args in [
             ("orc_compute_order", [D, D, D, I]),
+            ("orc_moltype_values", [P]),
             ("orc_rot_coeff", [P, D, D, D, I, I, P, P]),
             ("orc_npts_rule", [D, I]),
             ("orc_moltype_nsph", [P]),
@@ -322,7 +322,8 @@ class MolType:
         self.orc.L.orc_moltype_set_self(self.h, _p(F), _p(H))
 
     def values(self):
-        self.orc.L.orc_moltype_values(self.h)
+        if self.orc.L.orc_moltype_values(self.h) != 0:
+            raise RuntimeError("self values missing: run selfpol or set_self first")
 
     def intra(self, k):
         p = self.p
@@ -339,6 +340,8 @@ class System:
         self._arr = arr
         pos, quat = f64(pos).reshape(-1), f64(quat).reshape(-1)
         self.h = orc.L.orc_system_create(orc.h, len(moltypes), ctypes.cast(arr, P), _p(pos), _p(quat))
+        if not self.h:
+            raise RuntimeError("self values missing: run selfpol or set_self first")
         self.n = len(moltypes)
         self.p = moltypes[0].p
 

@@ -140,6 +140,45 @@ struct RJob {       // rotation coefficients stored to HBM (molecule orientation
   int pad;
 };
 
+// ---- algorithmic work per unit, counted from the reference's loop bounds ----
+// Used only by the optional kernel instrumentation (flops and HBM bytes a launch must do
+// by the algorithm, independent of how a kernel schedules them).
+// rotation recursion (rotcoeff.cpp:398-441): levels m = 0..q-2, each step makes one complex
+// coefficient from three (33 flops)
+__host__ __device__ inline double work_rotgen(int q) {
+  double s = 0;
+  for (int m = 0; m + 1 < q; m++) s += (double)(2 * q - m - 2) * (2 * q - m - 2) - (double)(m + 1) * (m + 1);
+  return 33.0 * s;
+}
+// rotation apply (rotcoeff.cpp:250-441): one complex MAC (8 flops) per (n, s, m>=0)
+__host__ __device__ inline double work_rotapply(int q) {
+  double s = 0;
+  for (int n = 0; n < q; n++) s += (double)(n + 1) * (2 * n + 1);
+  return 8.0 * s;
+}
+// coaxial translation apply (transcoeff.cpp:211-364): per level m a (q-m)^2 real-by-complex
+// product (4 flops per term)
+__host__ __device__ inline double work_transapply(int q) {
+  double s = 0;
+  for (int m = 0; m < q; m++) s += (double)(q - m) * (q - m);
+  return 4.0 * s;
+}
+// one analytical re-expansion R^H T R of a pair with block orders pH and (intra) pF
+__host__ __device__ inline double work_xform(int pH, int pF) {
+  int q = pH > pF ? pH : pF;
+  double w = work_rotgen(q) + 2 * work_rotapply(pH) + work_transapply(pH);
+  if (pF > 0) w += 2 * work_rotapply(pF) + work_transapply(pF);
+  return w;
+}
+// point-to-expansion term of one point at order q: harmonics (7 flops per (n, m>=0)),
+// radial factors (3 per n) and the accumulation (2 per real coefficient)
+__host__ __device__ inline double work_p2l_point(int q) {
+  return 7.0 * q * (q + 1) / 2 + 3.0 * q + 2.0 * q * q;
+}
+// surface charges of one point from F and H at order p (expcenter.cpp:1420-1440):
+// harmonics once, 3 flops per real coefficient for each block
+__host__ __device__ inline double work_surface_point(int p) { return 7.0 * p * (p + 1) / 2 + 6.0 * p * p; }
+
 // offset of translation matrix of level m (dense (p-m)^2) in a pair's block
 __host__ __device__ __forceinline__ long long tmat_off(int p, int m) {
   long long o = 0;

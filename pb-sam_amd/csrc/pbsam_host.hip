@@ -116,6 +116,15 @@ inline void quat_euler(const double *q, double &theta, double &phi, double &xi) 
 // ----------------------------------------------------------------------------
 // context
 // ----------------------------------------------------------------------------
+// per-kernel instrumentation: HIP events around each launch on the context's stream and
+// device counters of algorithmic flops and bytes
+struct KProf {
+  std::vector<hipEvent_t> ev;  // pairs (start, stop) of launches not yet harvested
+  size_t used = 0;
+  long long launches = 0;
+  double ms = 0.0;
+};
+
 struct pbsam_ctx {
   double kappa, sdiel;
   int device;
@@ -124,7 +133,41 @@ struct pbsam_ctx {
   DBuf<double> t_c1, t_c2, t_c3, t_c6, t_c4, t_zeta, t_eta, t_alpha, t_beta, t_gam, t_del, t_Q;
   Tabs tabs;
   size_t lds_max = 65536;
+  bool prof = false;
+  DBuf<unsigned long long> d_work;  // [kernel][flops, bytes]
+  KProf kp[PBSAM_K_COUNT];
 };
+
+static void prof_harvest(pbsam_ctx *c, int k) {
+  KProf &P = c->kp[k];
+  if (!P.used) return;
+  HIPCHK(hipEventSynchronize(P.ev[P.used - 1]));
+  for (size_t i = 0; i < P.used; i += 2) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, P.ev[i], P.ev[i + 1]));
+    P.ms += ms;
+  }
+  P.used = 0;
+}
+static unsigned long long *prof_begin(pbsam_ctx *c, int k) {
+  if (!c->prof) return nullptr;
+  KProf &P = c->kp[k];
+  if (P.used + 2 > 16384) prof_harvest(c, k);
+  while (P.used + 2 > P.ev.size()) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    P.ev.push_back(e);
+  }
+  HIPCHK(hipEventRecord(P.ev[P.used], c->stream));
+  return c->d_work.p + 2 * k;
+}
+static void prof_end(pbsam_ctx *c, int k) {
+  if (!c->prof) return;
+  KProf &P = c->kp[k];
+  HIPCHK(hipEventRecord(P.ev[P.used + 1], c->stream));
+  P.used += 2;
+  P.launches++;
+}
 
 static void build_tables(pbsam_ctx *c) {
   const int NL = NLT, PM = PMAX;
@@ -209,31 +252,39 @@ static void launch_trans(pbsam_ctx *c, int njobs, const DBuf<TJob> &jobs, int p,
   hipLaunchKernelGGL(k_trans, dim3(njobs), dim3(64), lds, c->stream, c->tabs, jobs.p, p, out);
   HIPCHK(hipGetLastError());
 }
-static void launch_xform(pbsam_ctx *c, int njobs, const XArgs &a) {
+static void launch_xform(pbsam_ctx *c, int njobs, XArgs a) {
   if (!njobs) return;
   size_t lds = xform_lds(a.p);
   set_lds((const void *)k_xform, lds);
+  a.work = prof_begin(c, PBSAM_K_XFORM);
   hipLaunchKernelGGL(k_xform, dim3(njobs), dim3(XT), lds, c->stream, c->tabs, a);
   HIPCHK(hipGetLastError());
+  prof_end(c, PBSAM_K_XFORM);
 }
 static void launch_p2l(pbsam_ctx *c, int njobs, PArgs a) {
   if (!njobs) return;
   a.tile = p2l_tile(a.p);
   size_t lds = p2l_lds(a.p);
+  a.work = prof_begin(c, PBSAM_K_P2L);
   hipLaunchKernelGGL(k_p2l, dim3(njobs), dim3(PT), lds, c->stream, c->tabs, a);
   HIPCHK(hipGetLastError());
+  prof_end(c, PBSAM_K_P2L);
 }
-static void launch_solve(pbsam_ctx *c, int n, const SArgs &a) {
+static void launch_solve(pbsam_ctx *c, int n, SArgs a) {
   if (!n) return;
   size_t lds = sizeof(double) * (3 * (size_t)a.p * a.p + 8);
+  a.work = prof_begin(c, PBSAM_K_SOLVE);
   hipLaunchKernelGGL(k_solve, dim3(n), dim3(ST), lds, c->stream, a);
   HIPCHK(hipGetLastError());
+  prof_end(c, PBSAM_K_SOLVE);
 }
-static void launch_surface(pbsam_ctx *c, int n, const QArgs &a) {
+static void launch_surface(pbsam_ctx *c, int n, QArgs a) {
   if (!n) return;
   size_t lds = sizeof(double) * (2 * (size_t)a.p * a.p + 8);
+  a.work = prof_begin(c, PBSAM_K_SURFACE);
   hipLaunchKernelGGL(k_surface, dim3(n), dim3(ST), lds, c->stream, c->tabs, a);
   HIPCHK(hipGetLastError());
+  prof_end(c, PBSAM_K_SURFACE);
 }
 static void launch_rotcoeff(pbsam_ctx *c, int n, RJob *jobs, int p, double *out) {
   if (!n) return;
@@ -1573,11 +1624,63 @@ int pbsam_ctx_create(double kappa, double eps_out, int device, pbsam_ctx **out) 
 void pbsam_ctx_destroy(pbsam_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &P : c->kp)
+    for (hipEvent_t e : P.ev) (void)hipEventDestroy(e);
+  c->d_work.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
 const char *pbsam_last_error(const pbsam_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int pbsam_profile(pbsam_ctx *c, int on) {
+  if (!c) return PBSAM_ERR_BADARG;
+  return guarded(&c->err, [&]() {
+    HIPCHK(hipSetDevice(c->device));
+    for (int k = 0; k < PBSAM_K_COUNT; k++) prof_harvest(c, k);
+    if (on) {
+      c->d_work.alloc(2 * PBSAM_K_COUNT);
+      c->d_work.zero(c->stream);
+      for (auto &P : c->kp) {
+        P.launches = 0;
+        P.ms = 0.0;
+      }
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    c->prof = (on != 0);
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_profile_read(pbsam_ctx *c, int kernel, long long *launches, double *ms, double *flops,
+                       double *bytes) {
+  if (!c || kernel < 0 || kernel >= PBSAM_K_COUNT) return PBSAM_ERR_BADARG;
+  return guarded(&c->err, [&]() {
+    HIPCHK(hipSetDevice(c->device));
+    prof_harvest(c, kernel);
+    const KProf &P = c->kp[kernel];
+    unsigned long long w[2] = {0, 0};
+    if (c->d_work.p) {
+      HIPCHK(hipMemcpyAsync(w, c->d_work.p + 2 * kernel, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (launches) *launches = P.launches;
+    if (ms) *ms = P.ms;
+    if (flops) *flops = (double)w[0];
+    if (bytes) *bytes = (double)w[1];
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_sync(pbsam_ctx *c) {
+  if (!c) return PBSAM_ERR_BADARG;
+  return guarded(&c->err, [&]() {
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PBSAM_OK;
+  });
+}
 
 int pbsam_moltype_create(pbsam_ctx *c, const pbsam_moltype_desc *d, pbsam_moltype **out) {
   if (!c || !d || !out) return PBSAM_ERR_BADARG;

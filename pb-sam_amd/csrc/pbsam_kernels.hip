@@ -143,6 +143,7 @@ struct XArgs {
   const int *sel;          // job indices to run (null: job = workgroup index)
   int p;
   int full;            // 1: use order p for both blocks (no adaptive truncation)
+  unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
 };
 
 __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
@@ -163,6 +164,10 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
   }
   const int q = pH > pF ? pH : pF;
   const int NR = 2 * q - 1;  // rotation rows n < 2q-1
+  if (A.work && tid == 0) {
+    atomicAdd(&A.work[0], (unsigned long long)work_xform(pH, intra ? pF : 0));
+    atomicAdd(&A.work[1], (unsigned long long)(8 * (pH * pH + pF * pF + (intra ? 2 : 1) * P2)));
+  }
   // LDS carve
   cd *cur = reinterpret_cast<cd *>(sh);
   cd *nxt = cur + NR * NR;
@@ -432,6 +437,7 @@ struct PArgs {
   int tile;                  // points per tile
   const int *sel;            // job indices to run (null: job = workgroup index)
   int p;
+  unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
 };
 
 __global__ void __launch_bounds__(PT) k_p2l(Tabs tb, PArgs A) {
@@ -445,6 +451,11 @@ __global__ void __launch_bounds__(PT) k_p2l(Tabs tb, PArgs A) {
   if (two && J.src >= 0 && A.TQ2) p2 = compute_order(rho0, A.TQ2[J.src], J.a_src, p);
   const int q = p1 > p2 ? p1 : p2;
   const int T = A.tile;
+  if (A.work && tid == 0) {
+    const double w = work_p2l_point(p1) + (two ? work_p2l_point(p2) : 0.0);
+    atomicAdd(&A.work[0], (unsigned long long)(w * J.npts));
+    atomicAdd(&A.work[1], (unsigned long long)(8 * ((size_t)J.npts * (two ? 5 : 4) + (two ? 2 : 1) * P2)));
+  }
   double *Y = sh;                    // [T][q*q]
   double *w1 = Y + (size_t)T * q * q; // [T][q]
   double *w2 = w1 + T * q;           // [T][q]
@@ -725,6 +736,7 @@ struct SArgs {
   double *dev;             // [sphere] out: computeDev2(H_in, H_out)
   int *sweeps;             // [sphere] out (may be null)
   int p;
+  unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
 };
 
 __device__ __forceinline__ double block_sum(double v, double *red) {
@@ -882,6 +894,11 @@ __global__ void __launch_bounds__(ST) k_solve(SArgs A) {
   if (tid == 0) {
     A.dev[g] = td / (4 * ts) / 1.0;
     if (A.sweeps) A.sweeps[g] = ct;
+    if (A.work) {
+      const double P4 = (double)P2 * P2;
+      atomicAdd(&A.work[0], (unsigned long long)(ct * ((IM ? 4 * P4 : 0.0) + 12.0 * P2)));
+      atomicAdd(&A.work[1], (unsigned long long)(ct * (IM ? 16 * P4 : 0.0) + 8.0 * 10 * P2));
+    }
   }
 }
 
@@ -896,6 +913,7 @@ struct QArgs {
   const double *F, *H;
   double *qF, *qH, *TQF, *TQH;
   int p;
+  unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
 };
 __global__ void __launch_bounds__(ST) k_surface(Tabs tb, QArgs A) {
   extern __shared__ double sh[];
@@ -944,6 +962,10 @@ __global__ void __launch_bounds__(ST) k_surface(Tabs tb, QArgs A) {
   if (tid == 0) {
     A.TQF[g] = tF;
     A.TQH[g] = tH;
+    if (A.work) {
+      atomicAdd(&A.work[0], (unsigned long long)(work_surface_point(p) * cnt));
+      atomicAdd(&A.work[1], (unsigned long long)(40 * (size_t)cnt + 16 * P2));
+    }
   }
 }
 

@@ -31,7 +31,10 @@ SYMBOLS = [
     "pbsam_system_set_pose", "pbsam_system_set_cutoffs", "pbsam_solve_mutual", "pbsam_mutual_rounds",
     "pbsam_system_get", "pbsam_energy", "pbsam_system_counts", "pbsam_xform_h_batch",
     "pbsam_xform_intra_batch", "pbsam_xform_num_batch", "pbsam_rotate_quat_batch",
+    "pbsam_profile", "pbsam_profile_read", "pbsam_sync",
 ]
+
+KERNELS = {"xform": 0, "p2l": 1, "solve": 2, "surface": 3}
 
 
 class PbsamError(RuntimeError):
@@ -93,6 +96,9 @@ def load():
         "pbsam_xform_intra_batch": ([P, I, I, P, P, P, P, P, P, P], I),
         "pbsam_xform_num_batch": ([P, I, I, P, P, I, P, P, P, D, D, P, P], I),
         "pbsam_rotate_quat_batch": ([P, I, I, P, I, P, P], I),
+        "pbsam_profile": ([P, I], I),
+        "pbsam_profile_read": ([P, I, P, P, P, P], I),
+        "pbsam_sync": ([P], I),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -130,6 +136,20 @@ class Context:
 
     def __del__(self):
         self.close()
+
+    def sync(self):
+        self.check(self.L.pbsam_sync(self.h))
+
+    def profile(self, on):
+        """Start (reset) or stop the per-kernel HIP-event timing and work counters."""
+        self.check(self.L.pbsam_profile(self.h, int(bool(on))))
+
+    def profile_read(self, kernel):
+        """{'launches', 'ms', 'flops', 'bytes'} of one kernel since profiling started."""
+        n, ms, fl, by = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self.check(self.L.pbsam_profile_read(self.h, KERNELS[kernel], ctypes.byref(n), ctypes.byref(ms),
+                                             ctypes.byref(fl), ctypes.byref(by)))
+        return dict(launches=n.value, ms=ms.value, flops=fl.value, bytes=by.value)
 
     # kernel-level batches
     def xform_h(self, P3, a, order, blocks, p):
@@ -201,7 +221,9 @@ class MolType:
         self.ctx.check(self.ctx.L.pbsam_moltype_points(self.h, k, _p(out), None, None))
         return out.reshape(-1, 3), nt.value
 
-    def selfpol(self, tol=1e-24, max_rounds=400):
+    def selfpol(self, tol=0.0, max_rounds=0):
+        """polarize_self in the reference's order; tol/max_rounds <= 0: its stock values.
+        Returns (rounds, status) with status OK or ERR_NOCONV."""
         r = ctypes.c_int()
         rc = self.ctx.L.pbsam_selfpol(self.h, tol, max_rounds, ctypes.byref(r))
         if rc not in (OK, ERR_NOCONV):
@@ -270,7 +292,9 @@ class System:
         pos, quat = f64(pos), f64(quat)
         self.ctx.check(self.ctx.L.pbsam_system_set_pose(self.h, i, _p(pos), _p(quat)))
 
-    def solve(self, tol=1e-24, max_rounds=400, refresh=False):
+    def solve(self, tol=0.0, max_rounds=0, refresh=False):
+        """polarize_mutual in the reference's order; tol/max_rounds <= 0: its stock values.
+        Returns (rounds, status) with status OK or ERR_NOCONV."""
         r = ctypes.c_int()
         rc = self.ctx.L.pbsam_solve_mutual(self.h, tol, max_rounds, HROT_REFRESH if refresh else 0,
                                            ctypes.byref(r))

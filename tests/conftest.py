@@ -40,3 +40,15 @@ def relerr(a, b):
     a, b = np.asarray(a, float), np.asarray(b, float)
     scale = max(np.abs(b).max(), 1e-300)
     return np.abs(a - b).max() / scale
+
+
+def rot_close(a, b, p, tol):
+    """True when block a matches block b to `tol` relative to max|b|, allowing in row n the
+    rounding growth of the rotation recursion (3-4x per order): 1e-15 * 4^n."""
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    scale = max(np.abs(b).max(), 1e-300)
+    for n in range(p):
+        s = slice(n * n, (n + 1) * (n + 1))
+        if np.abs(a[s] - b[s]).max() > max(tol, 1e-15 * 4.0 ** n) * scale:
+            return False
+    return True
