@@ -34,6 +34,21 @@ PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix), flop/s
 PEAK_HBM = 8.0e12        # MI355X HBM3E, B/s
 
 
+def roctx():
+    """ROCm's marker library (a no-op unless a profiler records markers), or None."""
+    import ctypes
+    for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
+        try:
+            lib = ctypes.CDLL(name)
+        except OSError:
+            continue
+        lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+        lib.roctxRangePushA.restype = ctypes.c_int
+        lib.roctxRangePop.restype = ctypes.c_int
+        return lib
+    return None
+
+
 def build_system(pbsam_amd, synth, ctx):
     mols = [synth.molecule(N_SPH, seed=s) for s in SEEDS]
     mts = []
@@ -116,6 +131,9 @@ def main():
         system.solve(tol=0, max_rounds=0)
     ctx.sync()
     ctx.profile(True)
+    mark = roctx()
+    if mark:
+        mark.roctxRangePushA(b"timed_region")
     rounds = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -123,6 +141,8 @@ def main():
         rounds.append(r)
     ctx.sync()
     elapsed = time.perf_counter() - t0
+    if mark:
+        mark.roctxRangePop()
     prof = {k: ctx.profile_read(k) for k in pbsam_amd.KERNELS}
     ctx.profile(False)
     ms_step = 1e3 * elapsed / args.steps

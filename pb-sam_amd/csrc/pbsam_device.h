@@ -119,6 +119,8 @@ struct XJob {       // analytical re-expansion of one source sphere onto one tar
   const double *TF; // translation matrices for F (unscreened seeds), kind 1 only
   double P[3];      // target centre - source centre
   double a_src, a_tgt;
+  int tord;         // order the translation matrices were made at (0: this job's H order)
+  int pad;
 };
 struct NJob {       // numerical P2L of a source sphere's surface charges onto a target
   int src, tgt, slot, pad;
@@ -131,7 +133,7 @@ struct TJob {       // translation matrices of one pair
   double kappa_seed;
   long long off;
   int grad;
-  int pad;
+  int src;          // source sphere whose total sets the order (used with a totals array)
 };
 struct RJob {       // rotation coefficients stored to HBM (molecule orientations)
   double theta, phi, xi;

@@ -135,6 +135,8 @@ struct pbsam_ctx {
   size_t lds_max = 65536;
   bool prof = false;
   DBuf<unsigned long long> d_work;  // [kernel][flops, bytes]
+  DBuf<double> d_qpart;             // per-workgroup partial surface-charge totals
+  DBuf<int> d_xbin;                 // re-expansion orders, order classes and their counts
   KProf kp[PBSAM_K_COUNT];
 };
 
@@ -246,20 +248,61 @@ static void set_lds(const void *k, size_t bytes) {
   if (bytes > 65536) HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-static void launch_trans(pbsam_ctx *c, int njobs, const DBuf<TJob> &jobs, int p, double *out) {
+// translation matrices at order p, or (TQ given) at each pair's truncation order
+static void launch_trans(pbsam_ctx *c, int njobs, const TJob *jobs, int p, double *out, const int *sel = nullptr,
+                         const double *TQ = nullptr) {
   if (!njobs) return;
   size_t lds = sizeof(double) * ((size_t)(2 * p) * p + 2 * p + p);
-  hipLaunchKernelGGL(k_trans, dim3(njobs), dim3(64), lds, c->stream, c->tabs, jobs.p, p, out);
+  hipLaunchKernelGGL(k_trans, dim3(njobs), dim3(64), lds, c->stream, c->tabs, jobs, sel, TQ, p, out);
   HIPCHK(hipGetLastError());
 }
-static void launch_xform(pbsam_ctx *c, int njobs, XArgs a) {
-  if (!njobs) return;
-  size_t lds = xform_lds(a.p);
-  set_lds((const void *)k_xform, lds);
+// one k_xform launch for n workgroups whose orders are all <= qmax
+static void launch_xform_q(pbsam_ctx *c, int n, XArgs a, int qmax) {
+  const size_t lds = xform_lds(qmax);
   a.work = prof_begin(c, PBSAM_K_XFORM);
-  hipLaunchKernelGGL(k_xform, dim3(njobs), dim3(XT), lds, c->stream, c->tabs, a);
+  if (qmax <= 8) {
+    hipLaunchKernelGGL(k_xform<128>, dim3(n), dim3(128), lds, c->stream, c->tabs, a);
+  } else if (qmax <= 16) {
+    hipLaunchKernelGGL(k_xform<256>, dim3(n), dim3(256), lds, c->stream, c->tabs, a);
+  } else {
+    set_lds((const void *)k_xform<512>, lds);
+    hipLaunchKernelGGL(k_xform<512>, dim3(n), dim3(512), lds, c->stream, c->tabs, a);
+  }
   HIPCHK(hipGetLastError());
   prof_end(c, PBSAM_K_XFORM);
+}
+// A batch of re-expansions.  Large batches with adaptive orders are first split by order
+// class (k_xorder) so that low-order pairs run in small workgroups with little LDS; this
+// costs one 16-byte read-back.  Results do not depend on the split.
+static constexpr int XBIN_MIN = 512;
+static void launch_xform(pbsam_ctx *c, int njobs, XArgs a) {
+  if (!njobs) return;
+  if (a.ordH || a.full || njobs < XBIN_MIN) {
+    launch_xform_q(c, njobs, a, a.p);
+    return;
+  }
+  const size_t need = (size_t)(2 + NXCLASS) * njobs + NXCLASS;
+  if (need > c->d_xbin.n) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->d_xbin.alloc(std::max(need, (size_t)1 << 16));
+  }
+  int *ordH = c->d_xbin.p, *ordF = ordH + njobs, *lists = ordF + njobs, *cnt = lists + (size_t)NXCLASS * njobs;
+  HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int) * NXCLASS, c->stream));
+  hipLaunchKernelGGL(k_xorder, dim3((njobs + 255) / 256), dim3(256), 0, c->stream, a.jobs, a.sel, njobs, a.TQH,
+                     a.TQF, a.p, a.full, ordH, ordF, cnt, lists);
+  HIPCHK(hipGetLastError());
+  int h[NXCLASS];
+  HIPCHK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  static const int qmax[NXCLASS] = {8, 16, 24, PMAX};
+  for (int k = 0; k < NXCLASS; k++) {
+    if (!h[k]) continue;
+    XArgs b = a;
+    b.ordH = ordH;
+    b.ordF = ordF;
+    b.cls = lists + (size_t)k * njobs;
+    launch_xform_q(c, h[k], b, std::min(qmax[k], a.p));
+  }
 }
 static void launch_p2l(pbsam_ctx *c, int njobs, PArgs a) {
   if (!njobs) return;
@@ -278,20 +321,31 @@ static void launch_solve(pbsam_ctx *c, int n, SArgs a) {
   HIPCHK(hipGetLastError());
   prof_end(c, PBSAM_K_SOLVE);
 }
-static void launch_surface(pbsam_ctx *c, int n, QArgs a) {
+// surface charges of n listed spheres with at most maxcnt exposed points each
+static void launch_surface(pbsam_ctx *c, int n, QArgs a, int maxcnt) {
   if (!n) return;
+  a.nchunk = std::max(1, (maxcnt + ST - 1) / ST);
+  const size_t need = (size_t)2 * n * a.nchunk;
+  if (need > c->d_qpart.n) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->d_qpart.alloc(std::max(need, (size_t)1 << 16));
+  }
+  a.part = c->d_qpart.p;
   size_t lds = sizeof(double) * (2 * (size_t)a.p * a.p + 8);
   a.work = prof_begin(c, PBSAM_K_SURFACE);
-  hipLaunchKernelGGL(k_surface, dim3(n), dim3(ST), lds, c->stream, c->tabs, a);
+  hipLaunchKernelGGL(k_surface, dim3(n * a.nchunk), dim3(ST), lds, c->stream, c->tabs, a);
   HIPCHK(hipGetLastError());
   prof_end(c, PBSAM_K_SURFACE);
+  hipLaunchKernelGGL(k_surface_totals, dim3((n + 63) / 64), dim3(64), 0, c->stream, a.sph, n, a.nchunk, a.part,
+                     a.TQF, a.TQH);
+  HIPCHK(hipGetLastError());
 }
 static void launch_rotcoeff(pbsam_ctx *c, int n, RJob *jobs, int p, double *out) {
   if (!n) return;
   const int NR = 2 * p - 1;
   size_t lds = (size_t)2 * NR * NR * sizeof(cd);
   set_lds((const void *)k_rotcoeff, lds);
-  hipLaunchKernelGGL(k_rotcoeff, dim3(n), dim3(XT), lds, c->stream, c->tabs, jobs, p, out);
+  hipLaunchKernelGGL(k_rotcoeff, dim3(n), dim3(RT), lds, c->stream, c->tabs, jobs, p, out);
   HIPCHK(hipGetLastError());
 }
 static void launch_rotapply(pbsam_ctx *c, int n, const RAJob *jobs, const RJob *rj, const double *Rst,
@@ -387,6 +441,7 @@ struct pbsam_moltype {
   std::vector<double> pts;        // exposed points, concatenated
   std::vector<double> consts;     // ns*10*p
   long long npts_total = 0;
+  int max_nexp = 0;               // most exposed points on one sphere
   DBuf<double> d_pts, d_consts, d_Dfix, d_Vfix, d_E0, d_imats;
   std::vector<const double *> imat_ptr;
   DBuf<const double *> d_imat;
@@ -453,6 +508,7 @@ static void moltype_setup(pbsam_moltype *m, const pbsam_moltype_desc *d) {
     }
     S.nspx = N;
     S.nexp = (int)(m->pts.size() / 3) - S.pt_off;
+    m->max_nexp = std::max(m->max_nexp, S.nexp);
   }
   m->npts_total = (long long)m->pts.size() / 3;
   // intra near/far lists (generateMolTypeIntraPolLists molecule.cpp:501-535)
@@ -614,6 +670,7 @@ static void moltype_jobs(pbsam_moltype *m) {
         J.tgt = ki;
         J.slot = slot;
         J.kind = 1;
+        J.tord = p;
         for (int x = 0; x < 3; x++) J.P[x] = P[x];
         J.a_src = Sj.rad;
         J.a_tgt = Si.rad;
@@ -660,7 +717,7 @@ static void moltype_jobs(pbsam_moltype *m) {
   m->d_lst_off.upload(lst_off, s);
   m->d_slotH.alloc((size_t)std::max(slot, 1) * P2);
   m->d_slotF.alloc((size_t)std::max(slot, 1) * P2);
-  launch_trans(c, (int)tj.size(), m->d_tj, p, m->d_T.p);
+  launch_trans(c, (int)tj.size(), m->d_tj.p, p, m->d_T.p);
   // state: H = rE, F = 0, charges from them (initSurfaceCharges expcenter.cpp:577-587)
   m->d_F.alloc((size_t)ns * P2);
   m->d_F.zero(s);
@@ -679,7 +736,7 @@ static void moltype_jobs(pbsam_moltype *m) {
   m->d_md.alloc(2);
   QArgs qa{m->d_all.p, m->d_pts.p, m->d_pt_off.p, m->d_pt_cnt.p, m->d_nspx.p, m->d_consts.p,
            m->d_F.p, m->d_H.p, m->d_qF.p, m->d_qH.p, m->d_TQF.p, m->d_TQH.p, p};
-  launch_surface(c, ns, qa);
+  launch_surface(c, ns, qa, m->max_nexp);
   HIPCHK(hipStreamSynchronize(s));
 }
 
@@ -730,7 +787,7 @@ static double moltype_round(pbsam_moltype *m) {
   launch_solve(c, ns, sa);
   QArgs qa{m->d_all.p, m->d_pts.p, m->d_pt_off.p, m->d_pt_cnt.p, m->d_nspx.p, m->d_consts.p,
            m->d_F.p, m->d_H.p, m->d_qF.p, m->d_qH.p, m->d_TQF.p, m->d_TQH.p, p};
-  launch_surface(c, ns, qa);
+  launch_surface(c, ns, qa, m->max_nexp);
   hipLaunchKernelGGL(k_maxdev, dim3(1), dim3(256), 0, c->stream, m->d_all.p, ns, m->d_dev.p, m->d_md.p);
   HIPCHK(hipGetLastError());
   std::vector<double> md = m->d_md.download(c->stream);
@@ -837,7 +894,7 @@ static void moltype_gs_pass(pbsam_moltype *m, const std::vector<int> &S) {
       sa.lst_off = m->d_lst_off.p + t;
       launch_solve(c, 1, sa);
       qa.sph = m->d_all.p + t;
-      launch_surface(c, 1, qa);
+      launch_surface(c, 1, qa, m->max_nexp);
     }
     xa.sel = m->d_selx.p + J.offx[seg];
     launch_xform(c, J.offx[seg + 1] - J.offx[seg], xa);
@@ -893,7 +950,7 @@ static void moltype_values(pbsam_moltype *m) {
   m->d_TQHself.alloc(ns);
   QArgs qa{m->d_all.p, m->d_pts.p, m->d_pt_off.p, m->d_pt_cnt.p, m->d_nspx.p, m->d_consts.p,
            m->d_Fself.p, m->d_Hself.p, m->d_qFself.p, m->d_qHself.p, m->d_TQFself.p, m->d_TQHself.p, p};
-  launch_surface(c, ns, qa);
+  launch_surface(c, ns, qa, m->max_nexp);
   // far pairs: analytical with full orders
   std::vector<XJob> fj;
   std::vector<TJob> tj;
@@ -908,6 +965,7 @@ static void moltype_values(pbsam_moltype *m) {
       J.tgt = ki;
       J.slot = (int)fj.size();
       J.kind = 1;
+      J.tord = p;
       for (int x = 0; x < 3; x++) J.P[x] = Si.cen[x] - Sj.cen[x];
       J.a_src = Sj.rad;
       J.a_tgt = Si.rad;
@@ -942,7 +1000,7 @@ static void moltype_values(pbsam_moltype *m) {
   dflst.upload(flst, s);
   fsH.alloc((size_t)std::max<size_t>(fj.size(), 1) * P2);
   fsF.alloc((size_t)std::max<size_t>(fj.size(), 1) * P2);
-  launch_trans(c, (int)tj.size(), dtj, p, m->d_Tfar.p);
+  launch_trans(c, (int)tj.size(), dtj.p, p, m->d_Tfar.p);
   XArgs xa{};
   xa.jobs = dfj.p;
   xa.H = m->d_Hself.p;
@@ -1012,6 +1070,7 @@ struct pbsam_system {
   pbsam_ctx *c;
   int p, nmol, nsph = 0;
   long long npts = 0;
+  int max_nexp = 0;
   std::vector<MolInst> mols;
   double interRc = 1e300, interactRc = 1e300;
   // static per-sphere data (copied from the molecule types)
@@ -1058,6 +1117,7 @@ static void system_static(pbsam_system *S) {
     M.pbase = (int)pt;
     g += M.mt->ns;
     pt += M.mt->npts_total;
+    S->max_nexp = std::max(S->max_nexp, M.mt->max_nexp);
   }
   S->nsph = g;
   S->npts = pt;
@@ -1237,6 +1297,7 @@ static int system_lists(pbsam_system *S) {
         T.a_tgt = J.a_tgt;
         T.kappa_seed = c->kappa;
         T.off = (long long)ti.size() * TS;
+        T.src = J.src;
         ti.push_back(T);
         xi.push_back(J);
         ilst.push_back(islot++);
@@ -1256,6 +1317,7 @@ static int system_lists(pbsam_system *S) {
           J.tgt = g;
           J.slot = aslot;
           J.kind = 1;
+          J.tord = p;
           for (int x = 0; x < 3; x++) J.P[x] = P[x];
           J.a_src = Sj.rad;
           J.a_tgt = Si.rad;
@@ -1314,7 +1376,6 @@ static int system_lists(pbsam_system *S) {
   S->d_slotI.alloc((size_t)std::max(islot, 1) * P2);
   S->d_slotH.alloc((size_t)std::max(aslot, 1) * P2);
   S->d_slotF.alloc((size_t)std::max(aslot, 1) * P2);
-  launch_trans(c, (int)ti.size(), S->d_tinter, p, S->d_Tinter.p);
   // molecule rotations (rotateRotCoeff moldynamics.cpp:682-685)
   std::vector<RJob> rj(nmol);
   const long long RS = 2 * rot_off(p);
@@ -1360,6 +1421,7 @@ static double system_round(pbsam_system *S, bool round1, bool refresh) {
   xa.TQH = round1 ? S->d_TQHself.p : S->d_TQH.p;
   xa.slotH = S->d_slotI.p;
   xa.p = p;
+  launch_trans(c, S->n_inter, S->d_tinter.p, p, S->d_Tinter.p, nullptr, xa.TQH);
   launch_xform(c, S->n_inter, xa);
   LArgs la{S->d_pol.p, S->d_inter_off.p, S->d_inter_lst.p, S->d_slotI.p, nullptr, S->d_LSrot.p, nullptr, p};
   launch_sumslots(c, np, la);
@@ -1417,7 +1479,7 @@ static double system_round(pbsam_system *S, bool round1, bool refresh) {
   if (round1 || refresh) launch_rotapply(c, np, S->d_ra_h.p, S->d_rj.p, S->d_Rst.p, S->d_H.p, S->d_Hrot.p, p);
   QArgs qa{S->d_pol.p, S->d_pts.p, S->d_pt_off.p, S->d_pt_cnt.p, S->d_nspx.p, S->d_consts.p,
            S->d_F.p, S->d_H.p, S->d_qF.p, S->d_qH.p, S->d_TQF.p, S->d_TQH.p, p};
-  launch_surface(c, np, qa);
+  launch_surface(c, np, qa, S->max_nexp);
   hipLaunchKernelGGL(k_maxdev, dim3(1), dim3(256), 0, c->stream, S->d_pol.p, np, S->d_dev.p, S->d_md.p);
   HIPCHK(hipGetLastError());
   std::vector<double> md = S->d_md.download(c->stream);
@@ -1457,6 +1519,7 @@ static void system_gs_round(pbsam_system *S, const std::vector<double> &dev, dou
     xa.p = p;
     int nsel = 0;
     for (int k : Sk) nsel += S->ioff[k + 1] - S->ioff[k];
+    launch_trans(c, nsel, S->d_tinter.p, p, S->d_Tinter.p, S->d_seli.p, S->d_TQH.p);
     launch_xform(c, nsel, xa);
     LArgs la{S->d_pol.p, S->d_inter_off.p, S->d_inter_lst.p, S->d_slotI.p, nullptr, S->d_LSrot.p, S->d_selk.p, p};
     launch_sumslots(c, (int)Sk.size(), la);
@@ -1516,7 +1579,7 @@ static void system_gs_round(pbsam_system *S, const std::vector<double> &dev, dou
         sa.lst_off = S->d_intra_off.p + k;
         launch_solve(c, 1, sa);
         qa.sph = S->d_pol.p + k;
-        launch_surface(c, 1, qa);
+        launch_surface(c, 1, qa, S->max_nexp);
         if (refresh)
           launch_rotapply(c, 1, S->d_ra_h.p, S->d_rj.p, S->d_Rst.p, S->d_H.p, S->d_Hrot.p, p, S->d_iota.p + k);
       }
@@ -1628,6 +1691,8 @@ void pbsam_ctx_destroy(pbsam_ctx *c) {
   for (auto &P : c->kp)
     for (hipEvent_t e : P.ev) (void)hipEventDestroy(e);
   c->d_work.release();
+  c->d_qpart.release();
+  c->d_xbin.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1956,6 +2021,7 @@ static int xform_batch(pbsam_ctx *c, int n, int p, const double *P, const double
       J.tgt = i;
       J.slot = i;
       J.kind = intra ? 1 : 0;
+      J.tord = p;
       for (int x = 0; x < 3; x++) J.P[x] = P[3 * i + x];
       J.a_src = a[2 * i];
       J.a_tgt = a[2 * i + 1];
@@ -1991,7 +2057,7 @@ static int xform_batch(pbsam_ctx *c, int n, int p, const double *P, const double
     if (intra) dF.upload(std::vector<double>(F, F + (size_t)n * P2), s);
     dsH.alloc((size_t)n * P2);
     dsF.alloc((size_t)n * P2);
-    launch_trans(c, (int)tj.size(), dtj, p, dT.p);
+    launch_trans(c, (int)tj.size(), dtj.p, p, dT.p);
     XArgs xa{};
     xa.jobs = dxj.p;
     xa.H = dH.p;

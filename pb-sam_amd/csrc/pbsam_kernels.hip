@@ -18,19 +18,24 @@ namespace pbsam {
 
 // ----------------------------------------------------------------------------
 // translation matrices: one wavefront per pair, recursion over (l, n) per level m
-// Output per pair: for each m, dense M_m[n-m][l-m] (n,l in [m,p)) holding the factor the
-// forward translate multiplies Min(l,m) by:  l<=n: FACT(n-l)*(-1)^(n+l)*T(n,l,m), l>n: T(l,n,m)
+// Output per pair: for each m, dense M_m[n-m][l-m] (n,l in [m,q)) holding the factor the
+// forward translate multiplies Min(l,m) by:  l<=n: FACT(n-l)*(-1)^(n+l)*T(n,l,m), l>n: T(l,n,m).
+// The order q is pmax, or with a totals array the pair's truncation order
+// (CXFormBase::computeOrder) -- the values are those of the order-pmax recursion
+// truncated, since row l, column n depends only on rows <= l+n.
 // ----------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) k_trans(Tabs tb, const TJob *jobs, int p, double *Tout) {
+__global__ void __launch_bounds__(64) k_trans(Tabs tb, const TJob *jobs, const int *sel, const double *TQ,
+                                              int pmax, double *Tout) {
   extern __shared__ double sh[];
+  const TJob J = jobs[sel ? sel[blockIdx.x] : blockIdx.x];
+  const int tid = threadIdx.x;
+  double rho, th, ph;
+  cart_to_sph(J.P[0], J.P[1], J.P[2], rho, th, ph);
+  const int p = TQ ? compute_order(rho, TQ[J.src], J.a_src, pmax) : pmax;
   const int L = 2 * p - 1;  // recursion rows l in [0, 2p-2]
   double *U = sh;           // [(L+1)][p]   U[l*p+n] for the current level (columns n>=m)
   double *Ks = U + (L + 1) * p;   // [2p]
   double *fact = Ks + 2 * p;      // [p]
-  const TJob J = jobs[blockIdx.x];
-  const int tid = threadIdx.x;
-  double rho, th, ph;
-  cart_to_sph(J.P[0], J.P[1], J.P[2], rho, th, ph);
   const double scalei = J.a_src, scaleo = J.a_tgt;
   double *out = Tout + J.off;
   for (int pass = 0; pass < (J.grad ? 2 : 1); pass++) {
@@ -124,7 +129,7 @@ __global__ void __launch_bounds__(64) k_trans(Tabs tb, const TJob *jobs, int p, 
 // fused rotate -> translate -> rotate-back, one workgroup per pair.
 // LDS: two rotation levels (2q-1)^2 complex each, source blocks, per-level vectors.
 // ----------------------------------------------------------------------------
-constexpr int XT = 512;  // threads per workgroup
+constexpr int RT = 512;  // threads per workgroup of k_rotcoeff
 constexpr int XG = 16;   // lanes per dot product in the forward rotate / translate
 
 __device__ __forceinline__ double seg_reduce(double v) {
@@ -139,16 +144,61 @@ struct XArgs {
   const double *TQH;   // source totals driving the truncation order
   const double *TQF;
   double *slotH, *slotF;
-  const int *ordH, *ordF;  // explicit orders per job (null: adaptive from the totals)
-  const int *sel;          // job indices to run (null: job = workgroup index)
+  const int *ordH, *ordF;  // explicit orders per batch position (null: adaptive from the totals)
+  const int *sel;          // job index of each batch position (null: the position itself)
+  const int *cls;          // batch position of each workgroup (null: the workgroup index)
   int p;
   int full;            // 1: use order p for both blocks (no adaptive truncation)
   unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
 };
 
+// Orders of a batch of re-expansion jobs (the same rule as k_xform), per batch position,
+// and the split of the positions into classes by q = max(pH, pF): class c holds
+// q <= XCLASS[c].  cnt[c] is advanced with atomics, so the order inside a class varies;
+// each job writes only its own slot, so results do not depend on it.
+constexpr int NXCLASS = 4;
+__constant__ int XCLASS[NXCLASS] = {8, 16, 24, PMAX};
+__global__ void __launch_bounds__(256) k_xorder(const XJob *jobs, const int *sel, int n, const double *TQH,
+                                                 const double *TQF, int p, int full, int *ordH, int *ordF,
+                                                 int *cnt, int *lists) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  int c = -1;
+  if (e < n) {
+    const XJob J = jobs[sel ? sel[e] : e];
+    double rho, th, ph;
+    cart_to_sph(J.P[0], J.P[1], J.P[2], rho, th, ph);
+    const int pH = full ? p : compute_order(rho, TQH[J.src], J.a_src, p);
+    const int pF = (J.kind == 1) ? (full ? p : compute_order(rho, TQF[J.src], J.a_src, p)) : 0;
+    ordH[e] = pH;
+    ordF[e] = pF;
+    const int q = pH > pF ? pH : pF;
+    c = 0;
+    while (c < NXCLASS - 1 && q > XCLASS[c]) c++;
+  }
+  // one atomic per (wavefront, class): the first lane of the class reserves its lanes' slots
+  const int lane = threadIdx.x % 64;
+  for (int k = 0; k < NXCLASS; k++) {
+    const unsigned long long mask = __ballot(c == k);
+    if (!mask) continue;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[k], __popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (c == k) {
+      const int rank = __popcll(mask & ((1ull << lane) - 1));
+      lists[(size_t)k * n + base + rank] = e;
+    }
+  }
+}
+
+// XT threads per workgroup: 128 for q <= 8, 256 for q <= 16, 512 above (each thread owns
+// up to two of the q(q+1)/2 back-rotated outputs)
+template <int XT>
 __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
   extern __shared__ double sh[];
-  const XJob J = A.jobs[A.sel ? A.sel[blockIdx.x] : blockIdx.x];
+  const int e = A.cls ? A.cls[blockIdx.x] : blockIdx.x;  // position in the batch
+  const int job = A.sel ? A.sel[e] : e;
+  const XJob J = A.jobs[job];
   const int p = A.p, P2 = p * p;
   const int tid = threadIdx.x;
   const bool intra = (J.kind == 1);
@@ -156,14 +206,15 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
   cart_to_sph(J.P[0], J.P[1], J.P[2], rho, th, ph);
   int pH, pF = 0;
   if (A.ordH) {
-    pH = A.ordH[blockIdx.x];
-    if (intra) pF = A.ordF[blockIdx.x];
+    pH = A.ordH[e];
+    if (intra) pF = A.ordF[e];
   } else {
     pH = A.full ? p : compute_order(rho, A.TQH[J.src], J.a_src, p);
     if (intra) pF = A.full ? p : compute_order(rho, A.TQF[J.src], J.a_src, p);
   }
   const int q = pH > pF ? pH : pF;
   const int NR = 2 * q - 1;  // rotation rows n < 2q-1
+  const int tq = J.tord ? J.tord : pH;  // layout order of the translation matrices
   if (A.work && tid == 0) {
     atomicAdd(&A.work[0], (unsigned long long)work_xform(pH, intra ? pF : 0));
     atomicAdd(&A.work[1], (unsigned long long)(8 * (pH * pH + pF * pF + (intra ? 2 : 1) * P2)));
@@ -242,14 +293,14 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
       }
       __syncthreads();
       for (int n = m + tid; n < q; n += XT) {
-        const int w = p - m;
+        const int w = tq - m;
         cd sH = {0, 0}, sF = {0, 0};
         if (n < pH) {
-          const double *Mm = J.TH + tmat_off(p, m) + (size_t)(n - m) * w;
+          const double *Mm = J.TH + tmat_off(tq, m) + (size_t)(n - m) * w;
           for (int l = m; l < pH; l++) sH = cadd(sH, cscale(Mm[l - m], aH[l]));
         }
         if (intra && n < pF) {
-          const double *Mm = J.TF + tmat_off(p, m) + (size_t)(n - m) * w;
+          const double *Mm = J.TF + tmat_off(tq, m) + (size_t)(n - m) * w;
           for (int l = m; l < pF; l++) sF = cadd(sF, cscale(Mm[l - m], aF[l]));
         }
         bH[n] = sH;
@@ -342,15 +393,15 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
       // (c) translate this level: b(n) = sum_l M_m[n][l] a(l)
       {
         int g = tid / XG, lane = tid % XG;
-        const int w = p - m;
+        const int w = tq - m;
         for (int n = m + g; n < q; n += XT / XG) {
           cd sH = {0, 0}, sF = {0, 0};
           if (n < pH) {
-            const double *Mm = J.TH + tmat_off(p, m) + (size_t)(n - m) * w;
+            const double *Mm = J.TH + tmat_off(tq, m) + (size_t)(n - m) * w;
             for (int l = m + lane; l < pH; l += XG) sH = cadd(sH, cscale(Mm[l - m], aH[l]));
           }
           if (intra && n < pF) {
-            const double *Mm = J.TF + tmat_off(p, m) + (size_t)(n - m) * w;
+            const double *Mm = J.TF + tmat_off(tq, m) + (size_t)(n - m) * w;
             for (int l = m + lane; l < pF; l += XG) sF = cadd(sF, cscale(Mm[l - m], aF[l]));
           }
           sH.x = seg_reduce(sH.x);
@@ -577,7 +628,7 @@ __global__ void __launch_bounds__(PT) k_p2l(Tabs tb, PArgs A) {
 // rotation coefficients stored to HBM: molecule orientations (CRotCoeff::reset
 // rotcoeff.cpp:155-171,780-810).  Stores ROT(n,s,m) for n < p.
 // ----------------------------------------------------------------------------
-__global__ void __launch_bounds__(XT) k_rotcoeff(Tabs tb, RJob *jobs, int p, double *out) {
+__global__ void __launch_bounds__(RT) k_rotcoeff(Tabs tb, RJob *jobs, int p, double *out) {
   extern __shared__ double sh[];
   RJob &J = jobs[blockIdx.x];
   const int tid = threadIdx.x, q = p, NR = 2 * q - 1;
@@ -607,7 +658,7 @@ __global__ void __launch_bounds__(XT) k_rotcoeff(Tabs tb, RJob *jobs, int p, dou
   if (tid == 0) J.sing = sing;
   if (sing != 0) return;
   cd *R = reinterpret_cast<cd *>(out + J.off);
-  for (int m = tid; m < NR; m += XT) {
+  for (int m = tid; m < NR; m += RT) {
     sh_column(tb, NR, m, cost, sint, ph, [&](int n, double re, double im) {
       if (m == 0) cur[n * n + n] = {re, 0.0};
       else {
@@ -621,11 +672,11 @@ __global__ void __launch_bounds__(XT) k_rotcoeff(Tabs tb, RJob *jobs, int p, dou
   for (int m = 0; m < q; m++) {
     // store level m rows n in [m, q)
     for (int n = m; n < q; n++)
-      for (int s = -n + tid; s <= n; s += XT) R[rot_off(n) + m * (2 * n + 1) + s + n] = cur[n * n + n + s];
+      for (int s = -n + tid; s <= n; s += RT) R[rot_off(n) + m * (2 * n + 1) + s + n] = cur[n * n + n + s];
     if (m < q - 1) {
       const int base = (m + 1) * (m + 1);
       const int total = (2 * q - m - 2) * (2 * q - m - 2) - (m + 1) * (m + 1);
-      for (int e = tid; e < total; e += XT) {
+      for (int e = tid; e < total; e += RT) {
         int t = e + base;
         int n = (int)sqrt((double)t);
         while (n * n > t) n--;
@@ -750,10 +801,165 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
   return s;
 }
 
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// computeDev term of one (n, m) pair given its old (ar, ai) and new (br, bi) values
+// (expcenter.cpp:1270-1328); returns false where the pair is skipped
+__device__ __forceinline__ bool dev_term(double ar, double ai, double br, double bi, bool mpos, double &s) {
+  const double PL = 1e-30;
+  if (fabs(ar) < PL && fabs(ai) < PL) {
+    if (fabs(br) < PL && fabs(bi) < PL) return false;
+    s = 1.0;
+  } else if (fabs(br) < PL && fabs(bi) < PL) {
+    s = 1.0;
+  } else {
+    double dr = ar - br, di = ai - bi;
+    double top = dr * dr + di * di;
+    double bot = ar * ar + ai * ai + br * br + bi * bi;
+    s = mpos ? 2 * top / bot : top / bot;
+  }
+  return true;
+}
+
+// Inner F/H iteration of a sphere whose surface-integral matrix is the identity: every
+// coefficient pair (F_i, H_i) evolves on its own, so one wavefront holds the block in
+// registers (lane l owns the (n, m) pairs l, l+64, ...) and only the stopping test is
+// reduced across lanes.  Same arithmetic per coefficient as the general path.
+template <int KW>
+__device__ void solve_identity(const SArgs &A, int g, int lst0, int lst1) {
+  const int p = A.p, P2 = p * p, lane = threadIdx.x;
+  const double *cs = A.consts + (size_t)g * 10 * p;
+  const double *cH1 = cs, *cF1 = cs + p, *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cH2 = cs + 4 * p,
+               *cF2 = cs + 5 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p, *cInvH = cs + 8 * p;
+  // pair k of this lane: (n, m) with m >= 0; real index i0 = n^2 + 2m - 1 (m > 0) or n^2
+  const int npair = p * (p + 1) / 2;
+  double Fr[KW], Fi[KW], Hr[KW], Hi[KW], fbr[KW], fbi[KW], hbr[KW], hbi[KW], oHr[KW], oHi[KW], uHr[KW], uHi[KW];
+  double kH1[KW], kF1[KW], kH2[KW], kF2[KW], kIH[KW];  // per-order constants, held in registers
+  int nn[KW], i0[KW];
+  for (int k = 0; k < KW; k++) {
+    const int e = lane + 64 * k;
+    nn[k] = -1;
+    if (e >= npair) continue;
+    int n = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    while (n * (n + 1) / 2 > e) n--;
+    while ((n + 1) * (n + 2) / 2 <= e) n++;
+    const int m = e - n * (n + 1) / 2;
+    nn[k] = n;
+    kH1[k] = cH1[n];
+    kF1[k] = cF1[n];
+    kH2[k] = cH2[n];
+    kF2[k] = cF2[n];
+    kIH[k] = cInvH[n];
+    const int i = (m == 0) ? n * n : n * n + 2 * m - 1;
+    i0[k] = i;
+    const int cnt = (m == 0) ? 1 : 2;
+    double LH[2] = {0, 0}, LF[2] = {0, 0}, fb[2], hb[2];
+    for (int c = 0; c < cnt; c++) {
+      const int ii = i + c;
+      double lh = A.LS ? A.LS[(size_t)g * P2 + ii] : 0.0;
+      double lf = 0.0;
+      if (A.LS) lh = 0.0 + lh;
+      for (int e2 = lst0; e2 < lst1; e2++) {
+        lf += A.slotF[(size_t)A.lst[e2] * P2 + ii];
+        lh += A.slotH[(size_t)A.lst[e2] * P2 + ii];
+      }
+      if (A.farH) {
+        lh += A.farH[(size_t)g * P2 + ii];
+        lf += A.farF[(size_t)g * P2 + ii];
+      }
+      A.LFS[(size_t)g * P2 + ii] = lf;
+      A.LHS[(size_t)g * P2 + ii] = lh;
+      LH[c] = lh;
+      LF[c] = lf;
+      fb[c] = A.Vfix[(size_t)g * P2 + ii] + cLH1[n] * LH[c] + cLF1[n] * LF[c];
+      hb[c] = A.Dfix[(size_t)g * P2 + ii] + cLH2[n] * LH[c] + cLF2[n] * LF[c];
+    }
+    fbr[k] = fb[0];
+    hbr[k] = hb[0];
+    fbi[k] = (cnt == 2) ? fb[1] : 0.0;
+    hbi[k] = (cnt == 2) ? hb[1] : 0.0;
+    Fr[k] = A.F[(size_t)g * P2 + i];
+    Hr[k] = A.H[(size_t)g * P2 + i];
+    Fi[k] = (cnt == 2) ? A.F[(size_t)g * P2 + i + 1] : 0.0;
+    Hi[k] = (cnt == 2) ? A.H[(size_t)g * P2 + i + 1] : 0.0;
+    oHr[k] = uHr[k] = Hr[k];
+    oHi[k] = uHi[k] = Hi[k];
+  }
+  int ct = 0;
+  double dev = 1e300;
+  while (ct == 0 || (dev > 1e-8 && ct < 100)) {
+    double sum = 0.0;
+    for (int k = 0; k < KW; k++) {
+      if (nn[k] < 0) continue;
+      const int n = nn[k];
+      const bool mpos = (i0[k] != n * n);
+      const double inv = 1 / (double)(2 * n + 1);
+      double xr = fbr[k] + kH1[k] * Hr[k] + kF1[k] * Fr[k];
+      double xi = fbi[k] + kH1[k] * Hi[k] + kF1[k] * Fi[k];
+      Fr[k] = xr * inv;
+      Fi[k] = mpos ? xi * inv : 0.0;
+      xr = hbr[k] + kH2[k] * Hr[k] + kF2[k] * Fr[k];
+      xi = hbi[k] + kH2[k] * Hi[k] + kF2[k] * Fi[k];
+      Hr[k] = xr * kIH[k];
+      Hi[k] = mpos ? xi * kIH[k] : 0.0;
+      double s;
+      if (dev_term(oHr[k], mpos ? oHi[k] : 0.0, Hr[k], mpos ? Hi[k] : 0.0, mpos, s)) sum += s;
+      oHr[k] = Hr[k];
+      oHi[k] = Hi[k];
+    }
+    sum = wave_sum(sum);
+    dev = sum / (4 * p * p);
+    ct++;
+  }
+  double td = 0.0, ts = 0.0;
+  for (int k = 0; k < KW; k++) {
+    if (nn[k] < 0) continue;
+    const int n = nn[k];
+    const bool mpos = (i0[k] != n * n);
+    const double w = mpos ? 2.0 : 1.0;
+    td += w * ((uHr[k] - Hr[k]) * (uHr[k] - Hr[k]));
+    ts += w * (uHr[k] * uHr[k] + Hr[k] * Hr[k]);
+    if (mpos) {
+      td += w * ((uHi[k] - Hi[k]) * (uHi[k] - Hi[k]));
+      ts += w * (uHi[k] * uHi[k] + Hi[k] * Hi[k]);
+    }
+    A.F[(size_t)g * P2 + i0[k]] = Fr[k];
+    A.H[(size_t)g * P2 + i0[k]] = Hr[k];
+    if (mpos) {
+      A.F[(size_t)g * P2 + i0[k] + 1] = Fi[k];
+      A.H[(size_t)g * P2 + i0[k] + 1] = Hi[k];
+    }
+  }
+  td = wave_sum(td);
+  ts = wave_sum(ts);
+  if (lane == 0) {
+    A.dev[g] = td / (4 * ts) / 1.0;
+    if (A.sweeps) A.sweeps[g] = ct;
+    if (A.work) {
+      atomicAdd(&A.work[0], (unsigned long long)(ct * 12.0 * P2));
+      atomicAdd(&A.work[1], (unsigned long long)(8.0 * 10 * P2));
+    }
+  }
+}
+
 __global__ void __launch_bounds__(ST) k_solve(SArgs A) {
   extern __shared__ double sh[];
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
   const int g = A.sph[blockIdx.x];
+  if (!A.imat[g]) {  // uniform across the workgroup
+    if (tid < 64) {
+      const int l0 = A.lst_off[blockIdx.x], l1 = A.lst_off[blockIdx.x + 1];
+      // KW pairs per lane: 64*KW >= p(p+1)/2
+      if (p <= 10) solve_identity<1>(A, g, l0, l1);
+      else if (p <= 15) solve_identity<2>(A, g, l0, l1);
+      else if (p <= 22) solve_identity<4>(A, g, l0, l1);
+      else solve_identity<9>(A, g, l0, l1);
+    }
+    return;
+  }
   double *F = sh, *H = F + P2, *X = H + P2, *red = X + P2;
   const double *cs = A.consts + (size_t)g * 10 * p;
   const double *cH1 = cs, *cF1 = cs + p, *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cH2 = cs + 4 * p,
@@ -914,25 +1120,35 @@ struct QArgs {
   double *qF, *qH, *TQF, *TQH;
   int p;
   unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
+  double *part;              // [workgroup][2] partial sums of |qF|, |qH|
+  int nchunk;                // workgroups per sphere (ST points each)
 };
+// One workgroup per (sphere, chunk of ST points); one point per thread.  The totals
+// TQF/TQH are finished by k_surface_totals from the per-workgroup partial sums.
 __global__ void __launch_bounds__(ST) k_surface(Tabs tb, QArgs A) {
   extern __shared__ double sh[];
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
-  const int g = A.sph[blockIdx.x];
+  const int ch = blockIdx.x % A.nchunk;
+  const int g = A.sph[blockIdx.x / A.nchunk];
   double *Fs = sh, *Hs = Fs + P2, *red = Hs + P2;
   const double *cChgH = A.consts + (size_t)g * 10 * p + 9 * p;
+  const int off = A.pt_off[g], cnt = A.pt_cnt[g];
+  if (ch * ST >= cnt) {  // uniform across the workgroup
+    if (tid == 0) {
+      A.part[2 * blockIdx.x] = 0.0;
+      A.part[2 * blockIdx.x + 1] = 0.0;
+    }
+    return;
+  }
   for (int i = tid; i < P2; i += ST) {
-    int n = (int)sqrt((double)i);
-    while (n * n > i) n--;
-    while ((n + 1) * (n + 1) <= i) n++;
     Fs[i] = A.F[(size_t)g * P2 + i];
     Hs[i] = A.H[(size_t)g * P2 + i];
   }
   __syncthreads();
-  const int off = A.pt_off[g], cnt = A.pt_cnt[g];
   const double dA = 4 * M_PI / double(A.nspx[g]);
   double tF = 0.0, tH = 0.0;
-  for (int h = tid; h < cnt; h += ST) {
+  const int h = ch * ST + tid;
+  if (h < cnt) {
     const double *x = A.xyz + 3 * (size_t)(off + h);
     double rho, th, ph;
     cart_to_sph(x[0], x[1], x[2], rho, th, ph);
@@ -940,33 +1156,46 @@ __global__ void __launch_bounds__(ST) k_surface(Tabs tb, QArgs A) {
     double sF = 0.0, sH = 0.0;
     for (int m = 0; m < p; m++) {
       sh_column(tb, p, m, cost, sint, ph, [&](int n, double re, double im) {
-        const double c2 = (2 * n + 1) / (4.0 * M_PI), ch = cChgH[n];
+        const double c2 = (2 * n + 1) / (4.0 * M_PI), chH = cChgH[n];
         if (m == 0) {
           sF += Fs[n * n] * (re * c2);
-          sH += Hs[n * n] * (re * ch);
+          sH += Hs[n * n] * (re * chH);
         } else {
           const int i = n * n + 2 * m - 1;
           sF += 2.0 * (Fs[i] * (re * c2) + Fs[i + 1] * (im * c2));
-          sH += 2.0 * (Hs[i] * (re * ch) + Hs[i + 1] * (im * ch));
+          sH += 2.0 * (Hs[i] * (re * chH) + Hs[i + 1] * (im * chH));
         }
       });
     }
     const double qf = sF * dA, qh = sH * dA;
     A.qF[off + h] = qf;
     A.qH[off + h] = qh;
-    tF += fabs(qf);
-    tH += fabs(qh);
+    tF = fabs(qf);
+    tH = fabs(qh);
   }
   tF = block_sum(tF, red);
   tH = block_sum(tH, red);
   if (tid == 0) {
-    A.TQF[g] = tF;
-    A.TQH[g] = tH;
-    if (A.work) {
+    A.part[2 * blockIdx.x] = tF;
+    A.part[2 * blockIdx.x + 1] = tH;
+    if (A.work && ch == 0) {
       atomicAdd(&A.work[0], (unsigned long long)(work_surface_point(p) * cnt));
       atomicAdd(&A.work[1], (unsigned long long)(40 * (size_t)cnt + 16 * P2));
     }
   }
+}
+// TQF/TQH of each listed sphere: its chunks' partial sums in chunk order
+__global__ void __launch_bounds__(64) k_surface_totals(const int *sph, int n, int nchunk, const double *part,
+                                                       double *TQF, double *TQH) {
+  const int e = blockIdx.x * 64 + threadIdx.x;
+  if (e >= n) return;
+  double f = 0.0, h = 0.0;
+  for (int c = 0; c < nchunk; c++) {
+    f += part[2 * ((size_t)e * nchunk + c)];
+    h += part[2 * ((size_t)e * nchunk + c) + 1];
+  }
+  TQF[sph[e]] = f;
+  TQH[sph[e]] = h;
 }
 
 // max over listed spheres of dev; also returns the sum

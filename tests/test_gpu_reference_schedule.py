@@ -31,7 +31,7 @@ def test_selfpol_reference_order(gpu, orc, n_sph, p, npts):
         assert relerr(a["H"], b["H"]) < 1e-9, k
 
 
-@pytest.mark.parametrize("n_sph,p,npts", [(10, 10, 400), (6, 30, 1800)])
+@pytest.mark.parametrize("n_sph,p,npts", [(10, 10, 400), (6, 30, 1800), (30, 10, 400), (24, 30, 1800)])
 def test_mutual_reference_order(gpu, orc, n_sph, p, npts):
     import pbsam_amd
     import pyoracle
