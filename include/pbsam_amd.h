@@ -127,6 +127,28 @@ int pbsam_system_get(const pbsam_system *sys, int mol, int k, double *F, double 
 int pbsam_energy(pbsam_system *sys, double *e_per_mol);
 int pbsam_system_counts(const pbsam_system *sys, int *n_inter_pairs, int *n_polarized);
 
+/* Gradient phase of CMolecule::polarize_mutual (src/molecule.cpp:2077-2228) after a
+ * mutual solve of the current poses: forward-mode derivatives of every polarized
+ * sphere's H with respect to each molecule's position, in the reference's Gauss-Seidel
+ * order (prepareDTA_LowMemory :1483-1571, recomputeGrad_LowMemory :1811-1843), until the
+ * mean change <= tol or max_rounds passes (tol <= 0 / max_rounds <= 0: 1e-4 and 1000).
+ * passes_out: total passes over the per-molecule lists. */
+int pbsam_gradient(pbsam_system *sys, double tol, int max_rounds, int *passes_out);
+/* CMolecule::computeMol_Force_and_Torque (src/moldynamics.cpp:525-550) for every
+ * molecule after pbsam_gradient: force and torque (3 doubles each per molecule), the
+ * torque being sum c_k x f_k over spheres (the surface term is off in the reference), both
+ * multiplied by conj(orientation) as the reference does before returning them. */
+int pbsam_forces(pbsam_system *sys, double *force, double *torque);
+/* CMolecule::computeForces (src/moldynamics.cpp:554-587): pair lists, mutual solve,
+ * gradient, forces and torques.  Returns PBSAM_ERR_OVERLAP when spheres of two molecules
+ * overlap; PBSAM_ERR_NOCONV when the mutual solve hit its round limit (forces are still
+ * computed, as the reference does). */
+int pbsam_compute_forces(pbsam_system *sys, double tol, int max_rounds, double *force, double *torque,
+                         int *rounds_out);
+/* gradient blocks of sphere k of molecule mol: gH (3 p^2, polarized spheres) and gLHN
+ * (3 p^2), as left by the last pbsam_gradient */
+int pbsam_system_get_grad(const pbsam_system *sys, int mol, int k, double *gH, double *gLHN);
+
 /* ---- kernel-level entry points (one batch of independent jobs; host arrays in/out) ---- */
 /* CXFormA::reset + xformH (src/xforms.cpp:51-77,123-132) for n pairs.
  * P: 3n (target - source), a: 2n (source radius, target radius), order: n (<= p),
@@ -141,6 +163,10 @@ int pbsam_xform_intra_batch(pbsam_ctx *ctx, int n, int p, const double *P, const
 int pbsam_xform_num_batch(pbsam_ctx *ctx, int n, int p, const double *P, const double *a, int npts,
                           const double *xyz, const double *qF, const double *qH, double tQF, double tQH,
                           double *outF, double *outH);
+/* CXFormA::reset + xformH(CExpan, CGradExpan) (src/xforms.cpp:51-77,86-115,165-192):
+ * derivative of the re-expansion with respect to P, out3: n * 3 * p*p (x, y, z blocks) */
+int pbsam_xform_h_grad_batch(pbsam_ctx *ctx, int n, int p, const double *P, const double *a, const int *order,
+                             const double *in, double *out3);
 /* CRotCoeff::reset(CQuat) + rotateWithXi (src/rotcoeff.cpp:181-201,250-350) */
 int pbsam_rotate_quat_batch(pbsam_ctx *ctx, int n, int p, const double *quat, int fwd, const double *in,
                             double *out);

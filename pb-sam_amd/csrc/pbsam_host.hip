@@ -214,7 +214,33 @@ static void build_tables(pbsam_ctx *c) {
         de[n * 2 * PM + PM - m] = kapsqr * gn / ((2 * n + 1) * (2 * n - 1));
       }
     }
-  std::vector<double> Q((2 * PM - 1) * PM * 2, 0.0);  // reserved for the gradient transform
+  // singular-rotation derivative coefficients Q(n, m, 0/1) (computeQCoeff rotcoeff.cpp:950-983,
+  // with the theta = 0 values of dY/dtheta, specialSH expansion.cpp:526-546)
+  std::vector<double> Q((2 * PM - 1) * PM * 2, 0.0);
+  {
+    auto ETAh = [&](int n, int m) { return eta[n * 2 * NL + NL + m]; };
+    auto ZETAh = [&](int n, int m) { return zeta[n * NL + m]; };
+    auto Qr = [&](int n, int m, int k) -> double & { return Q[(n * PM + m) * 2 + k]; };
+    const int n2 = 2 * PM - 1;
+    std::vector<double> S(n2, 0.0);
+    if (n2 > 1) S[1] = -1;
+    if (n2 > 2) S[2] = -1.0 * 3;
+    for (int l = 3; l < n2; l++) S[l] = (1.0 * (2 * l - 1) * S[l - 1] - l * S[l - 2]) / (l - 1);
+    for (int l = 1; l < n2; l++) S[l] *= (-c3[l * NL + 1]);
+    for (int n = 1; n < n2; n++) {
+      Qr(n, 0, 0) = 0.0;
+      Qr(n, 0, 1) = S[n];
+    }
+    for (int n = 2; n < n2; n++) {
+      Qr(n - 1, 1, 0) = (ETAh(n, -1) * Qr(n, 0, 1) + ZETAh(n - 1, 0)) / ETAh(n, 0);
+      Qr(n - 1, 1, 1) = (n > 2) ? (ETAh(n, 1) / ETAh(n, 0)) * Qr(n, 0, 1) : 0.0;
+    }
+    for (int m = 1; m < PM - 1; m++)
+      for (int n = m + 2; n < 2 * PM - m - 1; n++) {
+        Qr(n - 1, m + 1, 0) = (ETAh(n, m - 1) * Qr(n, m, 0) + ZETAh(n - 1, m)) / ETAh(n, m);
+        Qr(n - 1, m + 1, 1) = (n > m + 2) ? (ETAh(n, m + 1) / ETAh(n, m)) * Qr(n, m, 1) : 0.0;
+      }
+  }
   hipStream_t s = c->stream;
   c->t_c1.upload(c1, s); c->t_c2.upload(c2, s); c->t_c3.upload(c3, s);
   c->t_c6.upload(c6, s); c->t_c4.upload(c4, s);
@@ -289,7 +315,7 @@ static void launch_xform(pbsam_ctx *c, int njobs, XArgs a) {
   int *ordH = c->d_xbin.p, *ordF = ordH + njobs, *lists = ordF + njobs, *cnt = lists + (size_t)NXCLASS * njobs;
   HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int) * NXCLASS, c->stream));
   hipLaunchKernelGGL(k_xorder, dim3((njobs + 255) / 256), dim3(256), 0, c->stream, a.jobs, a.sel, njobs, a.TQH,
-                     a.TQF, a.p, a.full, ordH, ordF, cnt, lists);
+                     a.TQF, a.tqmap, a.p, a.full, ordH, ordF, cnt, lists);
   HIPCHK(hipGetLastError());
   int h[NXCLASS];
   HIPCHK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -463,6 +489,8 @@ struct pbsam_moltype {
   DBuf<double> d_Tfar;
   // index of the analytical job of (target ki, near entry e), -1 for numerical pairs
   std::vector<std::vector<int>> ana;
+  std::vector<const double *> tpair;  // [ki*ns+kj] translation matrices (H then F) of the pair,
+                                      // null where the re-expansion is numerical
   // near entries per target (source, analytical job or numerical job) and by source
   struct Entry {
     int src, xj, pj;
@@ -1001,6 +1029,13 @@ static void moltype_values(pbsam_moltype *m) {
   fsH.alloc((size_t)std::max<size_t>(fj.size(), 1) * P2);
   fsF.alloc((size_t)std::max<size_t>(fj.size(), 1) * P2);
   launch_trans(c, (int)tj.size(), dtj.p, p, m->d_Tfar.p);
+  m->tpair.assign((size_t)ns * ns, nullptr);
+  for (int ki = 0, f = 0; ki < ns; ki++) {
+    const SphHost &Si = m->sph[ki];
+    for (size_t e = 0; e < Si.near.size(); e++)
+      if (m->ana[ki][e] >= 0) m->tpair[(size_t)ki * ns + Si.near[e]] = m->d_T.p + (size_t)(2 * m->ana[ki][e]) * TS;
+    for (int kj : Si.far) m->tpair[(size_t)ki * ns + kj] = m->d_Tfar.p + (size_t)(2 * f++) * TS;
+  }
   XArgs xa{};
   xa.jobs = dfj.p;
   xa.H = m->d_Hself.p;
@@ -1066,6 +1101,22 @@ struct MolInst {
   std::vector<std::vector<SphRef>> interPol, interact;
 };
 
+// device buffers of the gradient phase (grow-only, reused across calls)
+struct GradState {
+  DBuf<double> tG, tGF, tGH, tGHrot;       // [list entry][3][p^2]
+  DBuf<double> gSF, gSH, gnF, gnH;         // [3][points], [points]
+  DBuf<double> maxgF, maxgH;               // [sphere]
+  DBuf<double> gLHN, gH;                   // [sphere][3][p^2]
+  DBuf<double> slot, slotF, slotH;         // [slot][3][p^2]
+  DBuf<double> tGrot, LGF, LGH, LGH2;      // [3][p^2]
+  DBuf<double> T, dev, part;
+  DBuf<XJob> xj, xj2;
+  DBuf<TJob> tj;
+  DBuf<PJob> pj0, pj1, pj2;
+  DBuf<RAJob> ra, ra2;
+  DBuf<int> ia, ib, ic, id, ie, tqmap;
+};
+
 struct pbsam_system {
   pbsam_ctx *c;
   int p, nmol, nsph = 0;
@@ -1104,6 +1155,8 @@ struct pbsam_system {
   std::vector<int> mol_pol_begin;                             // pol index range per molecule
   DBuf<int> d_iota, d_selx, d_selp, d_seli, d_selk;
   double itot = 0.0;
+  GradState gr;
+  bool solved = false, grad_ok = false;
 };
 
 static void system_static(pbsam_system *S) {
@@ -1213,6 +1266,8 @@ static int system_lists(pbsam_system *S) {
   hipStream_t s = c->stream;
   const int nmol = S->nmol, p = S->p, P2 = p * p;
   system_reset_state(S);
+  S->solved = false;
+  S->grad_ok = false;
   for (MolInst &M : S->mols) {
     M.bInterXForm = false;
     M.interPol.assign(M.mt->ns, {});
@@ -1626,6 +1681,8 @@ static void system_phase0(pbsam_system *S) {
   dcopy(S->c, S->d_Hrot, S->d_HselfRot);
 }
 
+#include "pbsam_host_grad.inc"
+
 // ----------------------------------------------------------------------------
 // C ABI
 // ----------------------------------------------------------------------------
@@ -1931,6 +1988,7 @@ int pbsam_mutual_rounds(pbsam_system *S, int rounds, unsigned flags, double *max
     system_phase0(S);
     double md = system_round(S, true, (flags & PBSAM_HROT_REFRESH) != 0);
     for (int r = 0; r < rounds; r++) md = system_round(S, false, (flags & PBSAM_HROT_REFRESH) != 0);
+    S->solved = true;
     if (maxdev_out) *maxdev_out = md;
     return PBSAM_OK;
   });
@@ -1948,6 +2006,7 @@ int pbsam_solve_mutual(pbsam_system *S, double tol, int max_rounds, unsigned fla
     bool refresh = (flags & PBSAM_HROT_REFRESH) != 0;
     double crit = 0.0;
     int r = system_mutual_ref(S, tol, max_rounds, refresh, &crit);
+    S->solved = true;
     if (rounds_out) *rounds_out = r;
     return crit > (tol > 0 ? tol : 1e-4) ? PBSAM_ERR_NOCONV : PBSAM_OK;
   });
@@ -2001,7 +2060,123 @@ int pbsam_system_counts(const pbsam_system *S, int *n_inter_pairs, int *n_polari
   return PBSAM_OK;
 }
 
+int pbsam_gradient(pbsam_system *S, double tol, int max_rounds, int *passes_out) {
+  if (!S) return PBSAM_ERR_BADARG;
+  if (!S->solved) {
+    S->c->err = "pbsam_gradient needs a mutual solve of the current poses";
+    return PBSAM_ERR_STATE;
+  }
+  return guarded(&S->c->err, [&]() {
+    HIPCHK(hipSetDevice(S->c->device));
+    int ct = system_gradient(S, tol, max_rounds);
+    S->grad_ok = true;
+    if (passes_out) *passes_out = ct;
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_forces(pbsam_system *S, double *force, double *torque) {
+  if (!S || !force || !torque) return PBSAM_ERR_BADARG;
+  if (!S->grad_ok) {
+    S->c->err = "pbsam_forces needs pbsam_gradient first";
+    return PBSAM_ERR_STATE;
+  }
+  return guarded(&S->c->err, [&]() {
+    HIPCHK(hipSetDevice(S->c->device));
+    system_forces(S, force, torque);
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_compute_forces(pbsam_system *S, double tol, int max_rounds, double *force, double *torque,
+                         int *rounds_out) {
+  if (!S || !force || !torque) return PBSAM_ERR_BADARG;
+  for (int i = 0; i < 3 * S->nmol; i++) force[i] = torque[i] = 0.0;
+  if (S->nmol == 1) {
+    if (rounds_out) *rounds_out = 0;
+    return PBSAM_OK;
+  }
+  int r = 0;
+  int rc = pbsam_solve_mutual(S, tol, max_rounds, 0, &r);
+  if (rounds_out) *rounds_out = r;
+  if (rc != PBSAM_OK && rc != PBSAM_ERR_NOCONV) return rc;
+  int rc2 = pbsam_gradient(S, tol, max_rounds, nullptr);
+  if (rc2 != PBSAM_OK) return rc2;
+  rc2 = pbsam_forces(S, force, torque);
+  if (rc2 != PBSAM_OK) return rc2;
+  return rc;
+}
+
+int pbsam_system_get_grad(const pbsam_system *S, int i, int k, double *gH, double *gLHN) {
+  if (!S || i < 0 || i >= S->nmol || k < 0 || k >= S->mols[i].mt->ns) return PBSAM_ERR_BADARG;
+  if (!S->grad_ok) return PBSAM_ERR_STATE;
+  return guarded(&S->c->err, [&]() {
+    hipStream_t s = S->c->stream;
+    const size_t B3 = 3 * (size_t)S->p * S->p, g = (size_t)S->mols[i].base + k;
+    if (gH) HIPCHK(hipMemcpyAsync(gH, S->gr.gH.p + g * B3, sizeof(double) * B3, hipMemcpyDeviceToHost, s));
+    if (gLHN) HIPCHK(hipMemcpyAsync(gLHN, S->gr.gLHN.p + g * B3, sizeof(double) * B3, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return PBSAM_OK;
+  });
+}
+
 // ---- kernel-level batches ----
+int pbsam_xform_h_grad_batch(pbsam_ctx *c, int n, int p, const double *P, const double *a, const int *order,
+                             const double *in, double *out3) {
+  if (!c || n < 1 || p < 1 || p > PBSAM_PMAX || !P || !a || !order || !in || !out3) return PBSAM_ERR_BADARG;
+  for (int i = 0; i < n; i++)
+    if (order[i] < 1 || order[i] > p) return PBSAM_ERR_BADARG;
+  return guarded(&c->err, [&]() {
+    hipStream_t s = c->stream;
+    const int P2 = p * p;
+    const long long TS = tmat_size(p);
+    std::vector<XJob> xj(n);
+    std::vector<TJob> tj(n);
+    DBuf<double> dT, dH, dG;
+    dT.alloc((size_t)n * 2 * TS);
+    for (int i = 0; i < n; i++) {
+      XJob &J = xj[i];
+      J = XJob{};
+      J.src = i;
+      J.tgt = i;
+      J.slot = i;
+      J.kind = 0;
+      J.tord = p;
+      for (int x = 0; x < 3; x++) J.P[x] = P[3 * i + x];
+      J.a_src = a[2 * i];
+      J.a_tgt = a[2 * i + 1];
+      J.TH = dT.p + (size_t)i * 2 * TS;
+      TJob &T = tj[i];
+      T = TJob{};
+      for (int x = 0; x < 3; x++) T.P[x] = J.P[x];
+      T.a_src = J.a_src;
+      T.a_tgt = J.a_tgt;
+      T.kappa_seed = c->kappa;
+      T.off = (long long)i * 2 * TS;
+      T.grad = 1;
+    }
+    DBuf<XJob> dxj;
+    DBuf<TJob> dtj;
+    DBuf<int> dord;
+    dxj.upload(xj, s);
+    dtj.upload(tj, s);
+    dord.upload(std::vector<int>(order, order + n), s);
+    dH.upload(std::vector<double>(in, in + (size_t)n * P2), s);
+    dG.alloc((size_t)n * 3 * P2);
+    launch_trans(c, n, dtj.p, p, dT.p);
+    GArgs ga{};
+    ga.jobs = dxj.p;
+    ga.H = dH.p;
+    ga.ord = dord.p;
+    ga.out = dG.p;
+    ga.p = p;
+    launch_xgrad(c, n, ga);
+    std::vector<double> r = dG.download(s);
+    memcpy(out3, r.data(), sizeof(double) * r.size());
+    return PBSAM_OK;
+  });
+}
+
 static int xform_batch(pbsam_ctx *c, int n, int p, const double *P, const double *a, const int *orders, bool intra,
                        const double *H, const double *F, double *outH, double *outF) {
   if (!c || n < 1 || p < 1 || p > PBSAM_PMAX || !P || !a || !orders || !H || !outH || (intra && (!F || !outF)))

@@ -147,6 +147,7 @@ struct XArgs {
   const int *ordH, *ordF;  // explicit orders per batch position (null: adaptive from the totals)
   const int *sel;          // job index of each batch position (null: the position itself)
   const int *cls;          // batch position of each workgroup (null: the workgroup index)
+  const int *tqmap;        // index into TQH/TQF of each source block (null: the block index)
   int p;
   int full;            // 1: use order p for both blocks (no adaptive truncation)
   unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
@@ -159,16 +160,17 @@ struct XArgs {
 constexpr int NXCLASS = 4;
 __constant__ int XCLASS[NXCLASS] = {8, 16, 24, PMAX};
 __global__ void __launch_bounds__(256) k_xorder(const XJob *jobs, const int *sel, int n, const double *TQH,
-                                                 const double *TQF, int p, int full, int *ordH, int *ordF,
-                                                 int *cnt, int *lists) {
+                                                 const double *TQF, const int *tqmap, int p, int full, int *ordH,
+                                                 int *ordF, int *cnt, int *lists) {
   const int e = blockIdx.x * 256 + threadIdx.x;
   int c = -1;
   if (e < n) {
     const XJob J = jobs[sel ? sel[e] : e];
     double rho, th, ph;
     cart_to_sph(J.P[0], J.P[1], J.P[2], rho, th, ph);
-    const int pH = full ? p : compute_order(rho, TQH[J.src], J.a_src, p);
-    const int pF = (J.kind == 1) ? (full ? p : compute_order(rho, TQF[J.src], J.a_src, p)) : 0;
+    const int tqi = tqmap ? tqmap[J.src] : J.src;
+    const int pH = full ? p : compute_order(rho, TQH[tqi], J.a_src, p);
+    const int pF = (J.kind == 1) ? (full ? p : compute_order(rho, TQF[tqi], J.a_src, p)) : 0;
     ordH[e] = pH;
     ordF[e] = pF;
     const int q = pH > pF ? pH : pF;
@@ -209,8 +211,9 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
     pH = A.ordH[e];
     if (intra) pF = A.ordF[e];
   } else {
-    pH = A.full ? p : compute_order(rho, A.TQH[J.src], J.a_src, p);
-    if (intra) pF = A.full ? p : compute_order(rho, A.TQF[J.src], J.a_src, p);
+    const int tqi = A.tqmap ? A.tqmap[J.src] : J.src;
+    pH = A.full ? p : compute_order(rho, A.TQH[tqi], J.a_src, p);
+    if (intra) pF = A.full ? p : compute_order(rho, A.TQF[tqi], J.a_src, p);
   }
   const int q = pH > pF ? pH : pF;
   const int NR = 2 * q - 1;  // rotation rows n < 2q-1
@@ -489,6 +492,9 @@ struct PArgs {
   const int *sel;            // job indices to run (null: job = workgroup index)
   int p;
   unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
+  // gradient charges: per-point squared norms of the 3-vector charge of each set; a point
+  // whose norm is below 1e-15 is skipped (triexpan.cpp:127-153).  Null: skip zero charges.
+  const double *n1, *n2;
 };
 
 __global__ void __launch_bounds__(PT) k_p2l(Tabs tb, PArgs A) {
@@ -529,7 +535,8 @@ __global__ void __launch_bounds__(PT) k_p2l(Tabs tb, PArgs A) {
         const int kap = (set == 0 ? A.kap1 : A.kap2);
         const int qq = (set == 0 ? p1 : p2);
         double *w = (set == 0 ? w1 : w2) + t * q;
-        if (ch == 0.0) {
+        const double *nrm = (set == 0 ? A.n1 : A.n2);
+        if (nrm ? (nrm[J.pt_off + t0 + t] < 1e-15) : (ch == 0.0)) {
           for (int n = 0; n < q; n++) w[n] = 0.0;
           continue;
         }
@@ -1231,10 +1238,11 @@ struct LArgs {
   double *out;           // [sphere][p^2]
   const int *sel;        // entries of sph/lst_off to run (null: workgroup index)
   int p;
+  int bs;                // doubles per slot and per output (0: p^2)
 };
 __global__ void __launch_bounds__(256) k_sumslots(LArgs A) {
   const int e0 = A.sel ? A.sel[blockIdx.x] : blockIdx.x;
-  const int p = A.p, P2 = p * p, g = A.sph[e0];
+  const int p = A.p, P2 = A.bs ? A.bs : p * p, g = A.sph[e0];
   const int l0 = A.lst_off[e0], l1 = A.lst_off[e0 + 1];
   for (int i = threadIdx.x; i < P2; i += 256) {
     double s = A.base ? A.base[(size_t)g * P2 + i] : 0.0;
@@ -1244,3 +1252,5 @@ __global__ void __launch_bounds__(256) k_sumslots(LArgs A) {
 }
 
 }  // namespace pbsam
+
+#include "pbsam_kernels_grad.hip"

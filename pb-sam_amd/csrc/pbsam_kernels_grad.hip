@@ -1,0 +1,807 @@
+// HIP kernels of the gradient (force) phase of the PB-SAM mutual-polarization path
+// (gfx950, FP64); included by pbsam_kernels.hip.
+//
+// Kernel map (reference functions restated, paths relative to the reference src/):
+//   k_xgrad          CXFormA::xformH(CExpan, CGradExpan) + sphToCart  xforms.cpp:86-115,165-192
+//                    (CRotCoeff::computeGradCoeff rotcoeff.cpp:818-850, dRotateT/P(+Sing)
+//                     :450-771, CTransCoeff dT :372-395) -- fused per pair
+//   k_solve3         CSolExpCenter::solveFH with 3 right-hand sides   expcenter.cpp:1080-1370
+//   k_surfgrad       computeExposedSurfaceGradientFH                  expcenter.cpp:1378-1412
+//   k_force          computeForceOn_0 + the per-sphere force terms    expcenter.cpp:124-149,
+//                                                                     moldynamics.cpp:525-550
+namespace pbsam {
+
+__device__ __forceinline__ double QC(const Tabs &t, int n, int m, int k) { return t.Q[(n * PMAX + m) * 2 + k]; }
+__device__ __forceinline__ cd derv(cd c, int s) { return {-s * c.y, s * c.x}; }  // i*s*c
+__device__ __forceinline__ cd blk_c(const double *B, int n, int m) {             // (n, m>=0) entry
+  return (m == 0) ? cd{B[n * n], 0.0} : cd{B[n * n + 2 * m - 1], B[n * n + 2 * m]};
+}
+__device__ __forceinline__ void blk_put(double *B, int n, int m, cd v) {
+  if (m == 0) B[n * n] = v.x * 1.0;
+  else {
+    B[n * n + 2 * m - 1] = v.x;
+    B[n * n + 2 * m] = v.y;
+  }
+}
+
+// ---- singular orientations (sin(theta) < 1e-12): closed forms on whole blocks in LDS ----
+// rotate / rotateWithXi with xi = pi at theta = 0 (identity) or pi (parity), rotcoeff.cpp:368-393
+__device__ void sing_rot(const double *in, double *out, int q, int sing, int tid, int nt) {
+  for (int n = tid; n < q; n += nt) {
+    if (sing == 1) {
+      for (int mm = 0; mm < 2 * n + 1; mm++) out[n * n + mm] = in[n * n + mm];
+    } else {
+      out[n * n] = (n % 2 == 0) ? in[n * n] : -in[n * n];
+      int s = (n % 2 == 0) ? 1 : -1;
+      for (int mm = 1; mm < 2 * n + 1; mm++, s = -s) out[n * n + mm] = s * in[n * n + mm];
+    }
+  }
+}
+// dRotateTSing rotcoeff.cpp:610-700 (sign flipped for the transposed direction)
+__device__ void sing_dT(const Tabs &tb, const double *in, double *out, int q, int sing, bool fwd, int tid, int nt) {
+  const double f = fwd ? 1.0 : -1.0;
+  for (int n = tid; n < q; n += nt) {
+    if (n == 0) {
+      out[0] = 0.0;
+      continue;
+    }
+    if (sing == 1) {
+      out[n * n] = f * (2.0 * QC(tb, n, 0, 1) * in[n * n + 1] * 1.0);
+      for (int m = 1; m < n; m++) {
+        cd i0 = (m == 1) ? cscale(1.0, cd{in[n * n], 0.0}) : blk_c(in, n, m - 1);
+        cd i1 = blk_c(in, n, m + 1);
+        cd o = cadd(cscale(QC(tb, n, m, 0), i0), cscale(QC(tb, n, m, 1), i1));
+        out[n * n + 2 * m - 1] = f * o.x;
+        out[n * n + 2 * m] = f * o.y;
+      }
+      cd i0 = (n == 1) ? cscale(1.0, cd{in[n * n], 0.0}) : blk_c(in, n, n - 1);
+      cd o = cscale(QC(tb, n, n, 0), i0);
+      out[n * n + 2 * n - 1] = f * o.x;
+      out[n * n + 2 * n] = f * o.y;
+    } else {
+      const double s = (n % 2 == 1) ? -1.0 : 1.0;
+      out[n * n] = f * (2.0 * s * QC(tb, n, 0, 1) * in[n * n + 1] * 1.0);
+      for (int m = 1; m < n; m++) {
+        cd i0 = (m == 1) ? cscale(1.0, cd{in[n * n], 0.0}) : blk_c(in, n, m - 1);
+        cd i1 = blk_c(in, n, m + 1);
+        cd o = cscale(s, cadd(cscale(QC(tb, n, m, 0), cconj(i0)), cscale(QC(tb, n, m, 1), cconj(i1))));
+        out[n * n + 2 * m - 1] = f * o.x;
+        out[n * n + 2 * m] = f * o.y;
+      }
+      cd i0 = (n == 1) ? cscale(1.0, cd{in[n * n], 0.0}) : blk_c(in, n, n - 1);
+      cd o = cscale(s * QC(tb, n, n, 0), cconj(i0));
+      out[n * n + 2 * n - 1] = f * o.x;
+      out[n * n + 2 * n] = f * o.y;
+    }
+  }
+}
+// dRotatePSing rotcoeff.cpp:704-771 (sign flipped for the transposed direction at theta = 0)
+__device__ void sing_dP(const Tabs &tb, const double *in, double *out, int q, int sing, bool fwd, int tid, int nt) {
+  const double f = (!fwd && sing == 1) ? -1.0 : 1.0;
+  for (int n = tid; n < q; n += nt) {
+    if (n == 0) {
+      out[0] = 0.0;
+      continue;
+    }
+    if (sing == 1) {
+      out[n * n] = f * (2.0 * QC(tb, n, 0, 1) * in[n * n + 2] * 1.0);
+      for (int m = 1; m < n; m++) {
+        cd i0 = (m == 1) ? cscale(1.0, cd{in[n * n], 0.0}) : blk_c(in, n, m - 1);
+        cd i1 = blk_c(in, n, m + 1);
+        cd a = cmul(cd{0.0, QC(tb, n, m, 0)}, i0), b = cmul(cd{0.0, QC(tb, n, m, 1)}, i1);
+        out[n * n + 2 * m - 1] = f * (a.x - b.x);
+        out[n * n + 2 * m] = f * (a.y - b.y);
+      }
+      cd i0 = (n == 1) ? cscale(1.0, cd{in[n * n], 0.0}) : blk_c(in, n, n - 1);
+      cd o = cmul(cd{0.0, QC(tb, n, n, 0)}, i0);
+      out[n * n + 2 * n - 1] = f * o.x;
+      out[n * n + 2 * n] = f * o.y;
+    } else {
+      const double s = (n % 2 == 1) ? 1.0 : -1.0;
+      out[n * n] = f * (s * 2 * QC(tb, n, 0, 1) * in[n * n + 2] * 1.0);
+      for (int m = 1; m < n; m++) {
+        cd i0 = (m == 1) ? cscale(1.0, cd{in[n * n], 0.0}) : blk_c(in, n, m - 1);
+        cd i1 = blk_c(in, n, m + 1);
+        cd a = cmul(cd{0.0, QC(tb, n, m, 1)}, cconj(i1)), b = cmul(cd{0.0, QC(tb, n, m, 0)}, cconj(i0));
+        cd o = cscale(s, cd{a.x - b.x, a.y - b.y});
+        out[n * n + 2 * m - 1] = f * o.x;
+        out[n * n + 2 * m] = f * o.y;
+      }
+      cd i0 = (n == 1) ? cscale(1.0, cd{in[n * n], 0.0}) : blk_c(in, n, n - 1);
+      cd o = cmul(cd{0.0, -s * QC(tb, n, n, 0)}, cconj(i0));
+      out[n * n + 2 * n - 1] = f * o.x;
+      out[n * n + 2 * n] = f * o.y;
+    }
+  }
+}
+// coaxial translation of a whole block with the level matrices of k_trans (order tq layout)
+__device__ void blk_translate(const double *T, int tq, const double *in, double *out, int q, int tid, int nt) {
+  for (int e = tid; e < q * q; e += nt) {
+    int n = (int)sqrt((double)e);
+    while (n * n > e) n--;
+    while ((n + 1) * (n + 1) <= e) n++;
+    const int mm = e - n * n;
+    const int m = (mm + 1) / 2;   // mm = 0 -> m = 0; mm = 2m-1 (re), 2m (im)
+    const int w = tq - m;
+    const double *Mm = T + tmat_off(tq, m) + (size_t)(n - m) * w;
+    double s = 0.0;
+    for (int l = m; l < q; l++) s += Mm[l - m] * in[l * l + mm];
+    out[e] = s;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// gradient re-expansion: G3[d] = d/dP_d of R^H T R applied to the source block (P = target
+// centre - source centre), one workgroup per pair.  The rotation coefficients ROT and
+// their theta-derivative dROT advance level by level (m) in place in LDS; each level's
+// forward rotations, translations (T and dT from k_trans, grad layout) and back-rotation
+// contributions are taken before the level is overwritten.
+// LDS (doubles): q^2 source block, 14 q level vectors, then ROT and dROT, (2q-1)^2 complex
+// each -- or, at theta = 0 / pi, six q^2 scratch blocks.
+// ----------------------------------------------------------------------------
+constexpr int GT = 512;
+struct GArgs {
+  const XJob *jobs;     // kind 0; TH -> T then dT (k_trans grad layout, order tord or pH)
+  const double *H;      // source blocks (indexed by J.src)
+  const double *TQ;     // source totals for the order (indexed by J.src)
+  const int *ord;       // explicit order per job (null: from TQ)
+  double *out;          // [slot][3][p^2]
+  int p;
+  unsigned long long *work;
+};
+
+__global__ void __launch_bounds__(GT) k_xgrad(Tabs tb, GArgs A) {
+  extern __shared__ double sh[];
+  const XJob J = A.jobs[blockIdx.x];
+  const int p = A.p, P2 = p * p, tid = threadIdx.x;
+  double rho, th, ph;
+  cart_to_sph(J.P[0], J.P[1], J.P[2], rho, th, ph);
+  const int q = A.ord ? A.ord[blockIdx.x] : compute_order(rho, A.TQ[J.src], J.a_src, p);
+  const int tq = J.tord ? J.tord : q;
+  const double *T = J.TH, *dT = J.TH + tmat_size(tq);
+  double *G = A.out + (size_t)J.slot * 3 * P2;
+  if (A.work && tid == 0) {
+    atomicAdd(&A.work[0], (unsigned long long)(3 * work_rotgen(q) + 5 * work_rotapply(q) + 4 * work_transapply(q)));
+    atomicAdd(&A.work[1], (unsigned long long)(8 * (q * q + 3 * P2)));
+  }
+  const int NR = 2 * q - 1;
+  double *hs = sh;                             // source block, q^2
+  cd *a0 = reinterpret_cast<cd *>(hs + q * q); // R M      at this level
+  cd *a1 = a0 + q;                             // dR M
+  cd *a2 = a1 + q;                             // dP M
+  cd *b0 = a2 + q;                             // dT (R M)
+  cd *b1 = b0 + q;                             // T (R M)
+  cd *b2 = b1 + q;                             // T (dR M)
+  cd *b3 = b2 + q;                             // T (dP M)
+  cd *R = b3 + q;                              // ROT level m, row n at n^2 + n + s
+  cd *dR = R + NR * NR;                        // dROT level m
+  for (int i = tid; i < q * q; i += GT) hs[i] = A.H[(size_t)J.src * P2 + i];
+  for (int i = tid; i < 3 * P2; i += GT) G[(i / P2) * P2 + i % P2] = 0.0;
+  // CRotCoeff::initParams (rotcoeff.cpp:209-244) with xi = pi
+  double sint = sin(th), cost, cott = 0.0;
+  int sing = 0;
+  cd exphi;
+  if (sint < 1e-12) {
+    sing = (th < M_PI / 2) ? 1 : -1;
+    sint = 0.0;
+    cost = (sing == 1) ? 1.0 : -1.0;
+    exphi = {1.0, 0.0};
+  } else {
+    cost = cos(th);
+    cott = cost / sint;
+    double s_, c_;
+    sincos(ph, &s_, &c_);
+    exphi = {c_, s_};
+  }
+  const cd exiphi = cconj(exphi);
+  cd exxi;
+  {
+    double s_, c_;
+    sincos(M_PI, &s_, &c_);
+    exxi = {c_, s_};
+  }
+  // Jacobian of (rho, theta, phi) -> (x, y, z) (xforms.cpp:165-192)
+  const double ir = 1.0 / rho, sinp = exphi.y, cosp = exphi.x;
+  double JR[3][3];
+  if (sing) {
+    JR[0][0] = 0.0; JR[0][1] = ir * cost; JR[0][2] = 0.0;
+    JR[1][0] = 0.0; JR[1][1] = 0.0;       JR[1][2] = ir;
+    JR[2][0] = cost; JR[2][1] = 0.0;      JR[2][2] = 0.0;
+  } else {
+    const double irst = ir / sint, irct = ir * cost;
+    JR[0][0] = sint * cosp; JR[0][1] = irct * cosp; JR[0][2] = -irst * sinp;
+    JR[1][0] = sint * sinp; JR[1][1] = irct * sinp; JR[1][2] = irst * cosp;
+    JR[2][0] = cost;        JR[2][1] = -ir * sint;  JR[2][2] = 0.0;
+  }
+  __syncthreads();
+
+  if (sing != 0) {
+    // closed forms on whole blocks (xforms.cpp:86-115 with rotcoeff.cpp:368-393, 610-771);
+    // six scratch blocks in place of the rotation buffers
+    double *t1 = reinterpret_cast<double *>(R), *t2 = t1 + q * q, *Gr = t2 + q * q, *Gt = Gr + q * q,
+           *Gp = Gt + q * q, *t3 = Gp + q * q;
+    sing_rot(hs, t1, q, sing, tid, GT);                      // tM1 = R M
+    __syncthreads();
+    blk_translate(dT, tq, t1, t2, q, tid, GT);               // dT tM1
+    __syncthreads();
+    sing_rot(t2, Gr, q, sing, tid, GT);                      // Gr = R^H dT R M
+    __syncthreads();
+    blk_translate(T, tq, t1, t2, q, tid, GT);                // tL = T R M
+    __syncthreads();
+    sing_dT(tb, t2, Gt, q, sing, false, tid, GT);            // Gt = dR^H tL
+    sing_dP(tb, t2, Gp, q, sing, false, tid, GT);            // Gp = dP^H tL
+    sing_dT(tb, hs, t1, q, sing, true, tid, GT);             // dR M
+    __syncthreads();
+    blk_translate(T, tq, t1, t2, q, tid, GT);
+    __syncthreads();
+    sing_rot(t2, t3, q, sing, tid, GT);
+    __syncthreads();
+    for (int i = tid; i < q * q; i += GT) Gt[i] += t3[i];
+    sing_dP(tb, hs, t1, q, sing, true, tid, GT);             // dP M
+    __syncthreads();
+    blk_translate(T, tq, t1, t2, q, tid, GT);
+    __syncthreads();
+    sing_rot(t2, t3, q, sing, tid, GT);
+    __syncthreads();
+    for (int i = tid; i < q * q; i += GT) {
+      Gp[i] += t3[i];
+      const double ar[3] = {Gr[i], Gt[i], Gp[i]};
+      for (int d = 0; d < 3; d++) G[d * P2 + i] = JR[d][0] * ar[0] + JR[d][1] * ar[1] + JR[d][2] * ar[2];
+    }
+    return;
+  }
+
+  // level-0 seeds: ROT(n,s,0) = conj-ordered SH of order 2q-1 (rotcoeff.cpp:785-795)
+  for (int m = tid; m < NR; m += GT) {
+    sh_column(tb, NR, m, cost, sint, ph, [&](int n, double re, double im) {
+      if (m == 0) {
+        R[n * n + n] = {re, 0.0};
+      } else {
+        R[n * n + n + m] = {re, -im};
+        R[n * n + n - m] = {re, im};
+      }
+    });
+  }
+  __syncthreads();
+  // dROT level 0 (rotcoeff.cpp:818-836); sh_comp(n, s) = ROT(n, -s, 0) for s >= 0
+  for (int n = tid; n < NR; n += GT) {
+    if (n == 0) {
+      dR[0] = {0.0, 0.0};
+      continue;
+    }
+    const cd *row = R + n * n + n;
+    const double c0 = -sqrt((double)n * (n + 1));
+    dR[n * n + n] = cmul(cscale(c0, exiphi), row[-1]);
+    for (int s = 1; s < n; s++) {
+      cd v = cscale(s * cott, row[-s]);
+      cd w = cmul(cscale(sqrt((double)(n - s) * (n + s + 1)), exiphi), row[-(s + 1)]);
+      cd dm = {v.x - w.x, v.y - w.y};
+      dR[n * n + n - s] = dm;
+      dR[n * n + n + s] = cconj(dm);
+    }
+    cd dm = cscale(n * cott, row[-n]);
+    dR[n * n + n - n] = dm;
+    dR[n * n + n + n] = cconj(dm);
+  }
+  // back-rotated accumulators: each thread owns up to two (n, s) outputs
+  const int nout = q * (q + 1) / 2;
+  int on[2], os[2];
+  cd aGr[2], aGt1[2], aGt2[2], aGp1[2], aGp2[2];
+  for (int k = 0; k < 2; k++) {
+    int e = tid + k * GT;
+    on[k] = -1;
+    os[k] = 0;
+    aGr[k] = aGt1[k] = aGt2[k] = aGp1[k] = aGp2[k] = {0.0, 0.0};
+    if (e < nout) {
+      int n = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+      while (n * (n + 1) / 2 > e) n--;
+      while ((n + 1) * (n + 2) / 2 <= e) n++;
+      on[k] = n;
+      os[k] = e - n * (n + 1) / 2;
+    }
+  }
+  const double r1 = -0.5 * (1 + cost), r2 = 0.5 * (1 - cost), r3 = -sint;
+  const double dr1 = 0.5 * sint, dr2 = dr1, dr3 = -cost;
+  constexpr int NV = ((2 * PMAX - 2) * (2 * PMAX - 2) + GT - 1) / GT;  // staged values per thread
+  __syncthreads();
+  for (int m = 0; m < q; m++) {
+    // (a) forward rotations of this level: R M, dR M, dP M for rows n in [m, q)
+    {
+      const int g = tid / XG, lane = tid % XG;
+      for (int n = m + g; n < q; n += GT / XG) {
+        const cd *row = R + n * n + n, *drow = dR + n * n + n;
+        cd s0 = {0, 0}, s1 = {0, 0}, s2 = {0, 0};
+        for (int s = lane; s <= n; s += XG) {
+          if (s == 0) {
+            cd c = {hs[n * n], 0.0};
+            s0 = cadd(s0, cmul(row[0], c));
+            s1 = cadd(s1, cmul(drow[0], c));
+          } else {
+            cd c = {hs[n * n + 2 * s - 1], hs[n * n + 2 * s]};
+            s0 = cadd(s0, cadd(cmul(row[s], c), cmul(row[-s], cconj(c))));
+            s1 = cadd(s1, cadd(cmul(drow[s], c), cmul(drow[-s], cconj(c))));
+            cd t = cadd(cmul(row[s], derv(c, s)), cmul(row[-s], derv(cconj(c), -s)));
+            s2 = {s2.x - t.x, s2.y - t.y};
+          }
+        }
+        s0.x = seg_reduce(s0.x); s0.y = seg_reduce(s0.y);
+        s1.x = seg_reduce(s1.x); s1.y = seg_reduce(s1.y);
+        s2.x = seg_reduce(s2.x); s2.y = seg_reduce(s2.y);
+        if (lane == 0) {
+          if (m == 0) { s0.y = 0.0; s1.y = 0.0; s2.y = 0.0; }  // Mout(n,0) keeps the real part
+          a0[n] = s0;
+          a1[n] = s1;
+          a2[n] = s2;
+        }
+      }
+    }
+    __syncthreads();
+    // (b) translations of this level: b0 = dT a0, b1 = T a0, b2 = T a1, b3 = T a2
+    {
+      const int g = tid / XG, lane = tid % XG;
+      const int w = tq - m;
+      for (int n = m + g; n < q; n += GT / XG) {
+        const double *Tm = T + tmat_off(tq, m) + (size_t)(n - m) * w;
+        const double *dTm = dT + tmat_off(tq, m) + (size_t)(n - m) * w;
+        cd t0 = {0, 0}, t1 = {0, 0}, t2 = {0, 0}, t3 = {0, 0};
+        for (int l = m + lane; l < q; l += XG) {
+          t0 = cadd(t0, cscale(dTm[l - m], a0[l]));
+          t1 = cadd(t1, cscale(Tm[l - m], a0[l]));
+          t2 = cadd(t2, cscale(Tm[l - m], a1[l]));
+          t3 = cadd(t3, cscale(Tm[l - m], a2[l]));
+        }
+        t0.x = seg_reduce(t0.x); t0.y = seg_reduce(t0.y);
+        t1.x = seg_reduce(t1.x); t1.y = seg_reduce(t1.y);
+        t2.x = seg_reduce(t2.x); t2.y = seg_reduce(t2.y);
+        t3.x = seg_reduce(t3.x); t3.y = seg_reduce(t3.y);
+        if (lane == 0) {
+          b0[n] = t0;
+          b1[n] = t1;
+          b2[n] = t2;
+          b3[n] = t3;
+        }
+      }
+    }
+    __syncthreads();
+    // (c) back-rotation contributions of level m to the owned outputs (n, s), n >= m
+    for (int k = 0; k < 2; k++) {
+      const int n = on[k], s = os[k];
+      if (n < m) continue;
+      const cd *row = R + n * n + n, *drow = dR + n * n + n;
+      if (m == 0) {
+        const cd c0 = {b0[n].x, 0.0}, c1 = {b1[n].x, 0.0}, c2 = {b2[n].x, 0.0}, c3 = {b3[n].x, 0.0};
+        aGr[k] = cmul(cscale(1.0, row[-s]), c0);
+        aGt1[k] = cmul(cscale(1.0, drow[-s]), c1);
+        aGp1[k] = cmul(cscale(1.0, row[-s]), derv(c1, s));
+        aGt2[k] = cmul(cscale(1.0, row[-s]), c2);
+        aGp2[k] = cmul(cscale(1.0, row[-s]), c3);
+      } else {
+        const cd Rp = row[s], Rm = row[-s], dRp = drow[s], dRm = drow[-s];
+        aGr[k] = cadd(aGr[k], cadd(cmul(cconj(Rp), b0[n]), cmul(Rm, cconj(b0[n]))));
+        aGt1[k] = cadd(aGt1[k], cadd(cmul(cconj(dRp), b1[n]), cmul(dRm, cconj(b1[n]))));
+        aGp1[k] = cadd(aGp1[k], cadd(cmul(cconj(Rp), derv(b1[n], s)), cmul(Rm, derv(cconj(b1[n]), s))));
+        aGt2[k] = cadd(aGt2[k], cadd(cmul(cconj(Rp), b2[n]), cmul(Rm, cconj(b2[n]))));
+        aGp2[k] = cadd(aGp2[k], cadd(cmul(cconj(Rp), b3[n]), cmul(Rm, cconj(b3[n]))));
+      }
+    }
+    // (d) advance ROT and dROT to level m+1 in place (rotcoeff.cpp:801-809, 838-848):
+    // stage the new rows in registers, then overwrite
+    if (m < q - 1) {
+      const int base = (m + 1) * (m + 1);
+      const int total = (2 * q - m - 2) * (2 * q - m - 2) - (m + 1) * (m + 1);
+      cd vR[NV], vD[NV];
+      int vi[NV];
+      int cnt = 0;
+      for (int e = tid; e < total; e += GT, cnt++) {
+        int t = e + base;
+        int n = (int)sqrt((double)t);
+        while (n * n > t) n--;
+        while ((n + 1) * (n + 1) <= t) n++;
+        n += 1;
+        const int s = t - (n - 1) * (n - 1) - (n - 1);
+        const cd *row = R + n * n + n, *drow = dR + n * n + n;
+        const double e1 = ETA(tb, n, s - 1), e2 = ETA(tb, n, -s - 1), z3 = ZETA(tb, n - 1, abs(s));
+        const double em = ETA(tb, n, m);
+        cd d1 = cscale(r1 * e1, row[s - 1]);
+        cd d2 = cscale(r2 * e2, row[s + 1]);
+        cd d3 = cscale(r3 * z3, row[s]);
+        cd v = cmul(exxi, cadd(cadd(cmul(d1, exiphi), cmul(d2, exphi)), d3));
+        vR[cnt] = {v.x / em, v.y / em};
+        cd g1 = cscale(e1, cadd(cscale(r1, drow[s - 1]), cscale(dr1, row[s - 1])));
+        cd g2 = cscale(e2, cadd(cscale(r2, drow[s + 1]), cscale(dr2, row[s + 1])));
+        cd g3 = cscale(z3, cadd(cscale(r3, drow[s]), cscale(dr3, row[s])));
+        cd u = cmul(exxi, cadd(cadd(cmul(g1, exiphi), cmul(g2, exphi)), g3));
+        vD[cnt] = {u.x / em, u.y / em};
+        vi[cnt] = (n - 1) * (n - 1) + (n - 1) + s;
+      }
+      __syncthreads();
+      for (int k = 0; k < cnt; k++) {
+        R[vi[k]] = vR[k];
+        dR[vi[k]] = vD[k];
+      }
+    }
+    __syncthreads();
+  }
+  // outputs: Gt = dR^H T R M + R^H T dR M, Gp likewise; G3 = JR (Gr, Gt, Gp)
+  for (int k = 0; k < 2; k++) {
+    const int n = on[k], s = os[k];
+    if (n < 0) continue;
+    cd gr = aGr[k], gt = cadd(aGt1[k], aGt2[k]), gp = cadd(aGp1[k], aGp2[k]);
+    if (s == 0) {
+      const int i = n * n;
+      const double ar[3] = {gr.x * 1.0, gt.x * 1.0, gp.x * 1.0};
+      for (int d = 0; d < 3; d++) G[d * P2 + i] = JR[d][0] * ar[0] + JR[d][1] * ar[1] + JR[d][2] * ar[2];
+    } else {
+      const int i = n * n + 2 * s - 1;
+      const double re[3] = {gr.x, gt.x, gp.x}, im[3] = {gr.y, gt.y, gp.y};
+      for (int d = 0; d < 3; d++) {
+        G[d * P2 + i] = JR[d][0] * re[0] + JR[d][1] * re[1] + JR[d][2] * re[2];
+        G[d * P2 + i + 1] = JR[d][0] * im[0] + JR[d][1] * im[1] + JR[d][2] * im[2];
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// inner F/H iteration with three right-hand sides (the gradient of each sphere's
+// coefficients along x, y, z): solveFH with compute_FHbase(3) (expcenter.cpp:1080-1135,
+// 1142-1210), computeDev over the three columns and computeDev2(...)/3.
+// Right-hand sides come in as LF, LH [sphere entry][3][p^2]; F, H in/out.
+// Identity matrix: one wavefront per column, a shared stopping test per sweep.
+// General matrix: the three columns of every sweep from LDS.
+// ----------------------------------------------------------------------------
+struct S3Args {
+  const int *sph;          // sphere (for constants and the matrix) per workgroup
+  const int *ent;          // entry (for F/H/dev) per workgroup
+  const double *LF, *LH;   // [workgroup][3][p^2] right-hand sides
+  const double *consts;    // [sphere][10][p]
+  const double *const *imat;
+  double *F, *H;           // [entry][3][p^2]
+  double *dev;             // [entry]
+  int p;
+  unsigned long long *work;
+};
+
+template <int KW>
+__device__ void solve3_identity(const S3Args &A, int g, int e, double *red) {
+  const int p = A.p, P2 = p * p, lane = threadIdx.x % 64, d = threadIdx.x / 64;
+  const double *cs = A.consts + (size_t)g * 10 * p;
+  const double *cH1 = cs, *cF1 = cs + p, *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cH2 = cs + 4 * p,
+               *cF2 = cs + 5 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p, *cInvH = cs + 8 * p;
+  const size_t o = ((size_t)e * 3 + d) * P2, ol = ((size_t)blockIdx.x * 3 + d) * P2;
+  const int npair = p * (p + 1) / 2;
+  double Fr[KW], Fi[KW], Hr[KW], Hi[KW], fbr[KW], fbi[KW], hbr[KW], hbi[KW], oHr[KW], oHi[KW], uHr[KW], uHi[KW];
+  double kH1[KW], kF1[KW], kH2[KW], kF2[KW], kIH[KW];
+  int nn[KW], i0[KW];
+  for (int k = 0; k < KW; k++) {
+    const int ep = lane + 64 * k;
+    nn[k] = -1;
+    if (ep >= npair) continue;
+    int n = (int)((sqrt(8.0 * ep + 1.0) - 1.0) * 0.5);
+    while (n * (n + 1) / 2 > ep) n--;
+    while ((n + 1) * (n + 2) / 2 <= ep) n++;
+    const int m = ep - n * (n + 1) / 2;
+    nn[k] = n;
+    kH1[k] = cH1[n]; kF1[k] = cF1[n]; kH2[k] = cH2[n]; kF2[k] = cF2[n]; kIH[k] = cInvH[n];
+    const int i = (m == 0) ? n * n : n * n + 2 * m - 1;
+    i0[k] = i;
+    const bool two = (m != 0);
+    fbr[k] = cLH1[n] * A.LH[ol + i] + cLF1[n] * A.LF[ol + i];
+    hbr[k] = cLH2[n] * A.LH[ol + i] + cLF2[n] * A.LF[ol + i];
+    fbi[k] = two ? cLH1[n] * A.LH[ol + i + 1] + cLF1[n] * A.LF[ol + i + 1] : 0.0;
+    hbi[k] = two ? cLH2[n] * A.LH[ol + i + 1] + cLF2[n] * A.LF[ol + i + 1] : 0.0;
+    Fr[k] = A.F[o + i];
+    Hr[k] = A.H[o + i];
+    Fi[k] = two ? A.F[o + i + 1] : 0.0;
+    Hi[k] = two ? A.H[o + i + 1] : 0.0;
+    oHr[k] = uHr[k] = Hr[k];
+    oHi[k] = uHi[k] = Hi[k];
+  }
+  int ct = 0;
+  double dev = 1e300;
+  while (ct == 0 || (dev > 1e-8 && ct < 100)) {
+    double sum = 0.0;
+    for (int k = 0; k < KW; k++) {
+      if (nn[k] < 0) continue;
+      const int n = nn[k];
+      const bool mpos = (i0[k] != n * n);
+      const double inv = 1 / (double)(2 * n + 1);
+      double xr = fbr[k] + kH1[k] * Hr[k] + kF1[k] * Fr[k];
+      double xi = fbi[k] + kH1[k] * Hi[k] + kF1[k] * Fi[k];
+      Fr[k] = xr * inv;
+      Fi[k] = mpos ? xi * inv : 0.0;
+      xr = hbr[k] + kH2[k] * Hr[k] + kF2[k] * Fr[k];
+      xi = hbi[k] + kH2[k] * Hi[k] + kF2[k] * Fi[k];
+      Hr[k] = xr * kIH[k];
+      Hi[k] = mpos ? xi * kIH[k] : 0.0;
+      double sv;
+      if (dev_term(oHr[k], mpos ? oHi[k] : 0.0, Hr[k], mpos ? Hi[k] : 0.0, mpos, sv)) sum += sv;
+      oHr[k] = Hr[k];
+      oHi[k] = Hi[k];
+    }
+    sum = wave_sum(sum);
+    const int buf = (ct & 1) * 4;  // double-buffered partials: one barrier per sweep
+    if (lane == 0) red[buf + d] = sum;
+    __syncthreads();
+    dev = (red[buf] + red[buf + 1] + red[buf + 2]) / (4 * p * p);
+    ct++;
+  }
+  double td = 0.0, ts = 0.0;
+  for (int k = 0; k < KW; k++) {
+    if (nn[k] < 0) continue;
+    const int n = nn[k];
+    const bool mpos = (i0[k] != n * n);
+    const double w = mpos ? 2.0 : 1.0;
+    td += w * ((uHr[k] - Hr[k]) * (uHr[k] - Hr[k]));
+    ts += w * (uHr[k] * uHr[k] + Hr[k] * Hr[k]);
+    if (mpos) {
+      td += w * ((uHi[k] - Hi[k]) * (uHi[k] - Hi[k]));
+      ts += w * (uHi[k] * uHi[k] + Hi[k] * Hi[k]);
+    }
+    A.F[o + i0[k]] = Fr[k];
+    A.H[o + i0[k]] = Hr[k];
+    if (mpos) {
+      A.F[o + i0[k] + 1] = Fi[k];
+      A.H[o + i0[k] + 1] = Hi[k];
+    }
+  }
+  td = wave_sum(td);
+  ts = wave_sum(ts);
+  __syncthreads();
+  if (lane == 0) {
+    red[8 + d] = td;
+    red[12 + d] = ts;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double TD = red[8] + red[9] + red[10], TS = red[12] + red[13] + red[14];
+    A.dev[e] = TD / (4 * TS) / double(3);
+    if (A.work) {
+      atomicAdd(&A.work[0], (unsigned long long)(ct * 3 * 12.0 * P2));
+      atomicAdd(&A.work[1], (unsigned long long)(8.0 * 3 * 6 * P2));
+    }
+  }
+}
+
+constexpr int S3T = 192;  // three wavefronts
+__global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
+  extern __shared__ double sh[];
+  const int p = A.p, P2 = p * p, tid = threadIdx.x;
+  const int g = A.sph[blockIdx.x], e = A.ent[blockIdx.x];
+  const double *IM = A.imat[g];
+  if (!IM) {  // uniform across the workgroup; wave d takes column d
+    __shared__ double red[16];
+    if (p <= 10) solve3_identity<1>(A, g, e, red);
+    else if (p <= 15) solve3_identity<2>(A, g, e, red);
+    else if (p <= 22) solve3_identity<4>(A, g, e, red);
+    else solve3_identity<9>(A, g, e, red);
+    return;
+  }
+  // general matrix: F, H, X of the three columns in LDS
+  double *F = sh, *H = F + 3 * P2, *X = H + 3 * P2, *oldH = X + 3 * P2, *red = oldH + 3 * P2;
+  const double *cs = A.consts + (size_t)g * 10 * p;
+  const double *cH1 = cs, *cF1 = cs + p, *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cH2 = cs + 4 * p,
+               *cF2 = cs + 5 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p, *cInvH = cs + 8 * p;
+  const size_t o = (size_t)e * 3 * P2, ol = (size_t)blockIdx.x * 3 * P2;
+  constexpr int KPT = 16;  // 3 * 32^2 / 192
+  double fbv[KPT], hbv[KPT], outerH[KPT];
+  int nn[KPT];
+  for (int k = 0; k < KPT; k++) {
+    const int i = tid + k * S3T;
+    nn[k] = -1;
+    if (i >= 3 * P2) continue;
+    const int c = i % P2;
+    int n = (int)sqrt((double)c);
+    while (n * n > c) n--;
+    while ((n + 1) * (n + 1) <= c) n++;
+    nn[k] = n;
+    fbv[k] = cLH1[n] * A.LH[ol + i] + cLF1[n] * A.LF[ol + i];
+    hbv[k] = cLH2[n] * A.LH[ol + i] + cLF2[n] * A.LF[ol + i];
+    F[i] = A.F[o + i];
+    H[i] = A.H[o + i];
+    oldH[i] = H[i];
+    outerH[k] = H[i];
+  }
+  __syncthreads();
+  int ct = 0;
+  double dev = 1e300;
+  while (ct == 0 || (dev > 1e-8 && ct < 100)) {
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * S3T;
+      if (nn[k] < 0) continue;
+      X[i] = fbv[k] + cH1[nn[k]] * H[i] + cF1[nn[k]] * F[i];
+    }
+    __syncthreads();
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * S3T;
+      if (nn[k] < 0) continue;
+      const int d = i / P2, r = i % P2;
+      double y = 0.0;
+      for (int c = 0; c < P2; c++) y += IM[r + (size_t)c * P2] * X[d * P2 + c];
+      F[i] = y * (1 / (double)(2 * nn[k] + 1));
+    }
+    __syncthreads();
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * S3T;
+      if (nn[k] < 0) continue;
+      X[i] = hbv[k] + cH2[nn[k]] * H[i] + cF2[nn[k]] * F[i];
+    }
+    __syncthreads();
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * S3T;
+      if (nn[k] < 0) continue;
+      const int d = i / P2, r = i % P2;
+      double y = 0.0;
+      for (int c = 0; c < P2; c++) y += IM[r + (size_t)c * P2] * X[d * P2 + c];
+      H[i] = y * cInvH[nn[k]];
+    }
+    __syncthreads();
+    double sum = 0.0;
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * S3T;
+      if (nn[k] < 0) continue;
+      const int n = nn[k], mm = i % P2 - n * n;
+      if (mm > 0 && (mm % 2) == 0) continue;
+      double sv;
+      if (dev_term(oldH[i], (mm > 0) ? oldH[i + 1] : 0.0, H[i], (mm > 0) ? H[i + 1] : 0.0, mm != 0, sv))
+        sum += sv;
+    }
+    sum = block_sum(sum, red);
+    dev = sum / (4 * p * p);
+    for (int k = 0; k < KPT; k++) {
+      const int i = tid + k * S3T;
+      if (nn[k] < 0) continue;
+      oldH[i] = H[i];
+    }
+    ct++;
+    __syncthreads();
+  }
+  double td = 0.0, ts = 0.0;
+  for (int k = 0; k < KPT; k++) {
+    const int i = tid + k * S3T;
+    if (nn[k] < 0) continue;
+    const int mm = i % P2 - nn[k] * nn[k];
+    const double a = outerH[k], b = H[i], w = (mm == 0) ? 1.0 : 2.0;
+    td += w * ((a - b) * (a - b));
+    ts += w * (a * a + b * b);
+    A.F[o + i] = F[i];
+    A.H[o + i] = H[i];
+  }
+  td = block_sum(td, red);
+  ts = block_sum(ts, red);
+  if (tid == 0) {
+    A.dev[e] = td / (4 * ts) / double(3);
+    if (A.work) {
+      const double P4 = (double)P2 * P2;
+      atomicAdd(&A.work[0], (unsigned long long)(ct * 3 * (4 * P4 + 12.0 * P2)));
+      atomicAdd(&A.work[1], (unsigned long long)(ct * 16 * P4 + 8.0 * 3 * 6 * P2));
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// exposed-surface gradient charges (computeExposedSurfaceGradientFH expcenter.cpp:1378-1412):
+// per point the three F and three H gradient charges, stored per direction
+// ([3][points]), the squared norms of the F and H 3-vectors (the point-skip test of the
+// gradient re-expansion, triexpan.cpp:127-153), and per direction the totals of |charge|
+// (partials per workgroup; k_surfgrad_totals takes the max over directions).
+// ----------------------------------------------------------------------------
+struct GQArgs {
+  int sph;                 // global sphere
+  int ent;                 // entry of gF/gH
+  const double *xyz;
+  const int *pt_off, *pt_cnt, *nspx;
+  const double *consts;
+  const double *gF, *gH;   // [entry][3][p^2]
+  double *gSF, *gSH;       // [3][all points]
+  double *nF, *nH;         // [all points]
+  long long npts_all;
+  double *part;            // [chunk][6]
+  int p;
+};
+__global__ void __launch_bounds__(ST) k_surfgrad(Tabs tb, GQArgs A) {
+  extern __shared__ double sh[];
+  const int p = A.p, P2 = p * p, tid = threadIdx.x;
+  const int g = A.sph;
+  double *B = sh, *red = B + 6 * P2;   // gF[0..2], gH[0..2]
+  const int off = A.pt_off[g], cnt = A.pt_cnt[g];
+  const int ch = blockIdx.x;
+  for (int i = tid; i < 6 * P2; i += ST)
+    B[i] = (i < 3 * P2) ? A.gF[(size_t)A.ent * 3 * P2 + i] : A.gH[(size_t)A.ent * 3 * P2 + i - 3 * P2];
+  __syncthreads();
+  const double *cChgH = A.consts + (size_t)g * 10 * p + 9 * p;
+  const double dA = 4 * M_PI / double(A.nspx[g]);
+  double t[6] = {0, 0, 0, 0, 0, 0};
+  const int h = ch * ST + tid;
+  if (h < cnt) {
+    const double *x = A.xyz + 3 * (size_t)(off + h);
+    double rho, th, ph;
+    cart_to_sph(x[0], x[1], x[2], rho, th, ph);
+    const double cost = cos(th), sint = sin(th);
+    double sv[6] = {0, 0, 0, 0, 0, 0};
+    for (int m = 0; m < p; m++) {
+      sh_column(tb, p, m, cost, sint, ph, [&](int n, double re, double im) {
+        const double c2 = (2 * n + 1) / (4.0 * M_PI), chH = cChgH[n];
+        for (int b = 0; b < 6; b++) {
+          const double cc = (b < 3) ? c2 : chH;
+          const double *E = B + b * P2;
+          if (m == 0) sv[b] += E[n * n] * (re * cc);
+          else {
+            const int i = n * n + 2 * m - 1;
+            sv[b] += 2.0 * (E[i] * (re * cc) + E[i + 1] * (im * cc));
+          }
+        }
+      });
+    }
+    double nf = 0.0, nh = 0.0;
+    for (int d = 0; d < 3; d++) {
+      const double gf = sv[d] * dA, gh = sv[3 + d] * dA;
+      A.gSF[(size_t)d * A.npts_all + off + h] = gf;
+      A.gSH[(size_t)d * A.npts_all + off + h] = gh;
+      t[d] = fabs(gf);
+      t[3 + d] = fabs(gh);
+      nf += gf * gf;
+      nh += gh * gh;
+    }
+    A.nF[off + h] = nf;
+    A.nH[off + h] = nh;
+  }
+  for (int b = 0; b < 6; b++) {
+    const double v = block_sum(t[b], red);
+    if (tid == 0) A.part[6 * ch + b] = v;
+  }
+}
+__global__ void k_surfgrad_totals(int nchunk, const double *part, int g, double *maxgF, double *maxgH) {
+  if (threadIdx.x != 0) return;
+  double t[6] = {0, 0, 0, 0, 0, 0};
+  for (int c = 0; c < nchunk; c++)
+    for (int b = 0; b < 6; b++) t[b] += part[6 * c + b];
+  double mf = (t[0] > t[1] ? t[0] : t[1]);
+  if (t[2] > mf) mf = t[2];
+  double mh = (t[3] > t[4] ? t[3] : t[4]);
+  if (t[5] > mh) mh = t[5];
+  maxgF[g] = mf;
+  maxgH[g] = mh;
+}
+
+// out = a + b over n doubles
+__global__ void __launch_bounds__(256) k_add(const double *a, const double *b, double *out, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = a[i] + b[i];
+}
+
+// ----------------------------------------------------------------------------
+// per-sphere force (computeMol_Force_and_Torque moldynamics.cpp:525-550 with
+// computeForceOn_0 expcenter.cpp:124-149): one thread per (sphere, direction), the
+// inner products in the reference's order (expansion.h:706-741).
+//   polarized:      f_d = -<H, gLHN_d> - <gH_d, LS>
+//   interact-only:  f_d = -<Hself, gLHN_d>
+// ----------------------------------------------------------------------------
+__device__ double inprod_blk(const double *M, const double *L, int p) {
+  double sum = 0.0;
+  for (int n = 0; n < p; n++) {
+    double t = 0.0;
+    for (int mm = 1; mm < 2 * n + 1; mm++) t += M[n * n + mm] * L[n * n + mm];
+    t *= 2.0;
+    t += M[n * n] * L[n * n];
+    sum += 1.0 * t;
+  }
+  return sum;
+}
+__global__ void __launch_bounds__(64) k_force(const int *sph, const int *kind, int n, const double *H,
+                                             const double *Hself, const double *LS, const double *gLHN,
+                                             const double *gH, double *f, int p) {
+  const int e = blockIdx.x * 64 + threadIdx.x;
+  if (e >= 3 * n) return;
+  const int k = e / 3, d = e % 3, g = sph[k], P2 = p * p;
+  double v = 0.0;
+  if (kind[k] == 1) {
+    v -= inprod_blk(H + (size_t)g * P2, gLHN + ((size_t)g * 3 + d) * P2, p);
+    v -= inprod_blk(gH + ((size_t)g * 3 + d) * P2, LS + (size_t)g * P2, p);
+  } else {
+    v -= inprod_blk(Hself + (size_t)g * P2, gLHN + ((size_t)g * 3 + d) * P2, p);
+  }
+  f[3 * k + d] = v;
+}
+
+}  // namespace pbsam

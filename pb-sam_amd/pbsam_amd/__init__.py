@@ -31,7 +31,8 @@ SYMBOLS = [
     "pbsam_system_set_pose", "pbsam_system_set_cutoffs", "pbsam_solve_mutual", "pbsam_mutual_rounds",
     "pbsam_system_get", "pbsam_energy", "pbsam_system_counts", "pbsam_xform_h_batch",
     "pbsam_xform_intra_batch", "pbsam_xform_num_batch", "pbsam_rotate_quat_batch",
-    "pbsam_profile", "pbsam_profile_read", "pbsam_sync",
+    "pbsam_profile", "pbsam_profile_read", "pbsam_sync", "pbsam_gradient", "pbsam_forces",
+    "pbsam_compute_forces", "pbsam_system_get_grad", "pbsam_xform_h_grad_batch",
 ]
 
 KERNELS = {"xform": 0, "p2l": 1, "solve": 2, "surface": 3}
@@ -96,6 +97,11 @@ def load():
         "pbsam_xform_intra_batch": ([P, I, I, P, P, P, P, P, P, P], I),
         "pbsam_xform_num_batch": ([P, I, I, P, P, I, P, P, P, D, D, P, P], I),
         "pbsam_rotate_quat_batch": ([P, I, I, P, I, P, P], I),
+        "pbsam_gradient": ([P, D, I, P], I),
+        "pbsam_forces": ([P, P, P], I),
+        "pbsam_compute_forces": ([P, D, I, P, P, P], I),
+        "pbsam_system_get_grad": ([P, I, I, P, P], I),
+        "pbsam_xform_h_grad_batch": ([P, I, I, P, P, P, P, P], I),
         "pbsam_profile": ([P, I], I),
         "pbsam_profile_read": ([P, I, P, P, P, P], I),
         "pbsam_sync": ([P], I),
@@ -176,6 +182,14 @@ class Context:
         self.check(self.L.pbsam_xform_num_batch(self.h, n, p, _p(P3), _p(a), len(qF), _p(xyz), _p(qF), _p(qH),
                                                 tQF, tQH, _p(oF), _p(oH)))
         return oF, oH
+
+    def xform_h_grad(self, P3, a, order, blocks, p):
+        P3, a, blocks = f64(P3), f64(a), f64(blocks)
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        n = len(order)
+        out = np.zeros((n, 3 * p * p))
+        self.check(self.L.pbsam_xform_h_grad_batch(self.h, n, p, _p(P3), _p(a), _p(order), _p(blocks), _p(out)))
+        return out
 
     def rotate_quat(self, quats, fwd, blocks, p):
         quats, blocks = f64(quats), f64(blocks)
@@ -312,6 +326,30 @@ class System:
         a = [np.zeros(self.p * self.p) for _ in range(4)]
         self.ctx.check(self.ctx.L.pbsam_system_get(self.h, i, k, *[_p(x) for x in a]))
         return dict(F=a[0], H=a[1], LS=a[2], Hrot=a[3])
+
+    def gradient(self, tol=0.0, max_rounds=0):
+        """Gradient phase in the reference's order; returns the number of passes."""
+        r = ctypes.c_int()
+        self.ctx.check(self.ctx.L.pbsam_gradient(self.h, tol, max_rounds, ctypes.byref(r)))
+        return r.value
+
+    def forces(self):
+        f, t = np.zeros(3 * self.n), np.zeros(3 * self.n)
+        self.ctx.check(self.ctx.L.pbsam_forces(self.h, _p(f), _p(t)))
+        return f.reshape(-1, 3), t.reshape(-1, 3)
+
+    def compute_forces(self, tol=0.0, max_rounds=0):
+        """computeForces: returns (rounds, status, force[n,3], torque[n,3])."""
+        f, t, r = np.zeros(3 * self.n), np.zeros(3 * self.n), ctypes.c_int()
+        rc = self.ctx.L.pbsam_compute_forces(self.h, tol, max_rounds, _p(f), _p(t), ctypes.byref(r))
+        if rc not in (OK, ERR_NOCONV):
+            self.ctx.check(rc)
+        return r.value, rc, f.reshape(-1, 3), t.reshape(-1, 3)
+
+    def get_grad(self, i, k):
+        gH, gL = np.zeros(3 * self.p * self.p), np.zeros(3 * self.p * self.p)
+        self.ctx.check(self.ctx.L.pbsam_system_get_grad(self.h, i, k, _p(gH), _p(gL)))
+        return gH, gL
 
     def energy(self):
         e = np.zeros(self.n)

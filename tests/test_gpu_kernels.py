@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from conftest import relerr
+from conftest import relerr, rot_close
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-11
@@ -98,3 +98,24 @@ def test_rotate_quaternion(gpu, orc, p):
             for k in range(p):
                 err = np.abs(got[i, k * k:(k + 1) * (k + 1)] - ref[k * k:(k + 1) * (k + 1)]).max()
                 assert err <= max(TOL, 1e-15 * 4.0 ** k) * scale, (i, fwd, k)
+
+
+@pytest.mark.parametrize("p", [10, 30])
+def test_xform_h_grad(gpu, orc, p):
+    """Gradient re-expansion (k_xgrad + k_trans grad pass) against the oracle, including
+    the singular directions theta = 0 and pi (closed forms with the Q table)."""
+    rng = np.random.default_rng(300 + p)
+    n = 8
+    P, a = pairs(rng, n)
+    orders = rng.integers(1, p + 1, n)
+    orders[:3] = p
+    scale = 0.6 ** np.repeat(np.arange(p), 2 * np.arange(p) + 1)   # decaying like physical blocks
+    blocks = rng.normal(size=(n, p * p)) * scale
+    got = gpu.xform_h_grad(P.ravel(), a.ravel(), orders, blocks, p)
+    for i in range(n):
+        q = int(orders[i])
+        r = orc.xform_h_grad(P[i], a[i, 0], a[i, 1], q, blocks[i, :q * q]).reshape(3, q * q)
+        for d in range(3):
+            ref = np.zeros(p * p)
+            ref[:q * q] = r[d]
+            assert rot_close(got[i, d * p * p:(d + 1) * p * p], ref, p, 1e-10), (i, q, d)
