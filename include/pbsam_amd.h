@@ -181,7 +181,9 @@ int pbsam_rotate_quat_batch(pbsam_ctx *ctx, int n, int p, const double *quat, in
 #define PBSAM_K_P2L 1     /* numerical re-expansion of surface charges */
 #define PBSAM_K_SOLVE 2   /* per-sphere inner F/H iteration */
 #define PBSAM_K_SURFACE 3 /* exposed surface charges */
-#define PBSAM_K_COUNT 4
+#define PBSAM_K_XGRAD 4   /* gradient re-expansion, one workgroup per pair */
+#define PBSAM_K_SURFGRAD 5 /* exposed-surface gradient charges */
+#define PBSAM_K_COUNT 6
 int pbsam_profile(pbsam_ctx *ctx, int on);
 int pbsam_profile_read(pbsam_ctx *ctx, int kernel, long long *launches, double *ms, double *flops,
                        double *bytes);

@@ -382,7 +382,9 @@ static void launch_rotapply(pbsam_ctx *c, int n, const RAJob *jobs, const RJob *
 }
 static void launch_sumslots(pbsam_ctx *c, int n, const LArgs &a) {
   if (!n) return;
-  hipLaunchKernelGGL(k_sumslots, dim3(n), dim3(256), 0, c->stream, a);
+  const int bs = a.bs ? a.bs : a.p * a.p;
+  const int chunks = (bs + 255) / 256;
+  hipLaunchKernelGGL(k_sumslots, dim3(n, chunks), dim3(256), 0, c->stream, a);
   HIPCHK(hipGetLastError());
 }
 
@@ -491,6 +493,11 @@ struct pbsam_moltype {
   std::vector<std::vector<int>> ana;
   std::vector<const double *> tpair;  // [ki*ns+kj] translation matrices (H then F) of the pair,
                                       // null where the re-expansion is numerical
+  // numerical pairs as stored linear maps (built on first use): [ki*ns+kj] offset in d_p2lm
+  // of the F matrix (npts(kj) x p^2, H after it), -1 for analytical pairs
+  std::vector<long long> p2l_off;
+  DBuf<double> d_p2lm;
+  DBuf<double> d_ysurf;  // harmonics of every exposed point about its sphere centre [point][p^2]
   // near entries per target (source, analytical job or numerical job) and by source
   struct Entry {
     int src, xj, pj;
@@ -988,6 +995,12 @@ static void moltype_values(pbsam_moltype *m) {
     const SphHost &Si = m->sph[ki];
     for (int kj : Si.far) {
       const SphHost &Sj = m->sph[kj];
+      {
+        double d[3] = {Si.cen[0] - Sj.cen[0], Si.cen[1] - Sj.cen[1], Si.cen[2] - Sj.cen[2]};
+        const double sep = sqrt(dot3(d, d)) - Si.rad - Sj.rad;
+        if (!(sep > 0 && !(sep < 5.0)))
+          fail(PBSAM_ERR_UNSUPPORTED, "a pair beyond the intra cutoff lies within 5 A: raise intra_rcutoff");
+      }
       XJob J{};
       J.src = kj;
       J.tgt = ki;
@@ -1115,6 +1128,9 @@ struct GradState {
   DBuf<PJob> pj0, pj1, pj2;
   DBuf<RAJob> ra, ra2;
   DBuf<int> ia, ib, ic, id, ie, tqmap;
+  DBuf<MJob> mj;
+  DBuf<double> mpart;
+  DBuf<int> isoff, psoff;  // [entry][2] slices of the sum lists
 };
 
 struct pbsam_system {

@@ -813,6 +813,29 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// a double moved across lanes by a DPP control (lanes without a source read 0.0)
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Sum over the 64 lanes of a wavefront with DPP row shifts and row broadcasts (gfx9
+// family), the total broadcast to every lane.  Fixed order: deterministic.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_d<0x111, 0xf>(v);   // row_shr:1
+  v += dpp_d<0x112, 0xf>(v);   // row_shr:2
+  v += dpp_d<0x114, 0xf>(v);   // row_shr:4
+  v += dpp_d<0x118, 0xf>(v);   // row_shr:8   -> lane 15 of each row holds the row sum
+  v += dpp_d<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+  v += dpp_d<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3 -> lane 63 holds the total
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // computeDev term of one (n, m) pair given its old (ar, ai) and new (br, bi) values
 // (expcenter.cpp:1270-1328); returns false where the pair is skipped
 __device__ __forceinline__ bool dev_term(double ar, double ai, double br, double bi, bool mpos, double &s) {
@@ -917,7 +940,7 @@ __device__ void solve_identity(const SArgs &A, int g, int lst0, int lst1) {
       oHr[k] = Hr[k];
       oHi[k] = Hi[k];
     }
-    sum = wave_sum(sum);
+    sum = wave_sum_dpp(sum);
     dev = sum / (4 * p * p);
     ct++;
   }
@@ -963,6 +986,7 @@ __global__ void __launch_bounds__(ST) k_solve(SArgs A) {
       if (p <= 10) solve_identity<1>(A, g, l0, l1);
       else if (p <= 15) solve_identity<2>(A, g, l0, l1);
       else if (p <= 22) solve_identity<4>(A, g, l0, l1);
+      else if (p <= 31) solve_identity<8>(A, g, l0, l1);
       else solve_identity<9>(A, g, l0, l1);
     }
     return;
@@ -1240,11 +1264,12 @@ struct LArgs {
   int p;
   int bs;                // doubles per slot and per output (0: p^2)
 };
+// grid (entries, element chunks of 256)
 __global__ void __launch_bounds__(256) k_sumslots(LArgs A) {
   const int e0 = A.sel ? A.sel[blockIdx.x] : blockIdx.x;
   const int p = A.p, P2 = A.bs ? A.bs : p * p, g = A.sph[e0];
   const int l0 = A.lst_off[e0], l1 = A.lst_off[e0 + 1];
-  for (int i = threadIdx.x; i < P2; i += 256) {
+  for (int i = blockIdx.y * 256 + threadIdx.x; i < P2; i += 256 * gridDim.y) {
     double s = A.base ? A.base[(size_t)g * P2 + i] : 0.0;
     for (int e = l0; e < l1; e++) s += A.slot[(size_t)A.lst[e] * P2 + i];
     A.out[(size_t)g * P2 + i] = s;

@@ -519,7 +519,7 @@ __device__ void solve3_identity(const S3Args &A, int g, int e, double *red) {
       oHr[k] = Hr[k];
       oHi[k] = Hi[k];
     }
-    sum = wave_sum(sum);
+    sum = wave_sum_dpp(sum);
     const int buf = (ct & 1) * 4;  // double-buffered partials: one barrier per sweep
     if (lane == 0) red[buf + d] = sum;
     __syncthreads();
@@ -574,6 +574,7 @@ __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
     if (p <= 10) solve3_identity<1>(A, g, e, red);
     else if (p <= 15) solve3_identity<2>(A, g, e, red);
     else if (p <= 22) solve3_identity<4>(A, g, e, red);
+    else if (p <= 31) solve3_identity<8>(A, g, e, red);
     else solve3_identity<9>(A, g, e, red);
     return;
   }
@@ -698,6 +699,7 @@ struct GQArgs {
   long long npts_all;
   double *part;            // [chunk][6]
   int p;
+  unsigned long long *work;
 };
 __global__ void __launch_bounds__(ST) k_surfgrad(Tabs tb, GQArgs A) {
   extern __shared__ double sh[];
@@ -750,12 +752,19 @@ __global__ void __launch_bounds__(ST) k_surfgrad(Tabs tb, GQArgs A) {
     const double v = block_sum(t[b], red);
     if (tid == 0) A.part[6 * ch + b] = v;
   }
+  if (A.work && tid == 0 && ch == 0) {
+    atomicAdd(&A.work[0], (unsigned long long)((7.0 * p * (p + 1) / 2 + 18.0 * p * p) * cnt));
+    atomicAdd(&A.work[1], (unsigned long long)(64.0 * cnt + 48.0 * P2));
+  }
 }
-__global__ void k_surfgrad_totals(int nchunk, const double *part, int g, double *maxgF, double *maxgH) {
-  if (threadIdx.x != 0) return;
+// one wavefront: lane l sums chunks l, l+64, ..., then the lanes are added
+__global__ void __launch_bounds__(64) k_surfgrad_totals(int nchunk, const double *part, int g, double *maxgF,
+                                                        double *maxgH) {
   double t[6] = {0, 0, 0, 0, 0, 0};
-  for (int c = 0; c < nchunk; c++)
+  for (int c = threadIdx.x; c < nchunk; c += 64)
     for (int b = 0; b < 6; b++) t[b] += part[6 * c + b];
+  for (int b = 0; b < 6; b++) t[b] = wave_sum(t[b]);
+  if (threadIdx.x != 0) return;
   double mf = (t[0] > t[1] ? t[0] : t[1]);
   if (t[2] > mf) mf = t[2];
   double mh = (t[3] > t[4] ? t[3] : t[4]);
@@ -802,6 +811,292 @@ __global__ void __launch_bounds__(64) k_force(const int *sph, const int *kind, i
     v -= inprod_blk(Hself + (size_t)g * P2, gLHN + ((size_t)g * 3 + d) * P2, p);
   }
   f[3 * k + d] = v;
+}
+
+// ----------------------------------------------------------------------------
+// Numerical re-expansion as a stored linear map.  For an intra-molecular pair whose
+// re-expansion is numerical (CXFormNIntra, xforms.cpp:514-582) the local expansion of a
+// unit charge at each exposed point of the source, about the target centre, depends on the
+// molecule's geometry only.  k_p2lmat stores it once per molecule type, point-major:
+//   M[h][c] = Y_c(x_h - P) * k^_n(kr) e^(-kr)/r (a_tgt/r)^n  (expansion.cpp:289-316)
+// for F (kappa = 0) and H (kappa) -- the terms CLocalExpan / CGradExpan add per point.
+// Applying it is then a streaming product over the points (k_p2lgemv), bounded by HBM.
+// ----------------------------------------------------------------------------
+struct MBuild {
+  long long moff;  // offset of the F matrix; H follows after npts * p^2
+  int npts, pt_off;
+  double P[3];     // target centre - source centre
+  double a_tgt;
+};
+// one wavefront per point; lane m evaluates harmonic column m
+__global__ void __launch_bounds__(256) k_p2lmat(Tabs tb, const MBuild *jobs, const double *xyz, double *mats,
+                                                int p) {
+  __shared__ double Y[4][PMAX * PMAX];
+  __shared__ double wF[4][PMAX], wH[4][PMAX];
+  const MBuild J = jobs[blockIdx.y];
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64, P2 = p * p;
+  const int h0 = blockIdx.x * 4 + w;
+  const bool valid = h0 < J.npts;
+  const int h = valid ? h0 : J.npts - 1;  // idle wavefronts recompute the last point, unstored
+  const double *x = xyz + 3 * (size_t)(J.pt_off + h);
+  const double dx = x[0] - J.P[0], dy = x[1] - J.P[1], dz = x[2] - J.P[2];
+  double rho, th, ph;
+  cart_to_sph(dx, dy, dz, rho, th, ph);
+  const double cost = cos(th), sint = sin(th);
+  if (lane < p) {
+    const int m = lane;
+    sh_column(tb, p, m, cost, sint, ph, [&](int n, double re, double im) {
+      if (m == 0) Y[w][n * n] = re;
+      else {
+        Y[w][n * n + 2 * m - 1] = re;
+        Y[w][n * n + 2 * m] = im;
+      }
+    });
+  }
+  if (lane == 63) {
+    // radial factors of a unit charge, F unscreened and H screened (CLExpan::init + BESSEL)
+    const double val = rho * tb.kappa;
+    double mrF = 1.0 / rho, mkF = 1.0;
+    mkF *= mrF * 1.0;
+    mrF *= J.a_tgt;
+    double mrH = 1.0 / rho, mkH = 1.0;
+    mkH *= mrH * exp(-val);
+    mrH *= J.a_tgt;
+    double km2 = 1.0, km1 = 1.0 + val;
+    const double vs = val * val;
+    for (int n = 0; n < p; n++) {
+      double K;
+      if (val == 0.0 || n == 0) K = 1.0;
+      else if (n == 1) K = km1;
+      else {
+        K = bessel_k_next(tb, n, km1, km2, vs);
+        km2 = km1;
+        km1 = K;
+      }
+      wF[w][n] = 1.0 * mkF;
+      wH[w][n] = K * mkH;
+      mkF *= mrF;
+      mkH *= mrH;
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  double *MF = mats + J.moff + (size_t)h * P2;
+  double *MH = mats + J.moff + (size_t)J.npts * P2 + (size_t)h * P2;
+  for (int c = lane; c < P2; c += 64) {
+    int n = (int)sqrt((double)c);
+    while (n * n > c) n--;
+    while ((n + 1) * (n + 1) <= c) n++;
+    MF[c] = Y[w][c] * wF[w][n];
+    MH[c] = Y[w][c] * wH[w][n];
+  }
+}
+
+// out[r][c] = sum over points h of M[h][c] q_r[h] for the F and H matrices of each job, c
+// below the job's order squared (orders from the source totals, CXFormBase::computeOrder);
+// points whose charge 3-vector has squared norm < 1e-15 are left out (triexpan.cpp:127-153).
+// Grid (job, point split); partial sums go to part and k_p2lsum adds them in split order.
+struct MJob {
+  long long moff;
+  int npts, pt_off;   // points of the source (global index into the charge planes)
+  int slot;           // output block of r = 0 (blocks slot .. slot+R-1)
+  int src;            // source sphere (totals)
+  double rho, a_src;
+};
+struct MArgs {
+  const MJob *jobs;
+  const double *mats;
+  const double *qF, *qH;     // charges, R planes of `plane` doubles
+  long long plane;
+  const double *nF, *nH;     // squared norms for the skip test (null: none)
+  const double *TQF, *TQH;
+  int nsplit, p;
+  double *part;              // [job][split][2][R][p^2]
+  unsigned long long *work;
+};
+constexpr int MT = 256;
+constexpr int MPTS = 32;  // points per workgroup (one split)
+template <int R>
+__global__ void __launch_bounds__(MT) k_p2lgemv(MArgs A) {
+  __shared__ double qs[MPTS][2 * R];
+  const MJob J = A.jobs[blockIdx.x];
+  const int p = A.p, P2 = p * p, tid = threadIdx.x;
+  const int pF = compute_order(J.rho, A.TQF[J.src], J.a_src, p);
+  const int pH = compute_order(J.rho, A.TQH[J.src], J.a_src, p);
+  const int nF = pF * pF, nH = pH * pH;
+  const int h0 = blockIdx.y * MPTS, h1 = min(J.npts, h0 + MPTS);
+  double *out = A.part + (((size_t)blockIdx.x * A.nsplit + blockIdx.y) * 2) * R * P2;
+  // stage this split's charges (zero where the point is skipped)
+  for (int e = tid; e < MPTS * 2 * R; e += MT) {
+    const int t = e / (2 * R), v = e % (2 * R), h = h0 + t;
+    double q = 0.0;
+    if (h < h1) {
+      const size_t pt = (size_t)J.pt_off + h;
+      if (v < R) q = (A.nF && A.nF[pt] < 1e-15) ? 0.0 : A.qF[v * A.plane + pt];
+      else q = (A.nH && A.nH[pt] < 1e-15) ? 0.0 : A.qH[(v - R) * A.plane + pt];
+    }
+    qs[t][v] = q;
+  }
+  __syncthreads();
+  const double *MF = A.mats + J.moff, *MH = MF + (size_t)J.npts * P2;
+  constexpr int KC = (PMAX * PMAX + MT - 1) / MT;  // coefficients per thread
+  for (int k = 0; k < KC; k++) {
+    const int c = tid + k * MT;
+    if (c >= P2) break;
+    double aF[R], aH[R];
+    for (int r = 0; r < R; r++) aF[r] = aH[r] = 0.0;
+    if (c < nF) {
+      const double *col = MF + (size_t)h0 * P2 + c;
+      int h = h0;
+      for (; h + 4 <= h1; h += 4) {  // four independent loads in flight
+        const double m0 = col[0], m1 = col[P2], m2 = col[2 * P2], m3 = col[3 * P2];
+        const int t = h - h0;
+        for (int r = 0; r < R; r++)
+          aF[r] += qs[t][r] * m0 + qs[t + 1][r] * m1 + qs[t + 2][r] * m2 + qs[t + 3][r] * m3;
+        col += 4 * (size_t)P2;
+      }
+      for (; h < h1; h++, col += P2)
+        for (int r = 0; r < R; r++) aF[r] += qs[h - h0][r] * col[0];
+    }
+    if (c < nH) {
+      const double *col = MH + (size_t)h0 * P2 + c;
+      int h = h0;
+      for (; h + 4 <= h1; h += 4) {
+        const double m0 = col[0], m1 = col[P2], m2 = col[2 * P2], m3 = col[3 * P2];
+        const int t = h - h0;
+        for (int r = 0; r < R; r++)
+          aH[r] += qs[t][R + r] * m0 + qs[t + 1][R + r] * m1 + qs[t + 2][R + r] * m2 + qs[t + 3][R + r] * m3;
+        col += 4 * (size_t)P2;
+      }
+      for (; h < h1; h++, col += P2)
+        for (int r = 0; r < R; r++) aH[r] += qs[h - h0][R + r] * col[0];
+    }
+    for (int r = 0; r < R; r++) {
+      out[(size_t)r * P2 + c] = aF[r];
+      out[(size_t)(R + r) * P2 + c] = aH[r];
+    }
+  }
+  if (A.work && tid == 0) {
+    atomicAdd(&A.work[0], (unsigned long long)(2.0 * R * (nF + nH) * (h1 - h0 > 0 ? h1 - h0 : 0)));
+    atomicAdd(&A.work[1], (unsigned long long)(8.0 * (nF + nH) * (h1 - h0 > 0 ? h1 - h0 : 0)));
+  }
+}
+// slotF/slotH[J.slot + r][c] = sum over splits of the partials, in split order
+// grid (job, output chunks of 256)
+__global__ void __launch_bounds__(256) k_p2lsum(const MJob *jobs, int nsplit, int R, int p, const double *part,
+                                               double *slotF, double *slotH) {
+  const MJob J = jobs[blockIdx.x];
+  const int P2 = p * p;
+  for (int e = blockIdx.y * 256 + threadIdx.x; e < 2 * R * P2; e += 256 * gridDim.y) {
+    double s = 0.0;
+    for (int sp = 0; sp < nsplit; sp++) s += part[(((size_t)blockIdx.x * nsplit + sp) * 2) * R * P2 + e];
+    const int half = e / (R * P2), rc = e % (R * P2);
+    (half == 0 ? slotF : slotH)[(size_t)J.slot * P2 + rc] = s;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Exposed-surface gradient charges as a stored linear map.  The harmonics of each exposed
+// point about its own sphere centre are fixed by the molecule type; k_surfmat stores them
+// point-major (Y[h][c], one row per point, rows in the molecule type's point order), and
+// k_surfgemv forms the six charges of each point (F and H gradient along x, y, z) as
+//   q[h] = dA * sum_c Y[h][c] w_c E[c],  w_c = (2 if m > 0 else 1) * C_n,
+// C_n = (2n+1)/4pi for F and 1/(i^_n(ka)) (2n+1)/4pi for H (expcenter.cpp:1378-1412).
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_surfmat(Tabs tb, const double *xyz, long long npts, double *Y, int p) {
+  __shared__ double Ys[4][PMAX * PMAX];
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64, P2 = p * p;
+  const long long h0 = (long long)blockIdx.x * 4 + w;
+  const bool valid = h0 < npts;
+  const long long h = valid ? h0 : npts - 1;
+  const double *x = xyz + 3 * h;
+  double rho, th, ph;
+  cart_to_sph(x[0], x[1], x[2], rho, th, ph);
+  const double cost = cos(th), sint = sin(th);
+  if (lane < p) {
+    const int m = lane;
+    sh_column(tb, p, m, cost, sint, ph, [&](int n, double re, double im) {
+      if (m == 0) Ys[w][n * n] = re;
+      else {
+        Ys[w][n * n + 2 * m - 1] = re;
+        Ys[w][n * n + 2 * m] = im;
+      }
+    });
+  }
+  __syncthreads();
+  if (!valid) return;
+  for (int c = lane; c < P2; c += 64) Y[h * P2 + c] = Ys[w][c];
+}
+
+constexpr int SGR = 16;  // points (rows) per workgroup
+struct SGArgs {
+  int ent;                   // list entry of gF/gH
+  int sph;                   // global sphere (totals, points, constants)
+  const double *Y;           // rows of this sphere's points
+  int npts;                  // exposed points of the sphere
+  long long pt_glob;         // global index of its first point
+  const double *gF, *gH;     // [entry][3][p^2]
+  const double *cChgH;       // [p] of the sphere
+  double dA;
+  double *gSF, *gSH;         // [3][all points]
+  double *nF, *nH;
+  long long npts_all;
+  double *part;              // [workgroup][6]
+  int p;
+  unsigned long long *work;
+};
+__global__ void __launch_bounds__(256) k_surfgemv(SGArgs A) {
+  __shared__ double E[6][PMAX * PMAX];
+  __shared__ double red[8];
+  const int p = A.p, P2 = p * p, tid = threadIdx.x, w = tid / 64, lane = tid % 64;
+  for (int e = tid; e < 6 * P2; e += 256) {
+    const int b = e / P2, c = e % P2;
+    int n = (int)sqrt((double)c);
+    while (n * n > c) n--;
+    while ((n + 1) * (n + 1) <= c) n++;
+    const double C = (b < 3) ? (2 * n + 1) / (4.0 * M_PI) : A.cChgH[n];
+    const double v = (b < 3) ? A.gF[((size_t)A.ent * 3 + b) * P2 + c] : A.gH[((size_t)A.ent * 3 + b - 3) * P2 + c];
+    E[b][c] = (c == n * n ? 1.0 : 2.0) * (C * v);
+  }
+  __syncthreads();
+  double t[6] = {0, 0, 0, 0, 0, 0};
+  for (int rr = 0; rr < SGR / 4; rr++) {
+    const int h = blockIdx.x * SGR + w * (SGR / 4) + rr;
+    if (h >= A.npts) break;  // rows left to this wavefront are all past the end
+    const double *row = A.Y + (size_t)h * P2;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int c = lane; c < P2; c += 64) {
+      const double y = row[c];
+      for (int b = 0; b < 6; b++) s[b] += y * E[b][c];
+    }
+    for (int b = 0; b < 6; b++) s[b] = wave_sum(s[b]);
+    if (lane == 0) {
+      const size_t pt = (size_t)A.pt_glob + h;
+      double nf = 0.0, nh = 0.0;
+      for (int d = 0; d < 3; d++) {
+        const double gf = s[d] * A.dA, gh = s[3 + d] * A.dA;
+        A.gSF[(size_t)d * A.npts_all + pt] = gf;
+        A.gSH[(size_t)d * A.npts_all + pt] = gh;
+        t[d] += fabs(gf);
+        t[3 + d] += fabs(gh);
+        nf += gf * gf;
+        nh += gh * gh;
+      }
+      A.nF[pt] = nf;
+      A.nH[pt] = nh;
+    }
+  }
+  // per-workgroup totals over its rows, in row order within each wavefront
+  for (int b = 0; b < 6; b++) {
+    __syncthreads();
+    if (lane == 0) red[w] = t[b];
+    __syncthreads();
+    if (tid == 0) A.part[6 * blockIdx.x + b] = red[0] + red[1] + red[2] + red[3];
+  }
+  if (A.work && tid == 0 && blockIdx.x == 0) {
+    atomicAdd(&A.work[0], (unsigned long long)(12.0 * P2 * A.npts));
+    atomicAdd(&A.work[1], (unsigned long long)(8.0 * P2 * A.npts + 64.0 * A.npts));
+  }
 }
 
 }  // namespace pbsam

@@ -35,7 +35,7 @@ SYMBOLS = [
     "pbsam_compute_forces", "pbsam_system_get_grad", "pbsam_xform_h_grad_batch",
 ]
 
-KERNELS = {"xform": 0, "p2l": 1, "solve": 2, "surface": 3}
+KERNELS = {"xform": 0, "p2l": 1, "solve": 2, "surface": 3, "xgrad": 4, "surfgrad": 5}
 
 
 class PbsamError(RuntimeError):

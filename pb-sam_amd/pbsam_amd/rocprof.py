@@ -13,7 +13,8 @@ import re
 import sys
 from collections import defaultdict
 
-INSTRUMENTED = {"xform": "k_xform", "p2l": "k_p2l", "solve": "k_solve", "surface": "k_surface"}
+INSTRUMENTED = {"xform": ("k_xform",), "p2l": ("k_p2l",), "solve": ("k_solve", "k_solve3"),
+                "surface": ("k_surface",), "xgrad": ("k_xgrad",), "surfgrad": ("k_surfgrad",)}
 
 
 def short(name):
@@ -76,9 +77,9 @@ def main(argv):
     for key, k in line["kernels"].items():
         if not k["launches"]:
             continue
-        d = [x for name, v in agg.items() if name.split("<")[0] == INSTRUMENTED[key] for x in v]
+        d = [x for name, v in agg.items() if name.split("<")[0] in INSTRUMENTED[key] for x in v]
         ev = k["ms_per_step"] * line["steps"] / k["launches"] * 1e3
-        print("| %s | %d | %.1f | %d | %.1f |" % (INSTRUMENTED[key], len(d), sum(d) / max(len(d), 1) / 1e3,
+        print("| %s | %d | %.1f | %d | %.1f |" % ("/".join(INSTRUMENTED[key]), len(d), sum(d) / max(len(d), 1) / 1e3,
                                                  k["launches"], ev))
 
 
