@@ -855,38 +855,113 @@ __device__ __forceinline__ bool dev_term(double ar, double ai, double br, double
 }
 
 // Inner F/H iteration of a sphere whose surface-integral matrix is the identity: every
-// coefficient pair (F_i, H_i) evolves on its own, so one wavefront holds the block in
-// registers (lane l owns the (n, m) pairs l, l+64, ...) and only the stopping test is
-// reduced across lanes.  Same arithmetic per coefficient as the general path.
+// coefficient pair (F_i, H_i) evolves on its own, so the block lives in registers spread
+// over the workgroup's NW wavefronts (KW (n, m) pairs per lane) and only the stopping test
+// is reduced across lanes and wavefronts, once per sweep.  Same arithmetic per coefficient
+// as the general path (expcenter.cpp:1080-1135, computeDev :1270-1328).
+template <int KW>
+struct IdState {
+  double Fr[KW], Fi[KW], Hr[KW], Hi[KW], fbr[KW], fbi[KW], hbr[KW], hbi[KW], uHr[KW], uHi[KW];
+  double kH1[KW], kF1[KW], kH2[KW], kF2[KW], kIH[KW], kInv[KW];
+  int nn[KW], i0[KW];
+};
+// (n, m) pair index ep -> n, and the real index of c(n, m)
+__device__ __forceinline__ void pair_nm(int ep, int &n, int &i) {
+  n = (int)((sqrt(8.0 * ep + 1.0) - 1.0) * 0.5);
+  while (n * (n + 1) / 2 > ep) n--;
+  while ((n + 1) * (n + 2) / 2 <= ep) n++;
+  const int m = ep - n * (n + 1) / 2;
+  i = (m == 0) ? n * n : n * n + 2 * m - 1;
+}
+template <int KW>
+__device__ __forceinline__ void id_consts(IdState<KW> &S, int k, int n, const double *cs, int p) {
+  S.kH1[k] = cs[n];
+  S.kF1[k] = cs[p + n];
+  S.kH2[k] = cs[4 * p + n];
+  S.kF2[k] = cs[5 * p + n];
+  S.kIH[k] = cs[8 * p + n];
+  S.kInv[k] = 1 / (double)(2 * n + 1);
+}
+// sweeps until the summed computeDev of all NW wavefronts' pairs <= 1e-8 (or 100 sweeps);
+// red holds 2*NW doubles.  Returns the number of sweeps.
+template <int KW, int NW>
+__device__ int id_iterate(IdState<KW> &S, int p, double *red) {
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  double oHr[KW], oHi[KW];
+  for (int k = 0; k < KW; k++) {
+    oHr[k] = S.Hr[k];
+    oHi[k] = S.Hi[k];
+  }
+  int ct = 0;
+  double dev = 1e300;
+  while (ct == 0 || (dev > 1e-8 && ct < 100)) {
+    double sum = 0.0;
+    for (int k = 0; k < KW; k++) {
+      if (S.nn[k] < 0) continue;
+      const bool mpos = (S.i0[k] != S.nn[k] * S.nn[k]);
+      double xr = S.fbr[k] + S.kH1[k] * S.Hr[k] + S.kF1[k] * S.Fr[k];
+      double xi = S.fbi[k] + S.kH1[k] * S.Hi[k] + S.kF1[k] * S.Fi[k];
+      S.Fr[k] = xr * S.kInv[k];
+      S.Fi[k] = mpos ? xi * S.kInv[k] : 0.0;
+      xr = S.hbr[k] + S.kH2[k] * S.Hr[k] + S.kF2[k] * S.Fr[k];
+      xi = S.hbi[k] + S.kH2[k] * S.Hi[k] + S.kF2[k] * S.Fi[k];
+      S.Hr[k] = xr * S.kIH[k];
+      S.Hi[k] = mpos ? xi * S.kIH[k] : 0.0;
+      double sv;
+      if (dev_term(oHr[k], mpos ? oHi[k] : 0.0, S.Hr[k], mpos ? S.Hi[k] : 0.0, mpos, sv)) sum += sv;
+      oHr[k] = S.Hr[k];
+      oHi[k] = S.Hi[k];
+    }
+    sum = wave_sum_dpp(sum);
+    const int buf = (ct & 1) * NW;  // double-buffered partials: one barrier per sweep
+    if (lane == 0) red[buf + w] = sum;
+    __syncthreads();
+    double tot = 0.0;
+    for (int x = 0; x < NW; x++) tot += red[buf + x];
+    dev = tot / (4 * p * p);
+    ct++;
+  }
+  return ct;
+}
+// computeDev2 terms of this lane's pairs (expcenter.cpp:1336-1370)
+template <int KW>
+__device__ __forceinline__ void id_dev2(const IdState<KW> &S, double &td, double &ts) {
+  td = 0.0;
+  ts = 0.0;
+  for (int k = 0; k < KW; k++) {
+    if (S.nn[k] < 0) continue;
+    const bool mpos = (S.i0[k] != S.nn[k] * S.nn[k]);
+    const double w = mpos ? 2.0 : 1.0;
+    td += w * ((S.uHr[k] - S.Hr[k]) * (S.uHr[k] - S.Hr[k]));
+    ts += w * (S.uHr[k] * S.uHr[k] + S.Hr[k] * S.Hr[k]);
+    if (mpos) {
+      td += w * ((S.uHi[k] - S.Hi[k]) * (S.uHi[k] - S.Hi[k]));
+      ts += w * (S.uHi[k] * S.uHi[k] + S.Hi[k] * S.Hi[k]);
+    }
+  }
+}
+
+// one right-hand side: the workgroup's ST/64 wavefronts share the sphere's pairs
 template <int KW>
 __device__ void solve_identity(const SArgs &A, int g, int lst0, int lst1) {
-  const int p = A.p, P2 = p * p, lane = threadIdx.x;
+  constexpr int NW = ST / 64;
+  __shared__ double red[2 * NW + 2 * NW];
+  const int p = A.p, P2 = p * p;
   const double *cs = A.consts + (size_t)g * 10 * p;
-  const double *cH1 = cs, *cF1 = cs + p, *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cH2 = cs + 4 * p,
-               *cF2 = cs + 5 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p, *cInvH = cs + 8 * p;
-  // pair k of this lane: (n, m) with m >= 0; real index i0 = n^2 + 2m - 1 (m > 0) or n^2
+  const double *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p;
   const int npair = p * (p + 1) / 2;
-  double Fr[KW], Fi[KW], Hr[KW], Hi[KW], fbr[KW], fbi[KW], hbr[KW], hbi[KW], oHr[KW], oHi[KW], uHr[KW], uHi[KW];
-  double kH1[KW], kF1[KW], kH2[KW], kF2[KW], kIH[KW];  // per-order constants, held in registers
-  int nn[KW], i0[KW];
+  IdState<KW> S;
   for (int k = 0; k < KW; k++) {
-    const int e = lane + 64 * k;
-    nn[k] = -1;
-    if (e >= npair) continue;
-    int n = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-    while (n * (n + 1) / 2 > e) n--;
-    while ((n + 1) * (n + 2) / 2 <= e) n++;
-    const int m = e - n * (n + 1) / 2;
-    nn[k] = n;
-    kH1[k] = cH1[n];
-    kF1[k] = cF1[n];
-    kH2[k] = cH2[n];
-    kF2[k] = cF2[n];
-    kIH[k] = cInvH[n];
-    const int i = (m == 0) ? n * n : n * n + 2 * m - 1;
-    i0[k] = i;
-    const int cnt = (m == 0) ? 1 : 2;
-    double LH[2] = {0, 0}, LF[2] = {0, 0}, fb[2], hb[2];
+    const int ep = threadIdx.x + ST * k;
+    S.nn[k] = -1;
+    if (ep >= npair) continue;
+    int n, i;
+    pair_nm(ep, n, i);
+    S.nn[k] = n;
+    S.i0[k] = i;
+    id_consts(S, k, n, cs, p);
+    const int cnt = (i == n * n) ? 1 : 2;
+    double fb[2] = {0, 0}, hb[2] = {0, 0};
     for (int c = 0; c < cnt; c++) {
       const int ii = i + c;
       double lh = A.LS ? A.LS[(size_t)g * P2 + ii] : 0.0;
@@ -902,71 +977,48 @@ __device__ void solve_identity(const SArgs &A, int g, int lst0, int lst1) {
       }
       A.LFS[(size_t)g * P2 + ii] = lf;
       A.LHS[(size_t)g * P2 + ii] = lh;
-      LH[c] = lh;
-      LF[c] = lf;
-      fb[c] = A.Vfix[(size_t)g * P2 + ii] + cLH1[n] * LH[c] + cLF1[n] * LF[c];
-      hb[c] = A.Dfix[(size_t)g * P2 + ii] + cLH2[n] * LH[c] + cLF2[n] * LF[c];
+      fb[c] = A.Vfix[(size_t)g * P2 + ii] + cLH1[n] * lh + cLF1[n] * lf;
+      hb[c] = A.Dfix[(size_t)g * P2 + ii] + cLH2[n] * lh + cLF2[n] * lf;
     }
-    fbr[k] = fb[0];
-    hbr[k] = hb[0];
-    fbi[k] = (cnt == 2) ? fb[1] : 0.0;
-    hbi[k] = (cnt == 2) ? hb[1] : 0.0;
-    Fr[k] = A.F[(size_t)g * P2 + i];
-    Hr[k] = A.H[(size_t)g * P2 + i];
-    Fi[k] = (cnt == 2) ? A.F[(size_t)g * P2 + i + 1] : 0.0;
-    Hi[k] = (cnt == 2) ? A.H[(size_t)g * P2 + i + 1] : 0.0;
-    oHr[k] = uHr[k] = Hr[k];
-    oHi[k] = uHi[k] = Hi[k];
+    S.fbr[k] = fb[0];
+    S.hbr[k] = hb[0];
+    S.fbi[k] = (cnt == 2) ? fb[1] : 0.0;
+    S.hbi[k] = (cnt == 2) ? hb[1] : 0.0;
+    S.Fr[k] = A.F[(size_t)g * P2 + i];
+    S.Hr[k] = A.H[(size_t)g * P2 + i];
+    S.Fi[k] = (cnt == 2) ? A.F[(size_t)g * P2 + i + 1] : 0.0;
+    S.Hi[k] = (cnt == 2) ? A.H[(size_t)g * P2 + i + 1] : 0.0;
+    S.uHr[k] = S.Hr[k];
+    S.uHi[k] = S.Hi[k];
   }
-  int ct = 0;
-  double dev = 1e300;
-  while (ct == 0 || (dev > 1e-8 && ct < 100)) {
-    double sum = 0.0;
-    for (int k = 0; k < KW; k++) {
-      if (nn[k] < 0) continue;
-      const int n = nn[k];
-      const bool mpos = (i0[k] != n * n);
-      const double inv = 1 / (double)(2 * n + 1);
-      double xr = fbr[k] + kH1[k] * Hr[k] + kF1[k] * Fr[k];
-      double xi = fbi[k] + kH1[k] * Hi[k] + kF1[k] * Fi[k];
-      Fr[k] = xr * inv;
-      Fi[k] = mpos ? xi * inv : 0.0;
-      xr = hbr[k] + kH2[k] * Hr[k] + kF2[k] * Fr[k];
-      xi = hbi[k] + kH2[k] * Hi[k] + kF2[k] * Fi[k];
-      Hr[k] = xr * kIH[k];
-      Hi[k] = mpos ? xi * kIH[k] : 0.0;
-      double s;
-      if (dev_term(oHr[k], mpos ? oHi[k] : 0.0, Hr[k], mpos ? Hi[k] : 0.0, mpos, s)) sum += s;
-      oHr[k] = Hr[k];
-      oHi[k] = Hi[k];
-    }
-    sum = wave_sum_dpp(sum);
-    dev = sum / (4 * p * p);
-    ct++;
-  }
-  double td = 0.0, ts = 0.0;
+  const int ct = id_iterate<KW, NW>(S, p, red);
+  double td, ts;
+  id_dev2(S, td, ts);
   for (int k = 0; k < KW; k++) {
-    if (nn[k] < 0) continue;
-    const int n = nn[k];
-    const bool mpos = (i0[k] != n * n);
-    const double w = mpos ? 2.0 : 1.0;
-    td += w * ((uHr[k] - Hr[k]) * (uHr[k] - Hr[k]));
-    ts += w * (uHr[k] * uHr[k] + Hr[k] * Hr[k]);
-    if (mpos) {
-      td += w * ((uHi[k] - Hi[k]) * (uHi[k] - Hi[k]));
-      ts += w * (uHi[k] * uHi[k] + Hi[k] * Hi[k]);
-    }
-    A.F[(size_t)g * P2 + i0[k]] = Fr[k];
-    A.H[(size_t)g * P2 + i0[k]] = Hr[k];
-    if (mpos) {
-      A.F[(size_t)g * P2 + i0[k] + 1] = Fi[k];
-      A.H[(size_t)g * P2 + i0[k] + 1] = Hi[k];
+    if (S.nn[k] < 0) continue;
+    const int i = S.i0[k];
+    A.F[(size_t)g * P2 + i] = S.Fr[k];
+    A.H[(size_t)g * P2 + i] = S.Hr[k];
+    if (i != S.nn[k] * S.nn[k]) {
+      A.F[(size_t)g * P2 + i + 1] = S.Fi[k];
+      A.H[(size_t)g * P2 + i + 1] = S.Hi[k];
     }
   }
   td = wave_sum(td);
   ts = wave_sum(ts);
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
   if (lane == 0) {
-    A.dev[g] = td / (4 * ts) / 1.0;
+    red[2 * NW + w] = td;
+    red[3 * NW + w] = ts;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double TD = 0.0, TS = 0.0;
+    for (int x = 0; x < NW; x++) {
+      TD += red[2 * NW + x];
+      TS += red[3 * NW + x];
+    }
+    A.dev[g] = TD / (4 * TS) / 1.0;
     if (A.sweeps) A.sweeps[g] = ct;
     if (A.work) {
       atomicAdd(&A.work[0], (unsigned long long)(ct * 12.0 * P2));
@@ -980,15 +1032,11 @@ __global__ void __launch_bounds__(ST) k_solve(SArgs A) {
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
   const int g = A.sph[blockIdx.x];
   if (!A.imat[g]) {  // uniform across the workgroup
-    if (tid < 64) {
-      const int l0 = A.lst_off[blockIdx.x], l1 = A.lst_off[blockIdx.x + 1];
-      // KW pairs per lane: 64*KW >= p(p+1)/2
-      if (p <= 10) solve_identity<1>(A, g, l0, l1);
-      else if (p <= 15) solve_identity<2>(A, g, l0, l1);
-      else if (p <= 22) solve_identity<4>(A, g, l0, l1);
-      else if (p <= 31) solve_identity<8>(A, g, l0, l1);
-      else solve_identity<9>(A, g, l0, l1);
-    }
+    const int l0 = A.lst_off[blockIdx.x], l1 = A.lst_off[blockIdx.x + 1];
+    // KW pairs per lane: ST * KW >= p(p+1)/2
+    if (p <= 22) solve_identity<1>(A, g, l0, l1);
+    else if (p <= 31) solve_identity<2>(A, g, l0, l1);
+    else solve_identity<3>(A, g, l0, l1);
     return;
   }
   double *F = sh, *H = F + P2, *X = H + P2, *red = X + P2;

@@ -462,99 +462,66 @@ struct S3Args {
   unsigned long long *work;
 };
 
+// three right-hand sides: WPC wavefronts per column, column d = wave / WPC
+constexpr int WPC = 4;
+constexpr int S3T = 3 * WPC * 64;
 template <int KW>
 __device__ void solve3_identity(const S3Args &A, int g, int e, double *red) {
-  const int p = A.p, P2 = p * p, lane = threadIdx.x % 64, d = threadIdx.x / 64;
+  constexpr int NW = 3 * WPC;
+  const int p = A.p, P2 = p * p, lane = threadIdx.x % 64, w = threadIdx.x / 64, d = w / WPC, sub = w % WPC;
   const double *cs = A.consts + (size_t)g * 10 * p;
-  const double *cH1 = cs, *cF1 = cs + p, *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cH2 = cs + 4 * p,
-               *cF2 = cs + 5 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p, *cInvH = cs + 8 * p;
+  const double *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p;
   const size_t o = ((size_t)e * 3 + d) * P2, ol = ((size_t)blockIdx.x * 3 + d) * P2;
   const int npair = p * (p + 1) / 2;
-  double Fr[KW], Fi[KW], Hr[KW], Hi[KW], fbr[KW], fbi[KW], hbr[KW], hbi[KW], oHr[KW], oHi[KW], uHr[KW], uHi[KW];
-  double kH1[KW], kF1[KW], kH2[KW], kF2[KW], kIH[KW];
-  int nn[KW], i0[KW];
+  IdState<KW> S;
   for (int k = 0; k < KW; k++) {
-    const int ep = lane + 64 * k;
-    nn[k] = -1;
+    const int ep = sub * 64 + lane + WPC * 64 * k;
+    S.nn[k] = -1;
     if (ep >= npair) continue;
-    int n = (int)((sqrt(8.0 * ep + 1.0) - 1.0) * 0.5);
-    while (n * (n + 1) / 2 > ep) n--;
-    while ((n + 1) * (n + 2) / 2 <= ep) n++;
-    const int m = ep - n * (n + 1) / 2;
-    nn[k] = n;
-    kH1[k] = cH1[n]; kF1[k] = cF1[n]; kH2[k] = cH2[n]; kF2[k] = cF2[n]; kIH[k] = cInvH[n];
-    const int i = (m == 0) ? n * n : n * n + 2 * m - 1;
-    i0[k] = i;
-    const bool two = (m != 0);
-    fbr[k] = cLH1[n] * A.LH[ol + i] + cLF1[n] * A.LF[ol + i];
-    hbr[k] = cLH2[n] * A.LH[ol + i] + cLF2[n] * A.LF[ol + i];
-    fbi[k] = two ? cLH1[n] * A.LH[ol + i + 1] + cLF1[n] * A.LF[ol + i + 1] : 0.0;
-    hbi[k] = two ? cLH2[n] * A.LH[ol + i + 1] + cLF2[n] * A.LF[ol + i + 1] : 0.0;
-    Fr[k] = A.F[o + i];
-    Hr[k] = A.H[o + i];
-    Fi[k] = two ? A.F[o + i + 1] : 0.0;
-    Hi[k] = two ? A.H[o + i + 1] : 0.0;
-    oHr[k] = uHr[k] = Hr[k];
-    oHi[k] = uHi[k] = Hi[k];
+    int n, i;
+    pair_nm(ep, n, i);
+    S.nn[k] = n;
+    S.i0[k] = i;
+    id_consts(S, k, n, cs, p);
+    const bool two = (i != n * n);
+    S.fbr[k] = cLH1[n] * A.LH[ol + i] + cLF1[n] * A.LF[ol + i];
+    S.hbr[k] = cLH2[n] * A.LH[ol + i] + cLF2[n] * A.LF[ol + i];
+    S.fbi[k] = two ? cLH1[n] * A.LH[ol + i + 1] + cLF1[n] * A.LF[ol + i + 1] : 0.0;
+    S.hbi[k] = two ? cLH2[n] * A.LH[ol + i + 1] + cLF2[n] * A.LF[ol + i + 1] : 0.0;
+    S.Fr[k] = A.F[o + i];
+    S.Hr[k] = A.H[o + i];
+    S.Fi[k] = two ? A.F[o + i + 1] : 0.0;
+    S.Hi[k] = two ? A.H[o + i + 1] : 0.0;
+    S.uHr[k] = S.Hr[k];
+    S.uHi[k] = S.Hi[k];
   }
-  int ct = 0;
-  double dev = 1e300;
-  while (ct == 0 || (dev > 1e-8 && ct < 100)) {
-    double sum = 0.0;
-    for (int k = 0; k < KW; k++) {
-      if (nn[k] < 0) continue;
-      const int n = nn[k];
-      const bool mpos = (i0[k] != n * n);
-      const double inv = 1 / (double)(2 * n + 1);
-      double xr = fbr[k] + kH1[k] * Hr[k] + kF1[k] * Fr[k];
-      double xi = fbi[k] + kH1[k] * Hi[k] + kF1[k] * Fi[k];
-      Fr[k] = xr * inv;
-      Fi[k] = mpos ? xi * inv : 0.0;
-      xr = hbr[k] + kH2[k] * Hr[k] + kF2[k] * Fr[k];
-      xi = hbi[k] + kH2[k] * Hi[k] + kF2[k] * Fi[k];
-      Hr[k] = xr * kIH[k];
-      Hi[k] = mpos ? xi * kIH[k] : 0.0;
-      double sv;
-      if (dev_term(oHr[k], mpos ? oHi[k] : 0.0, Hr[k], mpos ? Hi[k] : 0.0, mpos, sv)) sum += sv;
-      oHr[k] = Hr[k];
-      oHi[k] = Hi[k];
-    }
-    sum = wave_sum_dpp(sum);
-    const int buf = (ct & 1) * 4;  // double-buffered partials: one barrier per sweep
-    if (lane == 0) red[buf + d] = sum;
-    __syncthreads();
-    dev = (red[buf] + red[buf + 1] + red[buf + 2]) / (4 * p * p);
-    ct++;
-  }
-  double td = 0.0, ts = 0.0;
+  const int ct = id_iterate<KW, NW>(S, p, red);
+  double td, ts;
+  id_dev2(S, td, ts);
   for (int k = 0; k < KW; k++) {
-    if (nn[k] < 0) continue;
-    const int n = nn[k];
-    const bool mpos = (i0[k] != n * n);
-    const double w = mpos ? 2.0 : 1.0;
-    td += w * ((uHr[k] - Hr[k]) * (uHr[k] - Hr[k]));
-    ts += w * (uHr[k] * uHr[k] + Hr[k] * Hr[k]);
-    if (mpos) {
-      td += w * ((uHi[k] - Hi[k]) * (uHi[k] - Hi[k]));
-      ts += w * (uHi[k] * uHi[k] + Hi[k] * Hi[k]);
-    }
-    A.F[o + i0[k]] = Fr[k];
-    A.H[o + i0[k]] = Hr[k];
-    if (mpos) {
-      A.F[o + i0[k] + 1] = Fi[k];
-      A.H[o + i0[k] + 1] = Hi[k];
+    if (S.nn[k] < 0) continue;
+    const int i = S.i0[k];
+    A.F[o + i] = S.Fr[k];
+    A.H[o + i] = S.Hr[k];
+    if (i != S.nn[k] * S.nn[k]) {
+      A.F[o + i + 1] = S.Fi[k];
+      A.H[o + i + 1] = S.Hi[k];
     }
   }
   td = wave_sum(td);
   ts = wave_sum(ts);
   __syncthreads();
   if (lane == 0) {
-    red[8 + d] = td;
-    red[12 + d] = ts;
+    red[2 * NW + w] = td;
+    red[3 * NW + w] = ts;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const double TD = red[8] + red[9] + red[10], TS = red[12] + red[13] + red[14];
+    double TD = 0.0, TS = 0.0;
+    for (int x = 0; x < NW; x++) {
+      TD += red[2 * NW + x];
+      TS += red[3 * NW + x];
+    }
     A.dev[e] = TD / (4 * TS) / double(3);
     if (A.work) {
       atomicAdd(&A.work[0], (unsigned long long)(ct * 3 * 12.0 * P2));
@@ -563,19 +530,16 @@ __device__ void solve3_identity(const S3Args &A, int g, int e, double *red) {
   }
 }
 
-constexpr int S3T = 192;  // three wavefronts
 __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
   extern __shared__ double sh[];
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
   const int g = A.sph[blockIdx.x], e = A.ent[blockIdx.x];
   const double *IM = A.imat[g];
-  if (!IM) {  // uniform across the workgroup; wave d takes column d
-    __shared__ double red[16];
-    if (p <= 10) solve3_identity<1>(A, g, e, red);
-    else if (p <= 15) solve3_identity<2>(A, g, e, red);
-    else if (p <= 22) solve3_identity<4>(A, g, e, red);
-    else if (p <= 31) solve3_identity<8>(A, g, e, red);
-    else solve3_identity<9>(A, g, e, red);
+  if (!IM) {  // uniform across the workgroup
+    __shared__ double red[4 * 3 * WPC];
+    if (p <= 22) solve3_identity<1>(A, g, e, red);
+    else if (p <= 31) solve3_identity<2>(A, g, e, red);
+    else solve3_identity<3>(A, g, e, red);
     return;
   }
   // general matrix: F, H, X of the three columns in LDS
@@ -584,7 +548,7 @@ __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
   const double *cH1 = cs, *cF1 = cs + p, *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cH2 = cs + 4 * p,
                *cF2 = cs + 5 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p, *cInvH = cs + 8 * p;
   const size_t o = (size_t)e * 3 * P2, ol = (size_t)blockIdx.x * 3 * P2;
-  constexpr int KPT = 16;  // 3 * 32^2 / 192
+  constexpr int KPT = (3 * PMAX * PMAX + S3T - 1) / S3T;
   double fbv[KPT], hbv[KPT], outerH[KPT];
   int nn[KPT];
   for (int k = 0; k < KPT; k++) {
