@@ -9,6 +9,21 @@
  * Coefficient blocks use the reference layout (src/expansion.cpp:57-63): for
  * order n there are 2n+1 reals [c(n,0), Re c(n,1), Im c(n,1), ...] at offset n*n.
  * A block of order p holds p*p doubles.
+ *
+ * What pins each part (tests/test_oracle_ref.py, test_oracle_math.py,
+ * test_oracle_forces.py):
+ *   primitives, re-expansions (and their gradient), computeOrder, sphere points,
+ *   IMat of a buried sphere, molecule-type set-up and frozen intra values:
+ *     known-answer vectors of the reference compiled from its sources (p <= 15),
+ *     and at p = 30 closed forms (harmonics, Bessel, addition theorems, rotation
+ *     of charges, central differences);
+ *   orc_selfpol (reference order): Kirkwood's closed form for one sphere;
+ *   orc_mutual / orc_system_pot: Debye-Hueckel interaction of two spheres;
+ *   orc_gradient / orc_forces / orc_compute_forces: central differences of the energy.
+ * Parity unpinned: orc_system_total_int_energy (not on the product's ABI), and the
+ * Jacobi variants (orc_selfpol_round_jacobi, schedule 1 of orc_selfpol / orc_mutual /
+ * orc_gradient), which have no reference counterpart and serve only to check the
+ * device's Jacobi rounds against the same restatement.
  */
 #ifndef PBSAM_ORACLE_H
 #define PBSAM_ORACLE_H

@@ -882,8 +882,35 @@ __device__ __forceinline__ void id_consts(IdState<KW> &S, int k, int n, const do
   S.kIH[k] = cs[8 * p + n];
   S.kInv[k] = 1 / (double)(2 * n + 1);
 }
-// sweeps until the summed computeDev of all NW wavefronts' pairs <= 1e-8 (or 100 sweeps);
-// red holds 2*NW doubles.  Returns the number of sweeps.
+// one sweep of this lane's pairs; returns the lane's computeDev partial
+template <int KW>
+__device__ __forceinline__ double id_sweep(IdState<KW> &S, double (&oHr)[KW], double (&oHi)[KW]) {
+  double sum = 0.0;
+  for (int k = 0; k < KW; k++) {
+    if (S.nn[k] < 0) continue;
+    const bool mpos = (S.i0[k] != S.nn[k] * S.nn[k]);
+    double xr = S.fbr[k] + S.kH1[k] * S.Hr[k] + S.kF1[k] * S.Fr[k];
+    double xi = S.fbi[k] + S.kH1[k] * S.Hi[k] + S.kF1[k] * S.Fi[k];
+    S.Fr[k] = xr * S.kInv[k];
+    S.Fi[k] = mpos ? xi * S.kInv[k] : 0.0;
+    xr = S.hbr[k] + S.kH2[k] * S.Hr[k] + S.kF2[k] * S.Fr[k];
+    xi = S.hbi[k] + S.kH2[k] * S.Hi[k] + S.kF2[k] * S.Fi[k];
+    S.Hr[k] = xr * S.kIH[k];
+    S.Hi[k] = mpos ? xi * S.kIH[k] : 0.0;
+    double sv;
+    if (dev_term(oHr[k], mpos ? oHi[k] : 0.0, S.Hr[k], mpos ? S.Hi[k] : 0.0, mpos, sv)) sum += sv;
+    oHr[k] = S.Hr[k];
+    oHi[k] = S.Hi[k];
+  }
+  return sum;
+}
+// Sweeps until the summed computeDev of all NW wavefronts' pairs <= 1e-8 (or 100 sweeps),
+// the reference's loop (expcenter.cpp:1097-1135).  The stopping test is evaluated in
+// chunks of IC sweeps: each chunk runs IC sweeps from a saved state, exchanges the IC
+// per-sweep partials with one barrier, and if the loop ends inside the chunk restores the
+// state and reruns the sweeps up to the stopping one -- the same arithmetic in the same
+// order as sweep-by-sweep testing.  red holds 2 * IC * NW doubles.  Returns the sweeps.
+constexpr int IC = 10;  // divides the 100-sweep cap
 template <int KW, int NW>
 __device__ int id_iterate(IdState<KW> &S, int p, double *red) {
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
@@ -893,35 +920,44 @@ __device__ int id_iterate(IdState<KW> &S, int p, double *red) {
     oHi[k] = S.Hi[k];
   }
   int ct = 0;
-  double dev = 1e300;
-  while (ct == 0 || (dev > 1e-8 && ct < 100)) {
-    double sum = 0.0;
+  for (int chunk = 0;; chunk++) {
+    double sFr[KW], sFi[KW], sHr[KW], sHi[KW], sOr[KW], sOi[KW];
     for (int k = 0; k < KW; k++) {
-      if (S.nn[k] < 0) continue;
-      const bool mpos = (S.i0[k] != S.nn[k] * S.nn[k]);
-      double xr = S.fbr[k] + S.kH1[k] * S.Hr[k] + S.kF1[k] * S.Fr[k];
-      double xi = S.fbi[k] + S.kH1[k] * S.Hi[k] + S.kF1[k] * S.Fi[k];
-      S.Fr[k] = xr * S.kInv[k];
-      S.Fi[k] = mpos ? xi * S.kInv[k] : 0.0;
-      xr = S.hbr[k] + S.kH2[k] * S.Hr[k] + S.kF2[k] * S.Fr[k];
-      xi = S.hbi[k] + S.kH2[k] * S.Hi[k] + S.kF2[k] * S.Fi[k];
-      S.Hr[k] = xr * S.kIH[k];
-      S.Hi[k] = mpos ? xi * S.kIH[k] : 0.0;
-      double sv;
-      if (dev_term(oHr[k], mpos ? oHi[k] : 0.0, S.Hr[k], mpos ? S.Hi[k] : 0.0, mpos, sv)) sum += sv;
-      oHr[k] = S.Hr[k];
-      oHi[k] = S.Hi[k];
+      sFr[k] = S.Fr[k]; sFi[k] = S.Fi[k]; sHr[k] = S.Hr[k]; sHi[k] = S.Hi[k];
+      sOr[k] = oHr[k]; sOi[k] = oHi[k];
     }
-    sum = wave_sum_dpp(sum);
-    const int buf = (ct & 1) * NW;  // double-buffered partials: one barrier per sweep
-    if (lane == 0) red[buf + w] = sum;
+    double part[IC];
+    for (int t = 0; t < IC; t++) part[t] = id_sweep(S, oHr, oHi);
+    for (int t = 0; t < IC; t++) part[t] = wave_sum_dpp(part[t]);
+    double *rb = red + (chunk & 1) * IC * NW;  // double-buffered: one barrier per chunk
+    if (lane == 0)
+      for (int t = 0; t < IC; t++) rb[t * NW + w] = part[t];
     __syncthreads();
-    double tot = 0.0;
-    for (int x = 0; x < NW; x++) tot += red[buf + x];
-    dev = tot / (4 * p * p);
-    ct++;
+    int stop = -1;
+    for (int t = 0; t < IC; t++) {
+      double tot = 0.0;
+      for (int x = 0; x < NW; x++) tot += rb[t * NW + x];
+      const double dev = tot / (4 * p * p);
+      const int done = ct + t + 1;
+      if (!(dev > 1e-8 && done < 100)) {
+        stop = t;
+        break;
+      }
+    }
+    if (stop < 0) {
+      ct += IC;
+      continue;
+    }
+    if (stop < IC - 1) {
+      for (int k = 0; k < KW; k++) {
+        S.Fr[k] = sFr[k]; S.Fi[k] = sFi[k]; S.Hr[k] = sHr[k]; S.Hi[k] = sHi[k];
+        oHr[k] = sOr[k]; oHi[k] = sOi[k];
+      }
+      for (int t = 0; t <= stop; t++) (void)id_sweep(S, oHr, oHi);
+    }
+    ct += stop + 1;
+    return ct;
   }
-  return ct;
 }
 // computeDev2 terms of this lane's pairs (expcenter.cpp:1336-1370)
 template <int KW>
@@ -945,7 +981,7 @@ __device__ __forceinline__ void id_dev2(const IdState<KW> &S, double &td, double
 template <int KW>
 __device__ void solve_identity(const SArgs &A, int g, int lst0, int lst1) {
   constexpr int NW = ST / 64;
-  __shared__ double red[2 * NW + 2 * NW];
+  __shared__ double red[2 * IC * NW + 2 * NW];
   const int p = A.p, P2 = p * p;
   const double *cs = A.consts + (size_t)g * 10 * p;
   const double *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p;
@@ -1007,16 +1043,17 @@ __device__ void solve_identity(const SArgs &A, int g, int lst0, int lst1) {
   td = wave_sum(td);
   ts = wave_sum(ts);
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  double *r2 = red + 2 * IC * NW;
   if (lane == 0) {
-    red[2 * NW + w] = td;
-    red[3 * NW + w] = ts;
+    r2[w] = td;
+    r2[NW + w] = ts;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     double TD = 0.0, TS = 0.0;
     for (int x = 0; x < NW; x++) {
-      TD += red[2 * NW + x];
-      TS += red[3 * NW + x];
+      TD += r2[x];
+      TS += r2[NW + x];
     }
     A.dev[g] = TD / (4 * TS) / 1.0;
     if (A.sweeps) A.sweeps[g] = ct;

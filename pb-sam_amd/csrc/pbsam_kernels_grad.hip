@@ -510,17 +510,17 @@ __device__ void solve3_identity(const S3Args &A, int g, int e, double *red) {
   }
   td = wave_sum(td);
   ts = wave_sum(ts);
-  __syncthreads();
+  double *r2 = red + 2 * IC * NW;
   if (lane == 0) {
-    red[2 * NW + w] = td;
-    red[3 * NW + w] = ts;
+    r2[w] = td;
+    r2[NW + w] = ts;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     double TD = 0.0, TS = 0.0;
     for (int x = 0; x < NW; x++) {
-      TD += red[2 * NW + x];
-      TS += red[3 * NW + x];
+      TD += r2[x];
+      TS += r2[NW + x];
     }
     A.dev[e] = TD / (4 * TS) / double(3);
     if (A.work) {
@@ -536,7 +536,7 @@ __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
   const int g = A.sph[blockIdx.x], e = A.ent[blockIdx.x];
   const double *IM = A.imat[g];
   if (!IM) {  // uniform across the workgroup
-    __shared__ double red[4 * 3 * WPC];
+    __shared__ double red[2 * IC * 3 * WPC + 2 * 3 * WPC];
     if (p <= 22) solve3_identity<1>(A, g, e, red);
     else if (p <= 31) solve3_identity<2>(A, g, e, red);
     else solve3_identity<3>(A, g, e, red);

@@ -13,8 +13,8 @@ import re
 import sys
 from collections import defaultdict
 
-INSTRUMENTED = {"xform": ("k_xform",), "p2l": ("k_p2l",), "solve": ("k_solve", "k_solve3"),
-                "surface": ("k_surface",), "xgrad": ("k_xgrad",), "surfgrad": ("k_surfgrad",)}
+INSTRUMENTED = {"xform": ("k_xform",), "p2l": ("k_p2l", "k_p2lgemv"), "solve": ("k_solve", "k_solve3"),
+                "surface": ("k_surface",), "xgrad": ("k_xgrad",), "surfgrad": ("k_surfgrad", "k_surfgemv")}
 
 
 def short(name):
