@@ -104,6 +104,19 @@ int pbsam_moltype_get_intra(const pbsam_moltype *mt, int k, double *LF, double *
 int pbsam_system_create(pbsam_ctx *ctx, int n_mol, pbsam_moltype *const *mts, const double *pos,
                         const double *quat, pbsam_system **out);
 void pbsam_system_destroy(pbsam_system *sys);
+/* A batch of n_sys independent systems (BD snapshots or trajectories) held as one system
+ * whose molecules come in groups: system g has n_mol[g] molecules, listed after those of
+ * systems 0..g-1 in mts/pos/quat, and interacts only with itself.  Every call on a system
+ * handle works on a batch with molecule indices running over all groups; each group
+ * follows its own reference schedule exactly (its results equal those of a system made
+ * of that group alone), while the groups' Gauss-Seidel chains share kernel launches.
+ * The batched counterpart of calling CMolecule::computeForces (src/moldynamics.cpp:554-587)
+ * once per independent system; there is no single reference entry point for it. */
+int pbsam_batch_create(pbsam_ctx *ctx, int n_sys, const int *n_mol, pbsam_moltype *const *mts,
+                       const double *pos, const double *quat, pbsam_system **out);
+/* per-group results of the last solve / gradient (arrays of n_sys, any may be NULL):
+ * status (PBSAM_OK, PBSAM_ERR_OVERLAP, PBSAM_ERR_NOCONV), mutual rounds, gradient passes */
+int pbsam_batch_status(const pbsam_system *sys, int *status, int *mutual_rounds, int *grad_passes);
 int pbsam_system_set_pose(pbsam_system *sys, int i, const double *pos, const double *quat);
 int pbsam_system_set_cutoffs(pbsam_system *sys, double inter_rcutoff, double interact_rcutoff);
 
@@ -116,7 +129,9 @@ int pbsam_system_set_cutoffs(pbsam_system *sys, double inter_rcutoff, double int
  * 1000).  flags: PBSAM_HROT_REFRESH refreshes the rotated H used for inter-molecular
  * coupling after each update (the reference keeps round-1 values).
  * Returns PBSAM_ERR_OVERLAP when two molecules' spheres overlap and PBSAM_ERR_NOCONV
- * when the round limit ends the loop. */
+ * when the round limit ends the loop (for a batch: the first group's non-OK status; the
+ * other groups are solved, an overlapping group gets no pairs; rounds_out is the largest
+ * group's count). */
 #define PBSAM_HROT_REFRESH 1
 int pbsam_solve_mutual(pbsam_system *sys, double tol, int max_rounds, unsigned flags, int *rounds_out);
 /* round 1 from self values, then a fixed number of simultaneous (Jacobi) rounds */

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -1114,23 +1115,25 @@ struct MolInst {
   std::vector<std::vector<SphRef>> interPol, interact;
 };
 
-// device buffers of the gradient phase (grow-only, reused across calls)
+// device buffers of the gradient phase (grow-only, reused across calls).  "Entries" are
+// the gradient-list entries of every group being swept together; "items" are those groups
+// (per-item scratch holds one entry's intermediate blocks at a time).
 struct GradState {
-  DBuf<double> tG, tGF, tGH, tGHrot;       // [list entry][3][p^2]
+  DBuf<double> tG, tGF, tGH, tGHrot;       // [entry][3][p^2]
   DBuf<double> gSF, gSH, gnF, gnH;         // [3][points], [points]
   DBuf<double> maxgF, maxgH;               // [sphere]
   DBuf<double> gLHN, gH;                   // [sphere][3][p^2]
-  DBuf<double> slot, slotF, slotH;         // [slot][3][p^2]
-  DBuf<double> tGrot, LGF, LGH, LGH2;      // [3][p^2]
-  DBuf<double> T, dev, part;
-  DBuf<XJob> xj, xj2;
-  DBuf<TJob> tj;
-  DBuf<PJob> pj0, pj1, pj2;
+  DBuf<double> slot, slotF, slotH;         // [item][partner][3][p^2]
+  DBuf<double> tGrot, LGF, LGH, LGH2;      // [item][3][p^2]
+  DBuf<double> T, Tg, dev, devg, part;     // T: [item][partner][translation matrices]
+  DBuf<XJob> xj, xj2, dxj;
+  DBuf<TJob> tj, dtj;
   DBuf<RAJob> ra, ra2;
-  DBuf<int> ia, ib, ic, id, ie, tqmap;
+  DBuf<int> ia, ib, ic, tqmap;             // scratch lists of the self-value batches
+  DBuf<int> is_off, is_lst, ps_off, ps_lst, ident, esph, eitem, sbuf;
   DBuf<MJob> mj;
   DBuf<double> mpart;
-  DBuf<int> isoff, psoff;  // [entry][2] slices of the sum lists
+  DBuf<const double *> yptr;               // [sphere] harmonic rows of its exposed points
 };
 
 struct pbsam_system {
@@ -1169,10 +1172,18 @@ struct pbsam_system {
   std::vector<int> pidx;                                      // global sphere -> pol index (-1)
   std::vector<int> ioff;                                      // inter jobs of pol index k
   std::vector<int> mol_pol_begin;                             // pol index range per molecule
-  DBuf<int> d_iota, d_selx, d_selp, d_seli, d_selk;
+  DBuf<int> d_iota, d_selx, d_selp, d_seli, d_selk, d_gsbuf;
   double itot = 0.0;
   GradState gr;
   bool solved = false, grad_ok = false;
+  // independent systems solved together: groups of consecutive molecules that interact
+  // only among themselves (one group unless made by pbsam_batch_create)
+  int ngrp = 1;
+  std::vector<int> grp_mol;     // [ngrp + 1] molecule range of each group
+  std::vector<int> grp_pol;     // [ngrp + 1] polarized-sphere (pol index) range of each group
+  std::vector<int> grp_rc;      // status of each group's last solve
+  std::vector<int> grp_rounds;  // mutual rounds of each group's last solve
+  std::vector<int> grp_passes;  // gradient passes of each group's last gradient phase
 };
 
 static void system_static(pbsam_system *S) {
@@ -1276,24 +1287,13 @@ static void system_reset_state(pbsam_system *S) {
   S->d_dev.zero(s);
 }
 
-// generateInterXFormsForPolarize_LowMemory (molecule.cpp:1036-1126) and the job lists
-static int system_lists(pbsam_system *S) {
-  pbsam_ctx *c = S->c;
-  hipStream_t s = c->stream;
-  const int nmol = S->nmol, p = S->p, P2 = p * p;
-  system_reset_state(S);
-  S->solved = false;
-  S->grad_ok = false;
-  for (MolInst &M : S->mols) {
-    M.bInterXForm = false;
-    M.interPol.assign(M.mt->ns, {});
-    M.interact.assign(M.mt->ns, {});
-  }
-  S->lists_ok = false;
-  for (int j = 1; j < nmol; j++) {
+// sphere pairs of molecules m0..m1-1 within the cutoffs (generateInterXFormsForPolarize_
+// LowMemory, molecule.cpp:1036-1126); PBSAM_ERR_OVERLAP when two spheres overlap
+static int group_pairs(pbsam_system *S, int m0, int m1) {
+  for (int j = m0 + 1; j < m1; j++) {
     MolInst &Mj = S->mols[j];
     std::vector<int> neigh;
-    for (int i = 0; i < j; i++) {
+    for (int i = m0; i < j; i++) {
       MolInst &Mi = S->mols[i];
       double d[3] = {Mi.rcen[0] - Mj.rcen[0], Mi.rcen[1] - Mj.rcen[1], Mi.rcen[2] - Mj.rcen[2]};
       if (sqrt(dot3(d, d)) - Mi.maxR - Mj.maxR < S->interactRc) neigh.push_back(i);
@@ -1324,6 +1324,39 @@ static int system_lists(pbsam_system *S) {
           }
         }
       }
+    }
+  }
+  return PBSAM_OK;
+}
+
+// generateInterXFormsForPolarize_LowMemory (molecule.cpp:1036-1126) and the job lists
+static int system_lists(pbsam_system *S) {
+  pbsam_ctx *c = S->c;
+  hipStream_t s = c->stream;
+  const int nmol = S->nmol, p = S->p, P2 = p * p;
+  system_reset_state(S);
+  S->solved = false;
+  S->grad_ok = false;
+  for (MolInst &M : S->mols) {
+    M.bInterXForm = false;
+    M.interPol.assign(M.mt->ns, {});
+    M.interact.assign(M.mt->ns, {});
+  }
+  S->lists_ok = false;
+  S->grp_rc.assign(S->ngrp, PBSAM_OK);
+  S->grp_rounds.assign(S->ngrp, 0);
+  S->grp_passes.assign(S->ngrp, 0);
+  for (int gi = 0; gi < S->ngrp; gi++) {
+    const int m0 = S->grp_mol[gi], m1 = S->grp_mol[gi + 1];
+    if (group_pairs(S, m0, m1) == PBSAM_OK) continue;
+    // overlapping spheres: the group gets no pairs (computeForces returns false)
+    if (S->ngrp == 1) return PBSAM_ERR_OVERLAP;
+    S->grp_rc[gi] = PBSAM_ERR_OVERLAP;
+    for (int i = m0; i < m1; i++) {
+      MolInst &M = S->mols[i];
+      M.bInterXForm = false;
+      M.interPol.assign(M.mt->ns, {});
+      M.interact.assign(M.mt->ns, {});
     }
   }
   // polarized spheres and their jobs
@@ -1416,6 +1449,8 @@ static int system_lists(pbsam_system *S) {
     }
   }
   S->mol_pol_begin[nmol] = (int)S->pol.size();
+  S->grp_pol.assign(S->ngrp + 1, 0);
+  for (int gi = 0; gi <= S->ngrp; gi++) S->grp_pol[gi] = S->mol_pol_begin[S->grp_mol[gi]];
   S->ioff = ioff;
   S->itot = S->pol.empty() ? 0.0 : 1.0 / (double)S->pol.size();
   {
@@ -1560,135 +1595,199 @@ static double system_round(pbsam_system *S, bool round1, bool refresh) {
 static void system_phase0(pbsam_system *S);
 static double system_round(pbsam_system *S, bool round1, bool refresh);
 
-// mutual rounds >= 2 in the reference order (molecule.cpp:2042-2075): per molecule, spheres
-// whose last change exceeds 0.1 MAXDEV are recomputed in ascending order.
-static void system_gs_round(pbsam_system *S, const std::vector<double> &dev, double MAXDEV, bool refresh) {
+// Mutual rounds >= 2 in the reference order (molecule.cpp:2042-2075) for the groups in
+// `act`: per molecule, the spheres whose last change exceeds 0.1 of the group's MAXDEV are
+// recomputed in ascending order.  A group's chain is a sequence of steps (the molecule's
+// inter-molecular LS, then one sphere solve at a time, each followed by the intra-molecular
+// re-expansions that depend on it); step t of every group runs in the same launches.
+struct GSStep {
+  std::vector<int> seli, selk, solk, solg, selx, selp;
+};
+static void system_gs_round(pbsam_system *S, const std::vector<double> &dev, const std::vector<double> &maxdev,
+                            const std::vector<int> &act, bool refresh) {
   pbsam_ctx *c = S->c;
   hipStream_t st = c->stream;
   const int p = S->p;
-  for (int i = 0; i < S->nmol; i++) {
-    std::vector<int> Sk, Sg;
-    for (int k = S->mol_pol_begin[i]; k < S->mol_pol_begin[i + 1]; k++)
-      if (dev[S->pol[k]] > 0.1 * MAXDEV) {
-        Sk.push_back(k);
-        Sg.push_back(S->pol[k]);
+  std::vector<std::vector<GSStep>> chains(act.size());
+  size_t nsteps = 0;
+  for (size_t a = 0; a < act.size(); a++) {
+    const int gi = act[a];
+    std::vector<GSStep> &V = chains[a];
+    for (int i = S->grp_mol[gi]; i < S->grp_mol[gi + 1]; i++) {
+      std::vector<int> Sk, Sg;
+      for (int k = S->mol_pol_begin[i]; k < S->mol_pol_begin[i + 1]; k++)
+        if (dev[S->pol[k]] > 0.1 * maxdev[gi]) {
+          Sk.push_back(k);
+          Sg.push_back(S->pol[k]);
+        }
+      if (Sk.empty()) continue;
+      GSJobs J = gs_plan(Sg, S->nsph, [&](int g) -> const std::vector<pbsam_moltype::Entry> & { return S->ent[g]; },
+                         [&](int g) -> const std::vector<std::pair<int, int>> & { return S->bysrc[g]; });
+      for (size_t seg = 0; seg <= Sk.size(); seg++) {
+        GSStep t;
+        if (seg == 0) {
+          // LS from the partners' rotated H (orders from their current totals)
+          for (int k : Sk)
+            for (int q = S->ioff[k]; q < S->ioff[k + 1]; q++) t.seli.push_back(q);
+          t.selk = Sk;
+        } else {
+          t.solk.push_back(Sk[seg - 1]);
+          t.solg.push_back(Sg[seg - 1]);
+        }
+        t.selx.assign(J.selx.begin() + J.offx[seg], J.selx.begin() + J.offx[seg + 1]);
+        t.selp.assign(J.selp.begin() + J.offp[seg], J.selp.begin() + J.offp[seg + 1]);
+        V.push_back(std::move(t));
       }
-    if (Sk.empty()) continue;
-    // LS from the partners' rotated H (orders from their current totals)
-    std::vector<int> seli;
-    for (int k : Sk)
-      for (int j = S->ioff[k]; j < S->ioff[k + 1]; j++) seli.push_back(j);
-    if (seli.empty()) seli.push_back(0);
-    S->d_seli.upload(seli, st);
-    S->d_selk.upload(Sk, st);
-    XArgs xa{};
-    xa.jobs = S->d_xinter.p;
-    xa.H = S->d_Hrot.p;
-    xa.TQH = S->d_TQH.p;
-    xa.slotH = S->d_slotI.p;
-    xa.sel = S->d_seli.p;
-    xa.p = p;
-    int nsel = 0;
-    for (int k : Sk) nsel += S->ioff[k + 1] - S->ioff[k];
-    launch_trans(c, nsel, S->d_tinter.p, p, S->d_Tinter.p, S->d_seli.p, S->d_TQH.p);
-    launch_xform(c, nsel, xa);
-    LArgs la{S->d_pol.p, S->d_inter_off.p, S->d_inter_lst.p, S->d_slotI.p, nullptr, S->d_LSrot.p, S->d_selk.p, p};
-    launch_sumslots(c, (int)Sk.size(), la);
-    launch_rotapply(c, (int)Sk.size(), S->d_ra_ls.p, S->d_rj.p, S->d_Rst.p, S->d_LSrot.p, S->d_LS.p, p, S->d_selk.p);
-    // intra-molecular chain
-    GSJobs J = gs_plan(Sg, S->nsph, [&](int g) -> const std::vector<pbsam_moltype::Entry> & { return S->ent[g]; },
-                       [&](int g) -> const std::vector<std::pair<int, int>> & { return S->bysrc[g]; });
-    std::vector<int> sx = J.selx, sp = J.selp;
-    if (sx.empty()) sx.push_back(0);
-    if (sp.empty()) sp.push_back(0);
-    S->d_selx.upload(sx, st);
-    S->d_selp.upload(sp, st);
-    XArgs xb{};
-    xb.jobs = S->d_xintra.p;
-    xb.H = S->d_H.p;
-    xb.F = S->d_F.p;
-    xb.TQH = S->d_TQH.p;
-    xb.TQF = S->d_TQF.p;
-    xb.slotH = S->d_slotH.p;
-    xb.slotF = S->d_slotF.p;
-    xb.p = p;
-    PArgs pa{};
-    pa.jobs = S->d_pintra.p;
-    pa.xyz = S->d_pts.p;
-    pa.q1 = S->d_qF.p;
-    pa.q2 = S->d_qH.p;
-    pa.TQ1 = S->d_TQF.p;
-    pa.TQ2 = S->d_TQH.p;
-    pa.out1 = S->d_slotF.p;
-    pa.out2 = S->d_slotH.p;
-    pa.kap1 = 0;
-    pa.kap2 = 1;
-    pa.p = p;
-    SArgs sa{};
-    sa.lst = S->d_intra_lst.p;
-    sa.slotH = S->d_slotH.p;
-    sa.slotF = S->d_slotF.p;
-    sa.LS = S->d_LS.p;
-    sa.farH = S->d_LHfar.p;
-    sa.farF = S->d_LFfar.p;
-    sa.consts = S->d_consts.p;
-    sa.Dfix = S->d_Dfix.p;
-    sa.Vfix = S->d_Vfix.p;
-    sa.imat = S->d_imat.p;
-    sa.F = S->d_F.p;
-    sa.H = S->d_H.p;
-    sa.LFS = S->d_LFS.p;
-    sa.LHS = S->d_LHS.p;
-    sa.dev = S->d_dev.p;
-    sa.p = p;
-    QArgs qa{S->d_pol.p, S->d_pts.p, S->d_pt_off.p, S->d_pt_cnt.p, S->d_nspx.p, S->d_consts.p,
-             S->d_F.p, S->d_H.p, S->d_qF.p, S->d_qH.p, S->d_TQF.p, S->d_TQH.p, p};
-    for (size_t seg = 0; seg <= Sk.size(); seg++) {
-      if (seg > 0) {
-        const int k = Sk[seg - 1];
-        sa.sph = S->d_pol.p + k;
-        sa.lst_off = S->d_intra_off.p + k;
-        launch_solve(c, 1, sa);
-        qa.sph = S->d_pol.p + k;
-        launch_surface(c, 1, qa, S->max_nexp);
-        if (refresh)
-          launch_rotapply(c, 1, S->d_ra_h.p, S->d_rj.p, S->d_Rst.p, S->d_H.p, S->d_Hrot.p, p, S->d_iota.p + k);
-      }
-      xb.sel = S->d_selx.p + J.offx[seg];
-      launch_xform(c, J.offx[seg + 1] - J.offx[seg], xb);
-      pa.sel = S->d_selp.p + J.offp[seg];
-      launch_p2l(c, J.offp[seg + 1] - J.offp[seg], pa);
     }
+    nsteps = std::max(nsteps, V.size());
+  }
+  // all steps' lists in one buffer: per step six ranges
+  std::vector<int> buf;
+  std::vector<std::array<int, 12>> rng(nsteps);
+  for (size_t s0 = 0; s0 < nsteps; s0++) {
+    std::vector<int> GSStep::*field[6] = {&GSStep::seli, &GSStep::selk, &GSStep::solk,
+                                          &GSStep::solg, &GSStep::selx, &GSStep::selp};
+    for (int f = 0; f < 6; f++) {
+      rng[s0][2 * f] = (int)buf.size();
+      for (size_t a = 0; a < act.size(); a++)
+        if (s0 < chains[a].size()) {
+          const std::vector<int> &v = chains[a][s0].*field[f];
+          buf.insert(buf.end(), v.begin(), v.end());
+        }
+      rng[s0][2 * f + 1] = (int)buf.size() - rng[s0][2 * f];
+    }
+  }
+  if (buf.empty()) return;
+  if (buf.size() > S->d_gsbuf.n) {
+    HIPCHK(hipStreamSynchronize(st));
+    S->d_gsbuf.alloc(buf.size());
+  }
+  HIPCHK(hipMemcpyAsync(S->d_gsbuf.p, buf.data(), sizeof(int) * buf.size(), hipMemcpyHostToDevice, st));
+  const int *sb = S->d_gsbuf.p;
+  XArgs xa{};
+  xa.jobs = S->d_xinter.p;
+  xa.H = S->d_Hrot.p;
+  xa.TQH = S->d_TQH.p;
+  xa.slotH = S->d_slotI.p;
+  xa.p = p;
+  LArgs la{S->d_pol.p, S->d_inter_off.p, S->d_inter_lst.p, S->d_slotI.p, nullptr, S->d_LSrot.p, nullptr, p};
+  XArgs xb{};
+  xb.jobs = S->d_xintra.p;
+  xb.H = S->d_H.p;
+  xb.F = S->d_F.p;
+  xb.TQH = S->d_TQH.p;
+  xb.TQF = S->d_TQF.p;
+  xb.slotH = S->d_slotH.p;
+  xb.slotF = S->d_slotF.p;
+  xb.p = p;
+  PArgs pa{};
+  pa.jobs = S->d_pintra.p;
+  pa.xyz = S->d_pts.p;
+  pa.q1 = S->d_qF.p;
+  pa.q2 = S->d_qH.p;
+  pa.TQ1 = S->d_TQF.p;
+  pa.TQ2 = S->d_TQH.p;
+  pa.out1 = S->d_slotF.p;
+  pa.out2 = S->d_slotH.p;
+  pa.kap1 = 0;
+  pa.kap2 = 1;
+  pa.p = p;
+  SArgs sa{};
+  sa.sph = S->d_pol.p;
+  sa.lst_off = S->d_intra_off.p;
+  sa.lst = S->d_intra_lst.p;
+  sa.slotH = S->d_slotH.p;
+  sa.slotF = S->d_slotF.p;
+  sa.LS = S->d_LS.p;
+  sa.farH = S->d_LHfar.p;
+  sa.farF = S->d_LFfar.p;
+  sa.consts = S->d_consts.p;
+  sa.Dfix = S->d_Dfix.p;
+  sa.Vfix = S->d_Vfix.p;
+  sa.imat = S->d_imat.p;
+  sa.F = S->d_F.p;
+  sa.H = S->d_H.p;
+  sa.LFS = S->d_LFS.p;
+  sa.LHS = S->d_LHS.p;
+  sa.dev = S->d_dev.p;
+  sa.p = p;
+  QArgs qa{S->d_pol.p, S->d_pts.p, S->d_pt_off.p, S->d_pt_cnt.p, S->d_nspx.p, S->d_consts.p,
+           S->d_F.p, S->d_H.p, S->d_qF.p, S->d_qH.p, S->d_TQF.p, S->d_TQH.p, p};
+  for (size_t s0 = 0; s0 < nsteps; s0++) {
+    const std::array<int, 12> &r = rng[s0];
+    if (r[1]) {
+      launch_trans(c, r[1], S->d_tinter.p, p, S->d_Tinter.p, sb + r[0], S->d_TQH.p);
+      xa.sel = sb + r[0];
+      launch_xform(c, r[1], xa);
+    }
+    if (r[3]) {
+      la.sel = sb + r[2];
+      launch_sumslots(c, r[3], la);
+      launch_rotapply(c, r[3], S->d_ra_ls.p, S->d_rj.p, S->d_Rst.p, S->d_LSrot.p, S->d_LS.p, p, sb + r[2]);
+    }
+    if (r[5]) {
+      sa.sel = sb + r[4];
+      launch_solve(c, r[5], sa);
+      qa.sph = sb + r[6];
+      launch_surface(c, r[7], qa, S->max_nexp);
+      if (refresh)
+        launch_rotapply(c, r[5], S->d_ra_h.p, S->d_rj.p, S->d_Rst.p, S->d_H.p, S->d_Hrot.p, p, sb + r[4]);
+    }
+    xb.sel = sb + r[8];
+    launch_xform(c, r[9], xb);
+    pa.sel = sb + r[10];
+    launch_p2l(c, r[11], pa);
   }
 }
 
-// CMolecule::polarize_mutual rounds (molecule.cpp:2007-2075) in the reference order;
-// stop when the mean change <= tol (stock 1e-4) or after max_rounds (stock 1000).
+// CMolecule::polarize_mutual rounds (molecule.cpp:2007-2075) in the reference order, per
+// group: stop when the mean change <= tol (stock 1e-4) or after max_rounds (stock 1000).
+// Sets grp_rounds; groups that stop at max_rounds above tol get PBSAM_ERR_NOCONV.  Returns
+// the largest round count; *crit_out is the largest final mean change.
 static int system_mutual_ref(pbsam_system *S, double tol, int max_rounds, bool refresh, double *crit_out) {
   if (tol <= 0) tol = 1e-2 * 1e-2;
   if (max_rounds <= 0) max_rounds = 1000;
   system_phase0(S);
   system_round(S, true, refresh);
+  const int G = S->ngrp;
   std::vector<double> dev = S->d_dev.download(S->c->stream);
-  double total = 0.0, MAXDEV = 0.0;
-  for (int g : S->pol) {
-    total += dev[g];
-    MAXDEV = std::max(MAXDEV, dev[g]);
-  }
-  int ct = 1;
-  while (total * S->itot > tol && ct < max_rounds) {
-    system_gs_round(S, dev, MAXDEV, refresh);
-    dev = S->d_dev.download(S->c->stream);
-    total = 0.0;
-    MAXDEV = 0.0;
-    for (int g : S->pol) {
-      total += dev[g];
-      MAXDEV = std::max(MAXDEV, dev[g]);
+  std::vector<double> crit(G, 0.0), maxdev(G, 0.0);
+  auto stats = [&]() {
+    for (int gi = 0; gi < G; gi++) {
+      double total = 0.0, md = 0.0;
+      for (int k = S->grp_pol[gi]; k < S->grp_pol[gi + 1]; k++) {
+        total += dev[S->pol[k]];
+        md = std::max(md, dev[S->pol[k]]);
+      }
+      const int np = S->grp_pol[gi + 1] - S->grp_pol[gi];
+      crit[gi] = np ? total * (1.0 / (double)np) : 0.0;
+      maxdev[gi] = md;
     }
-    ct++;
+  };
+  stats();
+  std::vector<int> ct(G, 0);
+  for (int gi = 0; gi < G; gi++)
+    if (S->grp_pol[gi + 1] > S->grp_pol[gi]) ct[gi] = 1;
+  for (;;) {
+    std::vector<int> act;
+    for (int gi = 0; gi < G; gi++)
+      if (ct[gi] > 0 && crit[gi] > tol && ct[gi] < max_rounds) act.push_back(gi);
+    if (act.empty()) break;
+    system_gs_round(S, dev, maxdev, act, refresh);
+    dev = S->d_dev.download(S->c->stream);
+    stats();
+    for (int gi : act) ct[gi]++;
   }
-  if (crit_out) *crit_out = total * S->itot;
-  return ct;
+  int rmax = 0;
+  double cmax = 0.0;
+  for (int gi = 0; gi < G; gi++) {
+    S->grp_rounds[gi] = ct[gi];
+    if (ct[gi] > 0 && crit[gi] > tol && S->grp_rc[gi] == PBSAM_OK) S->grp_rc[gi] = PBSAM_ERR_NOCONV;
+    rmax = std::max(rmax, ct[gi]);
+    cmax = std::max(cmax, crit[gi]);
+  }
+  if (crit_out) *crit_out = cmax;
+  return rmax;
 }
 
 // phase 0 of polarize_mutual: molecule rotations and HselfRot
@@ -1945,18 +2044,32 @@ int pbsam_moltype_get_intra(const pbsam_moltype *m, int k, double *LF, double *L
   });
 }
 
-int pbsam_system_create(pbsam_ctx *c, int n_mol, pbsam_moltype *const *mts, const double *pos, const double *quat,
-                        pbsam_system **out) {
-  if (!c || n_mol < 1 || !mts || !pos || !quat || !out) return PBSAM_ERR_BADARG;
+static int system_make(pbsam_ctx *c, int n_sys, const int *n_mol, pbsam_moltype *const *mts, const double *pos,
+                       const double *quat, pbsam_system **out) {
+  if (!c || n_sys < 1 || !n_mol || !mts || !pos || !quat || !out) return PBSAM_ERR_BADARG;
   *out = nullptr;
+  long long tot = 0;
+  for (int g = 0; g < n_sys; g++) {
+    if (n_mol[g] < 1) return PBSAM_ERR_BADARG;
+    tot += n_mol[g];
+  }
+  if (tot > (1 << 24)) return PBSAM_ERR_BADARG;
   std::unique_ptr<pbsam_system> S(new pbsam_system());
   S->c = c;
-  S->nmol = n_mol;
-  S->p = mts[0]->p;
+  S->nmol = (int)tot;
+  S->ngrp = n_sys;
+  S->grp_mol.assign(n_sys + 1, 0);
+  for (int g = 0; g < n_sys; g++) S->grp_mol[g + 1] = S->grp_mol[g] + n_mol[g];
+  S->grp_pol.assign(n_sys + 1, 0);
+  S->grp_rc.assign(n_sys, PBSAM_OK);
+  S->grp_rounds.assign(n_sys, 0);
+  S->grp_passes.assign(n_sys, 0);
+  S->p = mts[0] ? mts[0]->p : 0;
+  const int nm = S->nmol;
   int rc = guarded(&c->err, [&]() {
     HIPCHK(hipSetDevice(c->device));
-    S->mols.resize(n_mol);
-    for (int i = 0; i < n_mol; i++) {
+    S->mols.resize(nm);
+    for (int i = 0; i < nm; i++) {
       if (!mts[i] || mts[i]->c != c) fail(PBSAM_ERR_BADARG, "molecule type from another context");
       if (mts[i]->p != S->p) fail(PBSAM_ERR_BADARG, "all molecules of a system must share the pole order");
       if (!mts[i]->have_self) fail(PBSAM_ERR_STATE, "molecule type has no self-polarized expansions");
@@ -1976,6 +2089,27 @@ int pbsam_system_create(pbsam_ctx *c, int n_mol, pbsam_moltype *const *mts, cons
   });
   if (rc == PBSAM_OK) *out = S.release();
   return rc;
+}
+
+int pbsam_system_create(pbsam_ctx *c, int n_mol, pbsam_moltype *const *mts, const double *pos, const double *quat,
+                        pbsam_system **out) {
+  if (n_mol < 1) return PBSAM_ERR_BADARG;
+  return system_make(c, 1, &n_mol, mts, pos, quat, out);
+}
+
+int pbsam_batch_create(pbsam_ctx *c, int n_sys, const int *n_mol, pbsam_moltype *const *mts, const double *pos,
+                       const double *quat, pbsam_system **out) {
+  return system_make(c, n_sys, n_mol, mts, pos, quat, out);
+}
+
+int pbsam_batch_status(const pbsam_system *S, int *status, int *mutual_rounds, int *grad_passes) {
+  if (!S) return PBSAM_ERR_BADARG;
+  for (int g = 0; g < S->ngrp; g++) {
+    if (status) status[g] = S->grp_rc.empty() ? PBSAM_OK : S->grp_rc[g];
+    if (mutual_rounds) mutual_rounds[g] = S->grp_rounds.empty() ? 0 : S->grp_rounds[g];
+    if (grad_passes) grad_passes[g] = S->grp_passes.empty() ? 0 : S->grp_passes[g];
+  }
+  return PBSAM_OK;
 }
 
 void pbsam_system_destroy(pbsam_system *S) { delete S; }
@@ -2010,21 +2144,30 @@ int pbsam_mutual_rounds(pbsam_system *S, int rounds, unsigned flags, double *max
   });
 }
 
+// the first group's non-OK status (PBSAM_OK when every group is OK)
+static int groups_rc(const pbsam_system *S) {
+  for (int g = 0; g < S->ngrp; g++)
+    if (S->grp_rc[g] != PBSAM_OK) return S->grp_rc[g];
+  return PBSAM_OK;
+}
+
 int pbsam_solve_mutual(pbsam_system *S, double tol, int max_rounds, unsigned flags, int *rounds_out) {
   if (!S) return PBSAM_ERR_BADARG;
   return guarded(&S->c->err, [&]() {
+    HIPCHK(hipSetDevice(S->c->device));
     int rc = system_lists(S);
     if (rc != PBSAM_OK) return rc;
-    if (S->nmol == 1 || S->pol.empty()) {
+    if (S->pol.empty()) {
+      S->solved = true;
       if (rounds_out) *rounds_out = 0;
-      return PBSAM_OK;
+      return groups_rc(S);
     }
     bool refresh = (flags & PBSAM_HROT_REFRESH) != 0;
     double crit = 0.0;
     int r = system_mutual_ref(S, tol, max_rounds, refresh, &crit);
     S->solved = true;
     if (rounds_out) *rounds_out = r;
-    return crit > (tol > 0 ? tol : 1e-4) ? PBSAM_ERR_NOCONV : PBSAM_OK;
+    return groups_rc(S);
   });
 }
 
@@ -2115,7 +2258,9 @@ int pbsam_compute_forces(pbsam_system *S, double tol, int max_rounds, double *fo
   int r = 0;
   int rc = pbsam_solve_mutual(S, tol, max_rounds, 0, &r);
   if (rounds_out) *rounds_out = r;
-  if (rc != PBSAM_OK && rc != PBSAM_ERR_NOCONV) return rc;
+  // a batch goes on past groups with overlapping spheres (they get no forces)
+  const bool go = rc == PBSAM_OK || rc == PBSAM_ERR_NOCONV || (S->ngrp > 1 && S->solved);
+  if (!go) return rc;
   int rc2 = pbsam_gradient(S, tol, max_rounds, nullptr);
   if (rc2 != PBSAM_OK) return rc2;
   rc2 = pbsam_forces(S, force, torque);

@@ -795,6 +795,7 @@ struct SArgs {
   int *sweeps;             // [sphere] out (may be null)
   int p;
   unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
+  const int *sel;          // entries of sph/lst_off to run (null: the workgroup index)
 };
 
 __device__ __forceinline__ double block_sum(double v, double *red) {
@@ -1067,9 +1068,10 @@ __device__ void solve_identity(const SArgs &A, int g, int lst0, int lst1) {
 __global__ void __launch_bounds__(ST) k_solve(SArgs A) {
   extern __shared__ double sh[];
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
-  const int g = A.sph[blockIdx.x];
+  const int e0 = A.sel ? A.sel[blockIdx.x] : blockIdx.x;
+  const int g = A.sph[e0];
   if (!A.imat[g]) {  // uniform across the workgroup
-    const int l0 = A.lst_off[blockIdx.x], l1 = A.lst_off[blockIdx.x + 1];
+    const int l0 = A.lst_off[e0], l1 = A.lst_off[e0 + 1];
     // KW pairs per lane: ST * KW >= p(p+1)/2
     if (p <= 22) solve_identity<1>(A, g, l0, l1);
     else if (p <= 31) solve_identity<2>(A, g, l0, l1);
@@ -1080,7 +1082,7 @@ __global__ void __launch_bounds__(ST) k_solve(SArgs A) {
   const double *cs = A.consts + (size_t)g * 10 * p;
   const double *cH1 = cs, *cF1 = cs + p, *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cH2 = cs + 4 * p,
                *cF2 = cs + 5 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p, *cInvH = cs + 8 * p;
-  const int l0 = A.lst_off[blockIdx.x], l1 = A.lst_off[blockIdx.x + 1];
+  const int l0 = A.lst_off[e0], l1 = A.lst_off[e0 + 1];
   const double *IM = A.imat[g];
   constexpr int KPT = 4;  // up to 1024 coefficients per sphere
   double fb[KPT], hb[KPT], oldH[KPT], outerH[KPT];
@@ -1348,16 +1350,18 @@ struct LArgs {
   const int *sel;        // entries of sph/lst_off to run (null: workgroup index)
   int p;
   int bs;                // doubles per slot and per output (0: p^2)
+  const int *obs;        // output block of each entry (null: its sph index)
 };
 // grid (entries, element chunks of 256)
 __global__ void __launch_bounds__(256) k_sumslots(LArgs A) {
   const int e0 = A.sel ? A.sel[blockIdx.x] : blockIdx.x;
   const int p = A.p, P2 = A.bs ? A.bs : p * p, g = A.sph[e0];
   const int l0 = A.lst_off[e0], l1 = A.lst_off[e0 + 1];
+  const int go = A.obs ? A.obs[e0] : g;
   for (int i = blockIdx.y * 256 + threadIdx.x; i < P2; i += 256 * gridDim.y) {
     double s = A.base ? A.base[(size_t)g * P2 + i] : 0.0;
     for (int e = l0; e < l1; e++) s += A.slot[(size_t)A.lst[e] * P2 + i];
-    A.out[(size_t)g * P2 + i] = s;
+    A.out[(size_t)go * P2 + i] = s;
   }
 }
 

@@ -6,7 +6,8 @@
 //                    (CRotCoeff::computeGradCoeff rotcoeff.cpp:818-850, dRotateT/P(+Sing)
 //                     :450-771, CTransCoeff dT :372-395) -- fused per pair
 //   k_solve3         CSolExpCenter::solveFH with 3 right-hand sides   expcenter.cpp:1080-1370
-//   k_surfgrad       computeExposedSurfaceGradientFH                  expcenter.cpp:1378-1412
+//   k_surfgemv       computeExposedSurfaceGradientFH (stored harmonics) expcenter.cpp:1378-1412
+//   k_p2lgemv        CXFormNIntra::xformGFH (stored maps)             xforms.cpp:514-582
 //   k_force          computeForceOn_0 + the per-sphere force terms    expcenter.cpp:124-149,
 //                                                                     moldynamics.cpp:525-550
 namespace pbsam {
@@ -460,18 +461,20 @@ struct S3Args {
   double *dev;             // [entry]
   int p;
   unsigned long long *work;
+  const int *sel;          // entries of sph/ent to run (null: the workgroup index)
+  const int *lfi;          // right-hand-side block of each entry (null: the workgroup index)
 };
 
 // three right-hand sides: WPC wavefronts per column, column d = wave / WPC
 constexpr int WPC = 4;
 constexpr int S3T = 3 * WPC * 64;
 template <int KW>
-__device__ void solve3_identity(const S3Args &A, int g, int e, double *red) {
+__device__ void solve3_identity(const S3Args &A, int g, int e, int lb, double *red) {
   constexpr int NW = 3 * WPC;
   const int p = A.p, P2 = p * p, lane = threadIdx.x % 64, w = threadIdx.x / 64, d = w / WPC, sub = w % WPC;
   const double *cs = A.consts + (size_t)g * 10 * p;
   const double *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p;
-  const size_t o = ((size_t)e * 3 + d) * P2, ol = ((size_t)blockIdx.x * 3 + d) * P2;
+  const size_t o = ((size_t)e * 3 + d) * P2, ol = ((size_t)lb * 3 + d) * P2;
   const int npair = p * (p + 1) / 2;
   IdState<KW> S;
   for (int k = 0; k < KW; k++) {
@@ -533,13 +536,14 @@ __device__ void solve3_identity(const S3Args &A, int g, int e, double *red) {
 __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
   extern __shared__ double sh[];
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
-  const int g = A.sph[blockIdx.x], e = A.ent[blockIdx.x];
+  const int e0 = A.sel ? A.sel[blockIdx.x] : blockIdx.x;
+  const int g = A.sph[e0], e = A.ent[e0], lb = A.lfi ? A.lfi[e0] : blockIdx.x;
   const double *IM = A.imat[g];
   if (!IM) {  // uniform across the workgroup
     __shared__ double red[2 * IC * 3 * WPC + 2 * 3 * WPC];
-    if (p <= 22) solve3_identity<1>(A, g, e, red);
-    else if (p <= 31) solve3_identity<2>(A, g, e, red);
-    else solve3_identity<3>(A, g, e, red);
+    if (p <= 22) solve3_identity<1>(A, g, e, lb, red);
+    else if (p <= 31) solve3_identity<2>(A, g, e, lb, red);
+    else solve3_identity<3>(A, g, e, lb, red);
     return;
   }
   // general matrix: F, H, X of the three columns in LDS
@@ -547,7 +551,7 @@ __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
   const double *cs = A.consts + (size_t)g * 10 * p;
   const double *cH1 = cs, *cF1 = cs + p, *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cH2 = cs + 4 * p,
                *cF2 = cs + 5 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p, *cInvH = cs + 8 * p;
-  const size_t o = (size_t)e * 3 * P2, ol = (size_t)blockIdx.x * 3 * P2;
+  const size_t o = (size_t)e * 3 * P2, ol = (size_t)lb * 3 * P2;
   constexpr int KPT = (3 * PMAX * PMAX + S3T - 1) / S3T;
   double fbv[KPT], hbv[KPT], outerH[KPT];
   int nn[KPT];
@@ -644,97 +648,44 @@ __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
   }
 }
 
-// ----------------------------------------------------------------------------
-// exposed-surface gradient charges (computeExposedSurfaceGradientFH expcenter.cpp:1378-1412):
-// per point the three F and three H gradient charges, stored per direction
-// ([3][points]), the squared norms of the F and H 3-vectors (the point-skip test of the
-// gradient re-expansion, triexpan.cpp:127-153), and per direction the totals of |charge|
-// (partials per workgroup; k_surfgrad_totals takes the max over directions).
-// ----------------------------------------------------------------------------
-struct GQArgs {
-  int sph;                 // global sphere
-  int ent;                 // entry of gF/gH
-  const double *xyz;
-  const int *pt_off, *pt_cnt, *nspx;
-  const double *consts;
-  const double *gF, *gH;   // [entry][3][p^2]
-  double *gSF, *gSH;       // [3][all points]
-  double *nF, *nH;         // [all points]
-  long long npts_all;
-  double *part;            // [chunk][6]
-  int p;
-  unsigned long long *work;
-};
-__global__ void __launch_bounds__(ST) k_surfgrad(Tabs tb, GQArgs A) {
-  extern __shared__ double sh[];
-  const int p = A.p, P2 = p * p, tid = threadIdx.x;
-  const int g = A.sph;
-  double *B = sh, *red = B + 6 * P2;   // gF[0..2], gH[0..2]
-  const int off = A.pt_off[g], cnt = A.pt_cnt[g];
-  const int ch = blockIdx.x;
-  for (int i = tid; i < 6 * P2; i += ST)
-    B[i] = (i < 3 * P2) ? A.gF[(size_t)A.ent * 3 * P2 + i] : A.gH[(size_t)A.ent * 3 * P2 + i - 3 * P2];
-  __syncthreads();
-  const double *cChgH = A.consts + (size_t)g * 10 * p + 9 * p;
-  const double dA = 4 * M_PI / double(A.nspx[g]);
-  double t[6] = {0, 0, 0, 0, 0, 0};
-  const int h = ch * ST + tid;
-  if (h < cnt) {
-    const double *x = A.xyz + 3 * (size_t)(off + h);
-    double rho, th, ph;
-    cart_to_sph(x[0], x[1], x[2], rho, th, ph);
-    const double cost = cos(th), sint = sin(th);
-    double sv[6] = {0, 0, 0, 0, 0, 0};
-    for (int m = 0; m < p; m++) {
-      sh_column(tb, p, m, cost, sint, ph, [&](int n, double re, double im) {
-        const double c2 = (2 * n + 1) / (4.0 * M_PI), chH = cChgH[n];
-        for (int b = 0; b < 6; b++) {
-          const double cc = (b < 3) ? c2 : chH;
-          const double *E = B + b * P2;
-          if (m == 0) sv[b] += E[n * n] * (re * cc);
-          else {
-            const int i = n * n + 2 * m - 1;
-            sv[b] += 2.0 * (E[i] * (re * cc) + E[i + 1] * (im * cc));
-          }
-        }
-      });
-    }
-    double nf = 0.0, nh = 0.0;
-    for (int d = 0; d < 3; d++) {
-      const double gf = sv[d] * dA, gh = sv[3 + d] * dA;
-      A.gSF[(size_t)d * A.npts_all + off + h] = gf;
-      A.gSH[(size_t)d * A.npts_all + off + h] = gh;
-      t[d] = fabs(gf);
-      t[3 + d] = fabs(gh);
-      nf += gf * gf;
-      nh += gh * gh;
-    }
-    A.nF[off + h] = nf;
-    A.nH[off + h] = nh;
-  }
-  for (int b = 0; b < 6; b++) {
-    const double v = block_sum(t[b], red);
-    if (tid == 0) A.part[6 * ch + b] = v;
-  }
-  if (A.work && tid == 0 && ch == 0) {
-    atomicAdd(&A.work[0], (unsigned long long)((7.0 * p * (p + 1) / 2 + 18.0 * p * p) * cnt));
-    atomicAdd(&A.work[1], (unsigned long long)(64.0 * cnt + 48.0 * P2));
-  }
-}
-// one wavefront: lane l sums chunks l, l+64, ..., then the lanes are added
-__global__ void __launch_bounds__(64) k_surfgrad_totals(int nchunk, const double *part, int g, double *maxgF,
-                                                        double *maxgH) {
+// Totals of the exposed-surface gradient charges of each listed entry (the max over
+// directions of sum |charge|, expcenter.cpp:1378-1412) from k_surfgemv's per-workgroup
+// partials: one wavefront per entry; lane l sums chunks l, l+64, ..., then the lanes are added.
+__global__ void __launch_bounds__(64) k_surfgrad_totals(const int *sel, const int *sph, const int *pbuf, int nchunk,
+                                                        const double *part, double *maxgF, double *maxgH) {
+  const int e = sel[blockIdx.x];
+  const double *pp = part + (size_t)pbuf[e] * nchunk * 6;
   double t[6] = {0, 0, 0, 0, 0, 0};
   for (int c = threadIdx.x; c < nchunk; c += 64)
-    for (int b = 0; b < 6; b++) t[b] += part[6 * c + b];
+    for (int b = 0; b < 6; b++) t[b] += pp[6 * c + b];
   for (int b = 0; b < 6; b++) t[b] = wave_sum(t[b]);
   if (threadIdx.x != 0) return;
   double mf = (t[0] > t[1] ? t[0] : t[1]);
   if (t[2] > mf) mf = t[2];
   double mh = (t[3] > t[4] ? t[3] : t[4]);
   if (t[5] > mh) mh = t[5];
-  maxgF[g] = mf;
-  maxgH[g] = mh;
+  maxgF[sph[e]] = mf;
+  maxgH[sph[e]] = mh;
+}
+
+// dst[i] = src[sel[i]]
+__global__ void __launch_bounds__(256) k_gather(const int *sel, int n, const double *src, double *dst) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[sel[i]];
+}
+// blocks of bs doubles: dst[didx[e]] = src[sidx[e]] for the listed entries e; grid (entry, chunk)
+__global__ void __launch_bounds__(256) k_copyblk(const int *sel, const int *sidx, const int *didx, int bs,
+                                                 const double *src, double *dst) {
+  const int e = sel[blockIdx.x];
+  const double *s = src + (size_t)sidx[e] * bs;
+  double *d = dst + (size_t)didx[e] * bs;
+  for (int i = blockIdx.y * 256 + threadIdx.x; i < bs; i += 256 * gridDim.y) d[i] = s[i];
+}
+// blocks of bs doubles: out[idx[e]] = a[idx[e]] + b[idx[e]] for the listed entries e
+__global__ void __launch_bounds__(256) k_addblk(const int *sel, const int *idx, int bs, const double *a,
+                                                const double *b, double *out) {
+  const size_t o = (size_t)idx[sel[blockIdx.x]] * bs;
+  for (int i = blockIdx.y * 256 + threadIdx.x; i < bs; i += 256 * gridDim.y) out[o + i] = a[o + i] + b[o + i];
 }
 
 // out = a + b over n doubles
@@ -861,7 +812,7 @@ __global__ void __launch_bounds__(256) k_p2lmat(Tabs tb, const MBuild *jobs, con
 // points whose charge 3-vector has squared norm < 1e-15 are left out (triexpan.cpp:127-153).
 // Grid (job, point split); partial sums go to part and k_p2lsum adds them in split order.
 struct MJob {
-  long long moff;
+  const double *M;    // F matrix (npts x p^2, point-major), followed by the H matrix
   int npts, pt_off;   // points of the source (global index into the charge planes)
   int slot;           // output block of r = 0 (blocks slot .. slot+R-1)
   int src;            // source sphere (totals)
@@ -869,87 +820,114 @@ struct MJob {
 };
 struct MArgs {
   const MJob *jobs;
-  const double *mats;
+  const int *sel;            // job of each grid row (null: the row index)
   const double *qF, *qH;     // charges, R planes of `plane` doubles
   long long plane;
   const double *nF, *nH;     // squared norms for the skip test (null: none)
   const double *TQF, *TQH;
-  int nsplit, p;
-  double *part;              // [job][split][2][R][p^2]
+  int nsplit, spl, p;        // splits per job, points per split (a multiple of MPTS)
+  double *part;              // [row][split][2][R][p^2]
   unsigned long long *work;
 };
 constexpr int MT = 256;
-constexpr int MPTS = 32;  // points per workgroup (one split)
+constexpr int MPTS = 32;  // points staged per tile
+constexpr int MKC = (PMAX * PMAX + MT - 1) / MT;  // coefficients per thread
 template <int R>
 __global__ void __launch_bounds__(MT) k_p2lgemv(MArgs A) {
   __shared__ double qs[MPTS][2 * R];
-  const MJob J = A.jobs[blockIdx.x];
+  const MJob J = A.jobs[A.sel ? A.sel[blockIdx.x] : blockIdx.x];
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
   const int pF = compute_order(J.rho, A.TQF[J.src], J.a_src, p);
   const int pH = compute_order(J.rho, A.TQH[J.src], J.a_src, p);
   const int nF = pF * pF, nH = pH * pH;
-  const int h0 = blockIdx.y * MPTS, h1 = min(J.npts, h0 + MPTS);
+  const int hb = blockIdx.y * A.spl, he = min(J.npts, hb + A.spl);
   double *out = A.part + (((size_t)blockIdx.x * A.nsplit + blockIdx.y) * 2) * R * P2;
-  // stage this split's charges (zero where the point is skipped)
-  for (int e = tid; e < MPTS * 2 * R; e += MT) {
-    const int t = e / (2 * R), v = e % (2 * R), h = h0 + t;
-    double q = 0.0;
-    if (h < h1) {
-      const size_t pt = (size_t)J.pt_off + h;
-      if (v < R) q = (A.nF && A.nF[pt] < 1e-15) ? 0.0 : A.qF[v * A.plane + pt];
-      else q = (A.nH && A.nH[pt] < 1e-15) ? 0.0 : A.qH[(v - R) * A.plane + pt];
+  const double *MF = J.M, *MH = MF + (size_t)J.npts * P2;
+  double aF[MKC][R], aH[MKC][R];
+#pragma unroll
+  for (int k = 0; k < MKC; k++)
+#pragma unroll
+    for (int r = 0; r < R; r++) aF[k][r] = aH[k][r] = 0.0;
+  for (int h0 = hb; h0 < he; h0 += MPTS) {  // uniform across the workgroup
+    const int h1 = min(he, h0 + MPTS);
+    __syncthreads();
+    // stage this tile's charges (zero where the point is skipped)
+    for (int e = tid; e < MPTS * 2 * R; e += MT) {
+      const int t = e / (2 * R), v = e % (2 * R), h = h0 + t;
+      double q = 0.0;
+      if (h < h1) {
+        const size_t pt = (size_t)J.pt_off + h;
+        if (v < R) q = (A.nF && A.nF[pt] < 1e-15) ? 0.0 : A.qF[v * A.plane + pt];
+        else q = (A.nH && A.nH[pt] < 1e-15) ? 0.0 : A.qH[(v - R) * A.plane + pt];
+      }
+      qs[t][v] = q;
     }
-    qs[t][v] = q;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < MKC; k++) {
+      const int c = tid + k * MT;
+      double tF[R], tH[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) tF[r] = tH[r] = 0.0;
+      if (c < nF) {
+        const double *col = MF + (size_t)h0 * P2 + c;
+        int h = h0;
+        for (; h + 4 <= h1; h += 4) {  // four independent loads in flight
+          const double m0 = col[0], m1 = col[P2], m2 = col[2 * P2], m3 = col[3 * P2];
+          const int t = h - h0;
+#pragma unroll
+          for (int r = 0; r < R; r++)
+            tF[r] += qs[t][r] * m0 + qs[t + 1][r] * m1 + qs[t + 2][r] * m2 + qs[t + 3][r] * m3;
+          col += 4 * (size_t)P2;
+        }
+        for (; h < h1; h++, col += P2)
+#pragma unroll
+          for (int r = 0; r < R; r++) tF[r] += qs[h - h0][r] * col[0];
+      }
+      if (c < nH) {
+        const double *col = MH + (size_t)h0 * P2 + c;
+        int h = h0;
+        for (; h + 4 <= h1; h += 4) {
+          const double m0 = col[0], m1 = col[P2], m2 = col[2 * P2], m3 = col[3 * P2];
+          const int t = h - h0;
+#pragma unroll
+          for (int r = 0; r < R; r++)
+            tH[r] += qs[t][R + r] * m0 + qs[t + 1][R + r] * m1 + qs[t + 2][R + r] * m2 + qs[t + 3][R + r] * m3;
+          col += 4 * (size_t)P2;
+        }
+        for (; h < h1; h++, col += P2)
+#pragma unroll
+          for (int r = 0; r < R; r++) tH[r] += qs[h - h0][R + r] * col[0];
+      }
+      // tiles are added in point order (a split of one tile keeps the tile's sum)
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        aF[k][r] = (h0 == hb) ? tF[r] : aF[k][r] + tF[r];
+        aH[k][r] = (h0 == hb) ? tH[r] : aH[k][r] + tH[r];
+      }
+    }
   }
-  __syncthreads();
-  const double *MF = A.mats + J.moff, *MH = MF + (size_t)J.npts * P2;
-  constexpr int KC = (PMAX * PMAX + MT - 1) / MT;  // coefficients per thread
-  for (int k = 0; k < KC; k++) {
+#pragma unroll
+  for (int k = 0; k < MKC; k++) {
     const int c = tid + k * MT;
-    if (c >= P2) break;
-    double aF[R], aH[R];
-    for (int r = 0; r < R; r++) aF[r] = aH[r] = 0.0;
-    if (c < nF) {
-      const double *col = MF + (size_t)h0 * P2 + c;
-      int h = h0;
-      for (; h + 4 <= h1; h += 4) {  // four independent loads in flight
-        const double m0 = col[0], m1 = col[P2], m2 = col[2 * P2], m3 = col[3 * P2];
-        const int t = h - h0;
-        for (int r = 0; r < R; r++)
-          aF[r] += qs[t][r] * m0 + qs[t + 1][r] * m1 + qs[t + 2][r] * m2 + qs[t + 3][r] * m3;
-        col += 4 * (size_t)P2;
+    if (c < P2)
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        out[(size_t)r * P2 + c] = aF[k][r];
+        out[(size_t)(R + r) * P2 + c] = aH[k][r];
       }
-      for (; h < h1; h++, col += P2)
-        for (int r = 0; r < R; r++) aF[r] += qs[h - h0][r] * col[0];
-    }
-    if (c < nH) {
-      const double *col = MH + (size_t)h0 * P2 + c;
-      int h = h0;
-      for (; h + 4 <= h1; h += 4) {
-        const double m0 = col[0], m1 = col[P2], m2 = col[2 * P2], m3 = col[3 * P2];
-        const int t = h - h0;
-        for (int r = 0; r < R; r++)
-          aH[r] += qs[t][R + r] * m0 + qs[t + 1][R + r] * m1 + qs[t + 2][R + r] * m2 + qs[t + 3][R + r] * m3;
-        col += 4 * (size_t)P2;
-      }
-      for (; h < h1; h++, col += P2)
-        for (int r = 0; r < R; r++) aH[r] += qs[h - h0][R + r] * col[0];
-    }
-    for (int r = 0; r < R; r++) {
-      out[(size_t)r * P2 + c] = aF[r];
-      out[(size_t)(R + r) * P2 + c] = aH[r];
-    }
   }
   if (A.work && tid == 0) {
-    atomicAdd(&A.work[0], (unsigned long long)(2.0 * R * (nF + nH) * (h1 - h0 > 0 ? h1 - h0 : 0)));
-    atomicAdd(&A.work[1], (unsigned long long)(8.0 * (nF + nH) * (h1 - h0 > 0 ? h1 - h0 : 0)));
+    const int np = he - hb > 0 ? he - hb : 0;
+    atomicAdd(&A.work[0], (unsigned long long)(2.0 * R * (nF + nH) * np));
+    atomicAdd(&A.work[1], (unsigned long long)(8.0 * (nF + nH) * np));
   }
 }
 // slotF/slotH[J.slot + r][c] = sum over splits of the partials, in split order
-// grid (job, output chunks of 256)
-__global__ void __launch_bounds__(256) k_p2lsum(const MJob *jobs, int nsplit, int R, int p, const double *part,
-                                               double *slotF, double *slotH) {
-  const MJob J = jobs[blockIdx.x];
+// grid (job row, output chunks of 256)
+__global__ void __launch_bounds__(256) k_p2lsum(const MJob *jobs, const int *sel, int nsplit, int R, int p,
+                                               const double *part, double *slotF, double *slotH) {
+  const MJob J = jobs[sel ? sel[blockIdx.x] : blockIdx.x];
   const int P2 = p * p;
   for (int e = blockIdx.y * 256 + threadIdx.x; e < 2 * R * P2; e += 256 * gridDim.y) {
     double s = 0.0;
@@ -994,40 +972,52 @@ __global__ void __launch_bounds__(256) k_surfmat(Tabs tb, const double *xyz, lon
 
 constexpr int SGR = 16;  // points (rows) per workgroup
 struct SGArgs {
-  int ent;                   // list entry of gF/gH
-  int sph;                   // global sphere (totals, points, constants)
-  const double *Y;           // rows of this sphere's points
-  int npts;                  // exposed points of the sphere
-  long long pt_glob;         // global index of its first point
+  const int *sel;            // entries to run, one per grid row y
+  const int *sph;            // [entry] global sphere
+  const int *pbuf;           // [entry] block of `part`
+  const double *const *Y;    // [global sphere] harmonic rows of its exposed points
+  const int *pt_off, *pt_cnt, *nspx;  // [global sphere] first point, exposed and spiral counts
+  const double *consts;      // [sphere][10][p] (cChgH at row 9)
   const double *gF, *gH;     // [entry][3][p^2]
-  const double *cChgH;       // [p] of the sphere
-  double dA;
   double *gSF, *gSH;         // [3][all points]
   double *nF, *nH;
   long long npts_all;
-  double *part;              // [workgroup][6]
+  double *part;              // [pbuf][chunk][6]
+  int nchunk;                // chunks per entry (grid x)
   int p;
   unsigned long long *work;
 };
+// grid (chunk of SGR points, entry)
 __global__ void __launch_bounds__(256) k_surfgemv(SGArgs A) {
   __shared__ double E[6][PMAX * PMAX];
   __shared__ double red[8];
   const int p = A.p, P2 = p * p, tid = threadIdx.x, w = tid / 64, lane = tid % 64;
+  const int ent = A.sel[blockIdx.y], g = A.sph[ent];
+  const int npts = A.pt_cnt[g];
+  double *part = A.part + ((size_t)A.pbuf[ent] * A.nchunk + blockIdx.x) * 6;
+  if ((int)blockIdx.x * SGR >= npts) {  // uniform: a chunk past the sphere's points
+    if (tid < 6) part[tid] = 0.0;
+    return;
+  }
+  const double *cChgH = A.consts + (size_t)g * 10 * p + 9 * p;
+  const double dA = 4 * M_PI / double(A.nspx[g]);
+  const long long pt_glob = A.pt_off[g];
+  const double *Yg = A.Y[g];
   for (int e = tid; e < 6 * P2; e += 256) {
     const int b = e / P2, c = e % P2;
     int n = (int)sqrt((double)c);
     while (n * n > c) n--;
     while ((n + 1) * (n + 1) <= c) n++;
-    const double C = (b < 3) ? (2 * n + 1) / (4.0 * M_PI) : A.cChgH[n];
-    const double v = (b < 3) ? A.gF[((size_t)A.ent * 3 + b) * P2 + c] : A.gH[((size_t)A.ent * 3 + b - 3) * P2 + c];
+    const double C = (b < 3) ? (2 * n + 1) / (4.0 * M_PI) : cChgH[n];
+    const double v = (b < 3) ? A.gF[((size_t)ent * 3 + b) * P2 + c] : A.gH[((size_t)ent * 3 + b - 3) * P2 + c];
     E[b][c] = (c == n * n ? 1.0 : 2.0) * (C * v);
   }
   __syncthreads();
   double t[6] = {0, 0, 0, 0, 0, 0};
   for (int rr = 0; rr < SGR / 4; rr++) {
     const int h = blockIdx.x * SGR + w * (SGR / 4) + rr;
-    if (h >= A.npts) break;  // rows left to this wavefront are all past the end
-    const double *row = A.Y + (size_t)h * P2;
+    if (h >= npts) break;  // rows left to this wavefront are all past the end
+    const double *row = Yg + (size_t)h * P2;
     double s[6] = {0, 0, 0, 0, 0, 0};
     for (int c = lane; c < P2; c += 64) {
       const double y = row[c];
@@ -1035,10 +1025,10 @@ __global__ void __launch_bounds__(256) k_surfgemv(SGArgs A) {
     }
     for (int b = 0; b < 6; b++) s[b] = wave_sum(s[b]);
     if (lane == 0) {
-      const size_t pt = (size_t)A.pt_glob + h;
+      const size_t pt = (size_t)pt_glob + h;
       double nf = 0.0, nh = 0.0;
       for (int d = 0; d < 3; d++) {
-        const double gf = s[d] * A.dA, gh = s[3 + d] * A.dA;
+        const double gf = s[d] * dA, gh = s[3 + d] * dA;
         A.gSF[(size_t)d * A.npts_all + pt] = gf;
         A.gSH[(size_t)d * A.npts_all + pt] = gh;
         t[d] += fabs(gf);
@@ -1055,11 +1045,11 @@ __global__ void __launch_bounds__(256) k_surfgemv(SGArgs A) {
     __syncthreads();
     if (lane == 0) red[w] = t[b];
     __syncthreads();
-    if (tid == 0) A.part[6 * blockIdx.x + b] = red[0] + red[1] + red[2] + red[3];
+    if (tid == 0) part[b] = red[0] + red[1] + red[2] + red[3];
   }
   if (A.work && tid == 0 && blockIdx.x == 0) {
-    atomicAdd(&A.work[0], (unsigned long long)(12.0 * P2 * A.npts));
-    atomicAdd(&A.work[1], (unsigned long long)(8.0 * P2 * A.npts + 64.0 * A.npts));
+    atomicAdd(&A.work[0], (unsigned long long)(12.0 * P2 * npts));
+    atomicAdd(&A.work[1], (unsigned long long)(8.0 * P2 * npts + 64.0 * npts));
   }
 }
 

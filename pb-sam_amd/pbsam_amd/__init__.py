@@ -33,6 +33,7 @@ SYMBOLS = [
     "pbsam_xform_intra_batch", "pbsam_xform_num_batch", "pbsam_rotate_quat_batch",
     "pbsam_profile", "pbsam_profile_read", "pbsam_sync", "pbsam_gradient", "pbsam_forces",
     "pbsam_compute_forces", "pbsam_system_get_grad", "pbsam_xform_h_grad_batch",
+    "pbsam_batch_create", "pbsam_batch_status",
 ]
 
 KERNELS = {"xform": 0, "p2l": 1, "solve": 2, "surface": 3, "xgrad": 4, "surfgrad": 5}
@@ -85,6 +86,8 @@ def load():
         "pbsam_moltype_get": ([P, I, P, P, P, P, P], I),
         "pbsam_moltype_get_intra": ([P, I, P, P, P, P], I),
         "pbsam_system_create": ([P, I, P, P, P, P], I),
+        "pbsam_batch_create": ([P, I, P, P, P, P, P], I),
+        "pbsam_batch_status": ([P, P, P, P], I),
         "pbsam_system_destroy": ([P], None),
         "pbsam_system_set_pose": ([P, I, P, P], I),
         "pbsam_system_set_cutoffs": ([P, D, D], I),
@@ -278,15 +281,26 @@ class MolType:
 class System:
     """Rigid molecules with poses; mutual polarization and energies on the device."""
 
-    def __init__(self, ctx, moltypes, pos, quat):
+    def __init__(self, ctx, moltypes, pos, quat, groups=None):
+        """groups: molecules per independent system (pbsam_batch_create), or None for one
+        system of all molecules (pbsam_system_create)."""
         self.ctx = ctx
         self.mts = list(moltypes)
         arr = (ctypes.c_void_p * len(moltypes))(*[m.h for m in moltypes])
         self._arr = arr
         pos, quat = f64(pos).reshape(-1), f64(quat).reshape(-1)
         h = ctypes.c_void_p()
-        ctx.check(ctx.L.pbsam_system_create(ctx.h, len(moltypes), ctypes.cast(arr, P), _p(pos), _p(quat),
-                                            ctypes.byref(h)))
+        if groups is None:
+            ctx.check(ctx.L.pbsam_system_create(ctx.h, len(moltypes), ctypes.cast(arr, P), _p(pos), _p(quat),
+                                                ctypes.byref(h)))
+            self.groups = [len(moltypes)]
+        else:
+            g = np.ascontiguousarray(groups, dtype=np.int32)
+            if int(g.sum()) != len(moltypes):
+                raise ValueError("groups must add up to the number of molecules")
+            ctx.check(ctx.L.pbsam_batch_create(ctx.h, len(g), g.ctypes.data_as(P), ctypes.cast(arr, P), _p(pos),
+                                               _p(quat), ctypes.byref(h)))
+            self.groups = [int(x) for x in g]
         self.h = h.value
         self.n = len(moltypes)
         self.p = moltypes[0].p
@@ -312,7 +326,7 @@ class System:
         r = ctypes.c_int()
         rc = self.ctx.L.pbsam_solve_mutual(self.h, tol, max_rounds, HROT_REFRESH if refresh else 0,
                                            ctypes.byref(r))
-        if rc not in (OK, ERR_NOCONV):
+        if rc not in (OK, ERR_NOCONV) and not (len(self.groups) > 1 and rc == ERR_OVERLAP):
             self.ctx.check(rc)
         return r.value, rc
 
@@ -342,7 +356,7 @@ class System:
         """computeForces: returns (rounds, status, force[n,3], torque[n,3])."""
         f, t, r = np.zeros(3 * self.n), np.zeros(3 * self.n), ctypes.c_int()
         rc = self.ctx.L.pbsam_compute_forces(self.h, tol, max_rounds, _p(f), _p(t), ctypes.byref(r))
-        if rc not in (OK, ERR_NOCONV):
+        if rc not in (OK, ERR_NOCONV) and not (len(self.groups) > 1 and rc == ERR_OVERLAP):
             self.ctx.check(rc)
         return r.value, rc, f.reshape(-1, 3), t.reshape(-1, 3)
 
@@ -355,6 +369,14 @@ class System:
         e = np.zeros(self.n)
         self.ctx.check(self.ctx.L.pbsam_energy(self.h, _p(e)))
         return e
+
+    def status(self):
+        """per group: (status, mutual rounds, gradient passes) of the last solve / gradient"""
+        n = len(self.groups)
+        st, r, g = (np.zeros(n, dtype=np.int32) for _ in range(3))
+        self.ctx.check(self.ctx.L.pbsam_batch_status(self.h, st.ctypes.data_as(P), r.ctypes.data_as(P),
+                                                     g.ctypes.data_as(P)))
+        return st, r, g
 
     def counts(self):
         a, b = ctypes.c_int(), ctypes.c_int()

@@ -1,0 +1,114 @@
+"""Batches of independent systems (pbsam_batch_create): every group's forces, torques,
+energies, mutual rounds and gradient passes against the same molecules solved as a system
+of their own, groups of one, two and three molecules, an overlapping group (no pairs, the
+others still solved), and one group against the oracle."""
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def moltypes(gpu, orc, n_sph, p, npts, seeds):
+    import pbsam_amd
+    import pyoracle
+    from pbsam_amd import synth
+    mols = [synth.molecule(n_sph, seed=s) for s in seeds]
+    gs, os_ = [], []
+    for m in mols:
+        args = (m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], p)
+        kw = dict(eps_in=4.0, rcen=m["rcen"], npts=[npts] * n_sph)
+        g, o = pbsam_amd.MolType(gpu, *args, **kw), pyoracle.MolType(orc, *args, **kw)
+        o.selfpol(0, 0.0, 0)
+        F, H = o.FH()
+        g.set_self(F, H)
+        o.set_self(F, H)
+        gs.append(g)
+        os_.append(o)
+    return mols, gs, os_
+
+
+def centres(mol, pos, q):
+    from pbsam_amd import synth
+    return np.array([pos + synth.quat_rot(q, c - mol["rcen"]) for c in mol["cen"]])
+
+
+def place_third(mols, pos, quat, mol_c, at, seed):
+    """a random orientation for mol_c at `at` that overlaps none of the placed molecules"""
+    from pbsam_amd import synth
+    rng = np.random.default_rng(seed)
+    for _ in range(1000):
+        q = synth.random_quat(rng)
+        cc = centres(mol_c, at, q)
+        ok = True
+        for m, p_, q_ in zip(mols, pos, quat):
+            c = centres(m, p_, q_)
+            gap = np.linalg.norm(cc[:, None, :] - c[None, :, :], axis=2) - mol_c["rad"][:, None] - m["rad"][None, :]
+            ok = ok and gap.min() > 0.5
+        if ok:
+            return q
+    raise RuntimeError("no overlap-free orientation")
+
+
+def build_batch(mols, gs, n_sph):
+    """groups: (A,B) (A,C) (B) (A,B,C) (A,B overlapping) (C,A); returns the per-group
+    molecule-type indices, positions and orientations"""
+    from pbsam_amd import synth
+    d = 2 * synth.ball_radius(n_sph) + 6.0
+    groups = []
+    p, q = synth.place_pair(mols[0], mols[1], d, seed=9)
+    groups.append(([0, 1], p, q))
+    p, q = synth.place_pair(mols[0], mols[2], d + 3.0, seed=11)
+    groups.append(([0, 2], p, q))
+    groups.append(([1], np.zeros((1, 3)), np.array([[1.0, 0.0, 0.0, 0.0]])))
+    p, q = synth.place_pair(mols[0], mols[1], d, seed=12)
+    at = np.array([0.5 * d, 0.9 * d, 0.0])
+    q3 = place_third([mols[0], mols[1]], p, q, mols[2], at, seed=13)
+    groups.append(([0, 1, 2], np.vstack([p, at]), np.vstack([q, q3])))
+    groups.append(([0, 1], np.array([[0.0, 0.0, 0.0], [2.0, 0.0, 0.0]]), np.tile([1.0, 0.0, 0.0, 0.0], (2, 1))))
+    p, q = synth.place_pair(mols[2], mols[0], d + 6.0, seed=14)
+    groups.append(([2, 0], p, q))
+    return groups
+
+
+@pytest.mark.parametrize("n_sph,p,npts", [(6, 10, 400), (5, 30, 1800)])
+def test_batch_equals_single_systems(gpu, orc, n_sph, p, npts):
+    import pbsam_amd
+    import pyoracle
+    mols, gs, os_ = moltypes(gpu, orc, n_sph, p, npts, seeds=(4, 5, 6))
+    groups = build_batch(mols, gs, n_sph)
+    mts = [gs[t] for g in groups for t in g[0]]
+    pos = np.vstack([g[1] for g in groups])
+    quat = np.vstack([g[2] for g in groups])
+    sizes = [len(g[0]) for g in groups]
+    batch = pbsam_amd.System(gpu, mts, pos, quat, groups=sizes)
+    batch.set_cutoffs(100.0, 100.0)
+    rounds, rc, fb, tb = batch.compute_forces(tol=0, max_rounds=0)
+    eb = batch.energy()
+    st, br, bp = batch.status()
+    assert rc == pbsam_amd.ERR_OVERLAP  # the first non-OK group
+    assert list(st) == [0, 0, 0, 0, pbsam_amd.ERR_OVERLAP, 0]
+    off = np.cumsum([0] + sizes)
+    for gi, (types, gp, gq) in enumerate(groups):
+        sl = slice(off[gi], off[gi + 1])
+        if gi == 4 or len(types) == 1:
+            assert np.all(fb[sl] == 0.0) and np.all(tb[sl] == 0.0), gi
+            continue
+        single = pbsam_amd.System(gpu, [gs[t] for t in types], gp, gq)
+        single.set_cutoffs(100.0, 100.0)
+        r1, rc1, f1, t1 = single.compute_forces(tol=0, max_rounds=0)
+        _, sr, sp = single.status()
+        assert rc1 == 0 and (br[gi], bp[gi]) == (sr[0], sp[0]), (gi, br[gi], bp[gi], sr[0], sp[0])
+        # the same kernels on the same data; only launch shapes (order-class split,
+        # point splits) change the rounding
+        assert relerr(fb[sl], f1) < 1e-10, (gi, fb[sl], f1)
+        assert relerr(tb[sl], t1) < 1e-10, (gi, tb[sl], t1)
+        assert relerr(eb[sl], single.energy()) < 1e-12, gi
+    # group 0 against the oracle (stock tolerances, reference order)
+    so = pyoracle.System(orc, [os_[0], os_[1]], groups[0][1], groups[0][2])
+    so.set_cutoffs(100.0, 100.0)
+    rco, fo, to = so.compute_forces(0, 0.0, 0)
+    assert rco == 0
+    assert relerr(fb[0:2], fo) < 1e-8
+    assert relerr(tb[0:2], to) < 1e-8
