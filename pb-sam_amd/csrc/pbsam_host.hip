@@ -259,9 +259,10 @@ static void build_tables(pbsam_ctx *c) {
 // ----------------------------------------------------------------------------
 // kernel launch helpers
 // ----------------------------------------------------------------------------
-static size_t xform_lds(int p) {
-  const int NR = 2 * p - 1;
-  return (size_t)2 * NR * NR * sizeof(cd) + (size_t)2 * p * p * sizeof(double) + (size_t)4 * p * sizeof(cd);
+// LDS of k_xform (its carve): one rotation level, the source blocks, the level vectors
+static size_t xform_lds(int q, int planes, bool with_f) {
+  const size_t NR = 2 * (size_t)q - 1, nb = planes * (with_f ? 2 : 1);
+  return NR * NR * sizeof(cd) + nb * q * q * sizeof(double) + 2 * nb * q * sizeof(cd);
 }
 static int p2l_tile(int p) {
   int t = (int)(60000 / (8 * (p * p + 2 * p + 3)));
@@ -284,17 +285,25 @@ static void launch_trans(pbsam_ctx *c, int njobs, const TJob *jobs, int p, doubl
   HIPCHK(hipGetLastError());
 }
 // one k_xform launch for n workgroups whose orders are all <= qmax
-static void launch_xform_q(pbsam_ctx *c, int n, XArgs a, int qmax) {
-  const size_t lds = xform_lds(qmax);
-  a.work = prof_begin(c, PBSAM_K_XFORM);
+template <int NPL>
+static void launch_xform_qp(pbsam_ctx *c, int n, const XArgs &a, int qmax) {
+  const size_t lds = xform_lds(qmax, NPL, a.F != nullptr);
   if (qmax <= 8) {
-    hipLaunchKernelGGL(k_xform<128>, dim3(n), dim3(128), lds, c->stream, c->tabs, a);
+    set_lds((const void *)k_xform<128, NPL>, lds);
+    hipLaunchKernelGGL((k_xform<128, NPL>), dim3(n), dim3(128), lds, c->stream, c->tabs, a);
   } else if (qmax <= 16) {
-    hipLaunchKernelGGL(k_xform<256>, dim3(n), dim3(256), lds, c->stream, c->tabs, a);
+    set_lds((const void *)k_xform<256, NPL>, lds);
+    hipLaunchKernelGGL((k_xform<256, NPL>), dim3(n), dim3(256), lds, c->stream, c->tabs, a);
   } else {
-    set_lds((const void *)k_xform<512>, lds);
-    hipLaunchKernelGGL(k_xform<512>, dim3(n), dim3(512), lds, c->stream, c->tabs, a);
+    set_lds((const void *)k_xform<512, NPL>, lds);
+    hipLaunchKernelGGL((k_xform<512, NPL>), dim3(n), dim3(512), lds, c->stream, c->tabs, a);
   }
+}
+static void launch_xform_q(pbsam_ctx *c, int n, XArgs a, int qmax) {
+  a.work = prof_begin(c, PBSAM_K_XFORM);
+  if (a.planes == 3) launch_xform_qp<3>(c, n, a, qmax);
+  else if (a.planes <= 1) launch_xform_qp<1>(c, n, a, qmax);
+  else fail(PBSAM_ERR_BADARG, "re-expansion planes must be 1 or 3");
   HIPCHK(hipGetLastError());
   prof_end(c, PBSAM_K_XFORM);
 }

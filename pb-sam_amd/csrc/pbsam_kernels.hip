@@ -127,7 +127,9 @@ __global__ void __launch_bounds__(64) k_trans(Tabs tb, const TJob *jobs, const i
 
 // ----------------------------------------------------------------------------
 // fused rotate -> translate -> rotate-back, one workgroup per pair.
-// LDS: two rotation levels (2q-1)^2 complex each, source blocks, per-level vectors.
+// LDS: one rotation level (2q-1)^2 complex (advanced in place), the source blocks and the
+// per-level vectors (the host's xform_lds).  NPL consecutive source blocks (planes: the x,y,z
+// gradient blocks of one sphere) share one generation of the rotation coefficients.
 // ----------------------------------------------------------------------------
 constexpr int RT = 512;  // threads per workgroup of k_rotcoeff
 constexpr int XG = 16;   // lanes per dot product in the forward rotate / translate
@@ -140,7 +142,7 @@ __device__ __forceinline__ double seg_reduce(double v) {
 struct XArgs {
   const XJob *jobs;
   const double *H;     // source H blocks (Hrot for inter pairs)
-  const double *F;     // source F blocks
+  const double *F;     // source F blocks (launches of kind-1 jobs)
   const double *TQH;   // source totals driving the truncation order
   const double *TQF;
   double *slotH, *slotF;
@@ -151,6 +153,7 @@ struct XArgs {
   int p;
   int full;            // 1: use order p for both blocks (no adaptive truncation)
   unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
+  int planes;          // consecutive source blocks / output slots per job (0 or 1: one)
 };
 
 // Orders of a batch of re-expansion jobs (the same rule as k_xform), per batch position,
@@ -193,11 +196,26 @@ __global__ void __launch_bounds__(256) k_xorder(const XJob *jobs, const int *sel
   }
 }
 
+// Mout(n, s) of one block for the singular rotations (theta = 0: identity; theta = pi:
+// Mout(n,mm) = s Min(n,mm), s = (-1)^n at mm = 0 and mm = 1, alternating over mm) as a
+// complex (n, m) value: re sign sg, im sign sgi (rotcoeff.cpp:368-393)
+__device__ __forceinline__ void sing_signs(int sing, int n, int m, double &sg, double &sgi) {
+  sg = 1.0;
+  sgi = 1.0;
+  if (sing == -1) {
+    const int s0 = (n % 2 == 0) ? 1 : -1;
+    sg = s0;
+    if (m > 0) sgi = -s0;
+  }
+}
+
 // XT threads per workgroup: 128 for q <= 8, 256 for q <= 16, 512 above (each thread owns
 // up to two of the q(q+1)/2 back-rotated outputs)
-template <int XT>
+template <int XT, int NPL>
 __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
   extern __shared__ double sh[];
+  constexpr int QM = (XT == 128) ? 8 : ((XT == 256) ? 16 : PMAX);
+  constexpr int NV = ((2 * QM - 2) * (2 * QM - 2) + XT - 1) / XT;  // staged recursion values per thread
   const int e = A.cls ? A.cls[blockIdx.x] : blockIdx.x;  // position in the batch
   const int job = A.sel ? A.sel[e] : e;
   const XJob J = A.jobs[job];
@@ -219,21 +237,24 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
   const int NR = 2 * q - 1;  // rotation rows n < 2q-1
   const int tq = J.tord ? J.tord : pH;  // layout order of the translation matrices
   if (A.work && tid == 0) {
-    atomicAdd(&A.work[0], (unsigned long long)work_xform(pH, intra ? pF : 0));
-    atomicAdd(&A.work[1], (unsigned long long)(8 * (pH * pH + pF * pF + (intra ? 2 : 1) * P2)));
+    double w = work_rotgen(q) + NPL * (2 * work_rotapply(pH) + work_transapply(pH));
+    if (intra) w += NPL * (2 * work_rotapply(pF) + work_transapply(pF));
+    atomicAdd(&A.work[0], (unsigned long long)w);
+    atomicAdd(&A.work[1], (unsigned long long)(8 * NPL * (pH * pH + pF * pF + (intra ? 2 : 1) * P2)));
   }
-  // LDS carve
-  cd *cur = reinterpret_cast<cd *>(sh);
-  cd *nxt = cur + NR * NR;
-  double *hs = reinterpret_cast<double *>(nxt + NR * NR);  // source H (q^2)
-  double *fs = hs + q * q;                                   // source F
-  cd *aH = reinterpret_cast<cd *>(fs + q * q);               // forward-rotated, per level [q]
-  cd *aF = aH + q;
-  cd *bH = aF + q;                                           // translated
-  cd *bF = bH + q;
-  for (int i = tid; i < q * q; i += XT) {
-    hs[i] = (i < pH * pH) ? A.H[(size_t)J.src * P2 + i] : 0.0;
-    fs[i] = (intra && i < pF * pF) ? A.F[(size_t)J.src * P2 + i] : 0.0;
+  // LDS carve (the host's xform_lds): R, source blocks, level vectors
+  const int nfb = A.F ? NPL : 0;
+  cd *R = reinterpret_cast<cd *>(sh);                        // ROT level m, row n at n^2 + n + s
+  double *hs = reinterpret_cast<double *>(R + NR * NR);     // [NPL][q^2] source H
+  double *fs = hs + NPL * q * q;                             // [nfb][q^2] source F
+  cd *aH = reinterpret_cast<cd *>(fs + nfb * q * q);         // [NPL][q] forward-rotated
+  cd *aF = aH + NPL * q;                                     // [nfb][q]
+  cd *bH = aF + nfb * q;                                     // [NPL][q] translated
+  cd *bF = bH + NPL * q;                                     // [nfb][q]
+  for (int i = tid; i < NPL * q * q; i += XT) {
+    const int r = i / (q * q), c = i % (q * q);
+    hs[i] = (c < pH * pH) ? A.H[(size_t)(J.src + r) * P2 + c] : 0.0;
+    if (nfb) fs[i] = (intra && c < pF * pF) ? A.F[(size_t)(J.src + r) * P2 + c] : 0.0;
   }
   // rotation parameters (CRotCoeff::initParams rotcoeff.cpp:209-244), xi = pi
   double sint = sin(th), cost;
@@ -259,19 +280,20 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
   // back-rotated accumulators: each thread owns up to two (n,s) outputs
   const int nout = q * (q + 1) / 2;
   int on[2], os[2];
-  cd accH[2], accF[2];
+  cd accH[2][NPL], accF[2][NPL];
+#pragma unroll
   for (int k = 0; k < 2; k++) {
-    int e = tid + k * XT;
+    int ek = tid + k * XT;
     on[k] = -1;
     os[k] = 0;
-    accH[k] = {0.0, 0.0};
-    accF[k] = {0.0, 0.0};
-    if (e < nout) {
-      int n = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-      while (n * (n + 1) / 2 > e) n--;
-      while ((n + 1) * (n + 2) / 2 <= e) n++;
+#pragma unroll
+    for (int r = 0; r < NPL; r++) accH[k][r] = accF[k][r] = {0.0, 0.0};
+    if (ek < nout) {
+      int n = (int)((sqrt(8.0 * ek + 1.0) - 1.0) * 0.5);
+      while (n * (n + 1) / 2 > ek) n--;
+      while ((n + 1) * (n + 2) / 2 <= ek) n++;
       on[k] = n;
-      os[k] = e - n * (n + 1) / 2;
+      os[k] = ek - n * (n + 1) / 2;
     }
   }
   if (sing != 0) {
@@ -279,48 +301,55 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
     __syncthreads();
     for (int m = 0; m < q; m++) {
       // forward "rotation" value a(n,m) = sign * c(n,m)
-      for (int n = m + tid; n < q; n += XT) {
-        double sg = 1.0, sgi = 1.0;
-        if (sing == -1) {
-          // Mout(n,mm) = s*Min(n,mm), s = (-1)^n at mm = 0 and mm = 1, alternating over mm
-          int s0 = (n % 2 == 0) ? 1 : -1;
-          sg = s0;                      // mm = 0 and odd mm (real parts)
-          if (m > 0) sgi = -s0;         // even mm >= 2 (imaginary parts)
+      for (int i = tid; i < NPL * q; i += XT) {
+        const int r = i / q, n = i % q;
+        if (n < m) continue;
+        double sg, sgi;
+        sing_signs(sing, n, m, sg, sgi);
+        const double *h = hs + r * q * q;
+        if (n < pH) aH[r * q + n] = (m == 0) ? cd{sg * h[n * n], 0.0}
+                                             : cd{sg * h[n * n + 2 * m - 1], sgi * h[n * n + 2 * m]};
+        else aH[r * q + n] = {0.0, 0.0};
+        if (nfb) {
+          const double *f = fs + r * q * q;
+          if (intra && n < pF) aF[r * q + n] = (m == 0) ? cd{sg * f[n * n], 0.0}
+                                                        : cd{sg * f[n * n + 2 * m - 1], sgi * f[n * n + 2 * m]};
+          else aF[r * q + n] = {0.0, 0.0};
         }
-        if (n < pH) aH[n] = (m == 0) ? cd{sg * hs[n * n], 0.0}
-                                     : cd{sg * hs[n * n + 2 * m - 1], sgi * hs[n * n + 2 * m]};
-        else aH[n] = {0.0, 0.0};
-        if (intra && n < pF) aF[n] = (m == 0) ? cd{sg * fs[n * n], 0.0}
-                                              : cd{sg * fs[n * n + 2 * m - 1], sgi * fs[n * n + 2 * m]};
-        else aF[n] = {0.0, 0.0};
       }
       __syncthreads();
-      for (int n = m + tid; n < q; n += XT) {
+      for (int i = tid; i < NPL * q; i += XT) {
+        const int r = i / q, n = i % q;
+        if (n < m) continue;
         const int w = tq - m;
         cd sH = {0, 0}, sF = {0, 0};
         if (n < pH) {
           const double *Mm = J.TH + tmat_off(tq, m) + (size_t)(n - m) * w;
-          for (int l = m; l < pH; l++) sH = cadd(sH, cscale(Mm[l - m], aH[l]));
+          for (int l = m; l < pH; l++) sH = cadd(sH, cscale(Mm[l - m], aH[r * q + l]));
         }
         if (intra && n < pF) {
           const double *Mm = J.TF + tmat_off(tq, m) + (size_t)(n - m) * w;
-          for (int l = m; l < pF; l++) sF = cadd(sF, cscale(Mm[l - m], aF[l]));
+          for (int l = m; l < pF; l++) sF = cadd(sF, cscale(Mm[l - m], aF[r * q + l]));
         }
-        bH[n] = sH;
-        bF[n] = sF;
+        bH[r * q + n] = sH;
+        if (nfb) bF[r * q + n] = sF;
       }
       __syncthreads();
+#pragma unroll
       for (int k = 0; k < 2; k++) {
-        int n = on[k], s = os[k];
+        const int n = on[k], s = os[k];
         if (n < 0 || s != m) continue;
-        double sg = 1.0, sgi = 1.0;
-        if (sing == -1) {
-          int s0 = (n % 2 == 0) ? 1 : -1;
-          sg = s0;
-          if (m > 0) sgi = -s0;
+        double sg, sgi;
+        sing_signs(sing, n, m, sg, sgi);
+#pragma unroll
+        for (int r = 0; r < NPL; r++) {
+          const cd b = bH[r * q + n];
+          accH[k][r] = (m == 0) ? cd{sg * b.x, 0.0} : cd{sg * b.x, sgi * b.y};
+          if (nfb) {
+            const cd bf = bF[r * q + n];
+            accF[k][r] = (m == 0) ? cd{sg * bf.x, 0.0} : cd{sg * bf.x, sgi * bf.y};
+          }
         }
-        accH[k] = (m == 0) ? cd{sg * bH[n].x, 0.0} : cd{sg * bH[n].x, sgi * bH[n].y};
-        accF[k] = (m == 0) ? cd{sg * bF[n].x, 0.0} : cd{sg * bF[n].x, sgi * bF[n].y};
       }
       __syncthreads();
     }
@@ -329,10 +358,10 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
     for (int m = tid; m < NR; m += XT) {
       sh_column(tb, NR, m, cost, sint, ph, [&](int n, double re, double im) {
         if (m == 0) {
-          cur[n * n + n] = {re, 0.0};
+          R[n * n + n] = {re, 0.0};
         } else {
-          cur[n * n + n + m] = {re, -im};  // ROT(n, m,0) = comp(n,-m)
-          cur[n * n + n - m] = {re, im};   // ROT(n,-m,0) = comp(n, m)
+          R[n * n + n + m] = {re, -im};  // ROT(n, m,0) = comp(n,-m)
+          R[n * n + n - m] = {re, im};   // ROT(n,-m,0) = comp(n, m)
         }
       });
     }
@@ -341,124 +370,175 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
     for (int m = 0; m < q; m++) {
       // (a) forward rotation of this level: a(n) for n in [m, q)
       {
-        int g = tid / XG, lane = tid % XG;
+        const int g = tid / XG, lane = tid % XG;
         for (int n = m + g; n < q; n += XT / XG) {
-          const cd *row = cur + n * n + n;  // row[s] = ROT(n,s,m)
-          cd sH = {0, 0}, sF = {0, 0};
+          const cd *row = R + n * n + n;  // row[s] = ROT(n,s,m)
+          cd sH[NPL], sF[NPL];
+#pragma unroll
+          for (int r = 0; r < NPL; r++) sH[r] = sF[r] = {0.0, 0.0};
           for (int s = lane; s <= n; s += XG) {
-            if (s == 0) {
-              if (n < pH) sH = cadd(sH, cmul(row[0], cd{hs[n * n], 0.0}));
-              if (intra && n < pF) sF = cadd(sF, cmul(row[0], cd{fs[n * n], 0.0}));
-            } else {
-              if (n < pH) {
-                cd c = {hs[n * n + 2 * s - 1], hs[n * n + 2 * s]};
-                sH = cadd(sH, cadd(cmul(row[s], c), cmul(row[-s], cconj(c))));
-              }
-              if (intra && n < pF) {
-                cd c = {fs[n * n + 2 * s - 1], fs[n * n + 2 * s]};
-                sF = cadd(sF, cadd(cmul(row[s], c), cmul(row[-s], cconj(c))));
+            const cd rp = row[s], rm = row[-s];
+#pragma unroll
+            for (int r = 0; r < NPL; r++) {
+              const double *h = hs + r * q * q, *f = fs + r * q * q;
+              if (s == 0) {
+                if (n < pH) sH[r] = cadd(sH[r], cmul(rp, cd{h[n * n], 0.0}));
+                if (intra && n < pF) sF[r] = cadd(sF[r], cmul(rp, cd{f[n * n], 0.0}));
+              } else {
+                if (n < pH) {
+                  const cd c = {h[n * n + 2 * s - 1], h[n * n + 2 * s]};
+                  sH[r] = cadd(sH[r], cadd(cmul(rp, c), cmul(rm, cconj(c))));
+                }
+                if (intra && n < pF) {
+                  const cd c = {f[n * n + 2 * s - 1], f[n * n + 2 * s]};
+                  sF[r] = cadd(sF[r], cadd(cmul(rp, c), cmul(rm, cconj(c))));
+                }
               }
             }
           }
-          sH.x = seg_reduce(sH.x);
-          sH.y = seg_reduce(sH.y);
-          sF.x = seg_reduce(sF.x);
-          sF.y = seg_reduce(sF.y);
-          if (lane == 0) {
-            if (m == 0) { sH.y = 0.0; sF.y = 0.0; }  // Mout(n,0) keeps the real part
-            aH[n] = sH;
-            aF[n] = sF;
+#pragma unroll
+          for (int r = 0; r < NPL; r++) {
+            sH[r].x = seg_reduce(sH[r].x);
+            sH[r].y = seg_reduce(sH[r].y);
+            if (nfb) {
+              sF[r].x = seg_reduce(sF[r].x);
+              sF[r].y = seg_reduce(sF[r].y);
+            }
+            if (lane == 0) {
+              if (m == 0) { sH[r].y = 0.0; sF[r].y = 0.0; }  // Mout(n,0) keeps the real part
+              aH[r * q + n] = sH[r];
+              if (nfb) aF[r * q + n] = sF[r];
+            }
           }
         }
       }
-      // (b) recursion to level m+1 (rotcoeff.cpp:801-809), rows n-1 for n in [m+2, 2q-m-2]
+      __syncthreads();
+      // (b) translate this level: b(n) = sum_l M_m[n][l] a(l)
+      {
+        const int g = tid / XG, lane = tid % XG;
+        const int w = tq - m;
+        for (int n = m + g; n < q; n += XT / XG) {
+          cd sH[NPL], sF[NPL];
+#pragma unroll
+          for (int r = 0; r < NPL; r++) sH[r] = sF[r] = {0.0, 0.0};
+          if (n < pH) {
+            const double *Mm = J.TH + tmat_off(tq, m) + (size_t)(n - m) * w;
+            for (int l = m + lane; l < pH; l += XG) {
+              const double t = Mm[l - m];
+#pragma unroll
+              for (int r = 0; r < NPL; r++) sH[r] = cadd(sH[r], cscale(t, aH[r * q + l]));
+            }
+          }
+          if (intra && n < pF) {
+            const double *Mm = J.TF + tmat_off(tq, m) + (size_t)(n - m) * w;
+            for (int l = m + lane; l < pF; l += XG) {
+              const double t = Mm[l - m];
+#pragma unroll
+              for (int r = 0; r < NPL; r++) sF[r] = cadd(sF[r], cscale(t, aF[r * q + l]));
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < NPL; r++) {
+            sH[r].x = seg_reduce(sH[r].x);
+            sH[r].y = seg_reduce(sH[r].y);
+            if (nfb) {
+              sF[r].x = seg_reduce(sF[r].x);
+              sF[r].y = seg_reduce(sF[r].y);
+            }
+            if (lane == 0) {
+              if (m == 0) { sH[r].y = 0.0; sF[r].y = 0.0; }
+              bH[r * q + n] = sH[r];
+              if (nfb) bF[r * q + n] = sF[r];
+            }
+          }
+        }
+      }
+      __syncthreads();
+      // (c) back-rotation contribution of level m (rotcoeff.cpp:420-438)
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        const int n = on[k], s = os[k];
+        if (n < m) continue;
+        const cd *row = R + n * n + n;
+        const cd rp = cconj(row[s]), rm = row[-s];
+#pragma unroll
+        for (int r = 0; r < NPL; r++) {
+          if (n < pH) {
+            const cd b = bH[r * q + n];
+            if (m == 0) accH[k][r] = cadd(accH[k][r], cmul(cscale(1.0, rm), cd{b.x, 0.0}));
+            else accH[k][r] = cadd(accH[k][r], cadd(cmul(rp, b), cmul(rm, cconj(b))));
+          }
+          if (intra && n < pF) {
+            const cd b = bF[r * q + n];
+            if (m == 0) accF[k][r] = cadd(accF[k][r], cmul(cscale(1.0, rm), cd{b.x, 0.0}));
+            else accF[k][r] = cadd(accF[k][r], cadd(cmul(rp, b), cmul(rm, cconj(b))));
+          }
+        }
+      }
+      // (d) recursion to level m+1 in place (rotcoeff.cpp:801-809), rows n-1 for n in
+      // [m+2, 2q-m-2]: the new values are staged in registers, then written over level m
       if (m < q - 1) {
         const int base = (m + 1) * (m + 1);
         const int total = (2 * q - m - 2) * (2 * q - m - 2) - (m + 1) * (m + 1);
-        for (int e = tid; e < total; e += XT) {
-          int t = e + base;
+        cd vR[NV];
+        int vi[NV];
+        int cnt = 0;
+#pragma unroll
+        for (int t0 = 0; t0 < NV; t0++) {
+          const int ee = tid + t0 * XT;
+          if (ee >= total) break;
+          int t = ee + base;
           int n = (int)sqrt((double)t);
           while (n * n > t) n--;
           while ((n + 1) * (n + 1) <= t) n++;
-          n += 1;                         // element t belongs to row n: start (n-1)^2
-          int s = t - (n - 1) * (n - 1) - (n - 1);
-          const cd *row = cur + n * n + n;
+          n += 1;  // element t belongs to row n: start (n-1)^2
+          const int s = t - (n - 1) * (n - 1) - (n - 1);
+          const cd *row = R + n * n + n;
           cd d1 = cscale(r1 * ETA(tb, n, s - 1), row[s - 1]);
           cd d2 = cscale(r2 * ETA(tb, n, -s - 1), row[s + 1]);
           cd d3 = cscale(r3 * ZETA(tb, n - 1, abs(s)), row[s]);
           cd v = cadd(cadd(cmul(d1, exiphi), cmul(d2, exphi)), d3);
           v = cmul(exxi, v);
-          double em = ETA(tb, n, m);
-          nxt[(n - 1) * (n - 1) + (n - 1) + s] = {v.x / em, v.y / em};
+          const double em = ETA(tb, n, m);
+          vR[t0] = {v.x / em, v.y / em};
+          vi[t0] = (n - 1) * (n - 1) + (n - 1) + s;
+          cnt = t0 + 1;
         }
+        __syncthreads();
+#pragma unroll
+        for (int t0 = 0; t0 < NV; t0++)
+          if (t0 < cnt) R[vi[t0]] = vR[t0];
       }
       __syncthreads();
-      // (c) translate this level: b(n) = sum_l M_m[n][l] a(l)
-      {
-        int g = tid / XG, lane = tid % XG;
-        const int w = tq - m;
-        for (int n = m + g; n < q; n += XT / XG) {
-          cd sH = {0, 0}, sF = {0, 0};
-          if (n < pH) {
-            const double *Mm = J.TH + tmat_off(tq, m) + (size_t)(n - m) * w;
-            for (int l = m + lane; l < pH; l += XG) sH = cadd(sH, cscale(Mm[l - m], aH[l]));
-          }
-          if (intra && n < pF) {
-            const double *Mm = J.TF + tmat_off(tq, m) + (size_t)(n - m) * w;
-            for (int l = m + lane; l < pF; l += XG) sF = cadd(sF, cscale(Mm[l - m], aF[l]));
-          }
-          sH.x = seg_reduce(sH.x);
-          sH.y = seg_reduce(sH.y);
-          sF.x = seg_reduce(sF.x);
-          sF.y = seg_reduce(sF.y);
-          if (lane == 0) {
-            if (m == 0) { sH.y = 0.0; sF.y = 0.0; }
-            bH[n] = sH;
-            bF[n] = sF;
-          }
-        }
-      }
-      __syncthreads();
-      // (d) back-rotation contribution of level m (rotcoeff.cpp:420-438)
-      for (int k = 0; k < 2; k++) {
-        int n = on[k], s = os[k];
-        if (n < m) continue;
-        const cd *row = cur + n * n + n;
-        if (n < pH) {
-          cd b = bH[n];
-          if (m == 0) accH[k] = cadd(accH[k], cmul(cscale(1.0, row[-s]), cd{b.x, 0.0}));
-          else accH[k] = cadd(accH[k], cadd(cmul(cconj(row[s]), b), cmul(row[-s], cconj(b))));
-        }
-        if (intra && n < pF) {
-          cd b = bF[n];
-          if (m == 0) accF[k] = cadd(accF[k], cmul(cscale(1.0, row[-s]), cd{b.x, 0.0}));
-          else accF[k] = cadd(accF[k], cadd(cmul(cconj(row[s]), b), cmul(row[-s], cconj(b))));
-        }
-      }
-      __syncthreads();
-      cd *tmp = cur;
-      cur = nxt;
-      nxt = tmp;
     }
   }
   // write slots (orders beyond the truncation are zero)
-  double *oH = A.slotH + (size_t)J.slot * P2;
-  double *oF = intra ? A.slotF + (size_t)J.slot * P2 : nullptr;
-  for (int i = tid; i < P2; i += XT) {
-    oH[i] = 0.0;
-    if (oF) oF[i] = 0.0;
+#pragma unroll
+  for (int r = 0; r < NPL; r++) {
+    double *oH = A.slotH + (size_t)(J.slot + r) * P2;
+    double *oF = intra ? A.slotF + (size_t)(J.slot + r) * P2 : nullptr;
+    for (int i = tid; i < P2; i += XT) {
+      oH[i] = 0.0;
+      if (oF) oF[i] = 0.0;
+    }
   }
   __syncthreads();
+#pragma unroll
   for (int k = 0; k < 2; k++) {
-    int n = on[k], s = os[k];
+    const int n = on[k], s = os[k];
     if (n < 0) continue;
-    if (n < pH) {
-      if (s == 0) oH[n * n] = accH[k].x;
-      else { oH[n * n + 2 * s - 1] = accH[k].x; oH[n * n + 2 * s] = accH[k].y; }
-    }
-    if (oF && n < pF) {
-      if (s == 0) oF[n * n] = accF[k].x;
-      else { oF[n * n + 2 * s - 1] = accF[k].x; oF[n * n + 2 * s] = accF[k].y; }
+#pragma unroll
+    for (int r = 0; r < NPL; r++) {
+      double *oH = A.slotH + (size_t)(J.slot + r) * P2;
+      if (n < pH) {
+        if (s == 0) oH[n * n] = accH[k][r].x;
+        else { oH[n * n + 2 * s - 1] = accH[k][r].x; oH[n * n + 2 * s] = accH[k][r].y; }
+      }
+      if (intra && n < pF) {
+        double *oF = A.slotF + (size_t)(J.slot + r) * P2;
+        if (s == 0) oF[n * n] = accF[k][r].x;
+        else { oF[n * n + 2 * s - 1] = accF[k][r].x; oF[n * n + 2 * s] = accF[k][r].y; }
+      }
     }
   }
 }
