@@ -13,6 +13,9 @@ polarization from the self values, the gradient phase and the forces and torques
 reference's Gauss-Seidel order with its stock stopping rules.  Inputs stay resident in
 HBM; the timed region is bracketed by stream synchronizations.  `solves_per_s` is the
 mutual solve alone (CMolecule::polarize_mutual, src/molecule.cpp:1989-2075), timed after.
+`batch` is the batched path: 256 Brownian-dynamics snapshots of the same two molecules
+(centre distance uniform in [50, 80] A) evaluated together through pbsam_batch_create,
+beside the plain one-system-per-call path timed on a sample of the same snapshots.
 
 Prints one JSON line.  `roofline` is the kernel with the largest share of device time,
 timed with HIP events around each of its launches on the library's stream, with the
@@ -31,7 +34,8 @@ sys.path.insert(0, os.path.join(ROOT, "pb-sam_amd"))
 
 METRIC = "mutual-polarization solves/s & force+torque evals/s, p=30, FP64 rel err ≤1e-8"
 N_SPH, P, NPTS, DIST = 60, 30, 1800, 50.0
-SEEDS, POSE_SEED = (101, 102), 7
+P_ORD = P
+SEEDS, POSE_SEED, BATCH_SEED = (101, 102), 7, 5000
 PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix), flop/s
 PEAK_HBM = 8.0e12        # MI355X HBM3E, B/s
 
@@ -64,6 +68,59 @@ def build_system(pbsam_amd, synth, ctx):
     system = pbsam_amd.System(ctx, mts, pos, quat)
     system.set_cutoffs(100.0, 100.0)
     return mols, mts, pos, quat, system
+
+
+def batch_leg(pbsam_amd, synth, ctx, mols, mts, nsys, reps, nplain):
+    """`nsys` Brownian-dynamics snapshots of the same two molecules (centre distance uniform
+    in [50, 80] A, random orientations, seeded) as one batch (pbsam_batch_create): complete
+    force+torque evaluations of every snapshot, timed over `reps` batched calls after one
+    warm-up.  The plain path, one pbsam_compute_forces per system, is timed on the first
+    `nplain` snapshots, whose forces must equal the batch's."""
+    import numpy as np
+    rng = np.random.default_rng(BATCH_SEED)
+    P, Q = [], []
+    for b in range(nsys):
+        p, q = synth.place_pair(mols[0], mols[1], rng.uniform(50.0, 80.0), seed=BATCH_SEED + 1 + b)
+        P.append(p)
+        Q.append(q)
+    P, Q = np.vstack(P), np.vstack(Q)
+    batch = pbsam_amd.System(ctx, [mts[0], mts[1]] * nsys, P, Q, groups=[2] * nsys)
+    batch.set_cutoffs(100.0, 100.0)
+    batch.compute_forces(tol=0, max_rounds=0)
+    ctx.sync()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, rc, fb, tb = batch.compute_forces(tol=0, max_rounds=0)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    prof = {k: ctx.profile_read(k) for k in pbsam_amd.KERNELS}
+    ctx.profile(False)
+    st, rounds, passes = batch.status()
+    # the plain path on a sample of the same systems
+    singles = [pbsam_amd.System(ctx, [mts[0], mts[1]], P[2 * b:2 * b + 2], Q[2 * b:2 * b + 2])
+               for b in range(nplain)]
+    for s in singles:
+        s.set_cutoffs(100.0, 100.0)
+    singles[0].compute_forces(tol=0, max_rounds=0)
+    ctx.sync()
+    err = 0.0
+    t0 = time.perf_counter()
+    for b, s in enumerate(singles):
+        _, _, f1, t1 = s.compute_forces(tol=0, max_rounds=0)
+        scale = max(np.abs(f1).max(), 1e-300)
+        err = max(err, float(np.abs(fb[2 * b:2 * b + 2] - f1).max() / scale))
+    ctx.sync()
+    plain = nplain / (time.perf_counter() - t0)
+    value = reps * nsys / el
+    return dict(systems=nsys, workload="%d snapshots of the two %d-sphere molecules, p=%d, %d points/sphere, "
+                                      "centre distance U[50,80] A, random orientations" % (nsys, N_SPH, P_ORD, NPTS),
+                evals_per_s=value, ms_per_batch=1e3 * el / reps, reps=reps,
+                plain_loop_evals_per_s=plain, plain_loop_sample=nplain, speedup_vs_plain_loop=value / plain,
+                max_rel_err_force_batch_vs_plain=err, groups_ok=int((st == 0).sum()),
+                mutual_rounds_max=int(rounds.max()), grad_passes_mean=float(passes.mean()),
+                kernels={k: dict(launches=v["launches"], ms_per_batch=v["ms"] / reps,
+                                 gflop_per_batch=v["flops"] / reps / 1e9) for k, v in prof.items()})
 
 
 def cpu_baseline(kappa, mols, mts, pos, quat, gpu_force, gpu_torque, budget_s):
@@ -118,6 +175,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0,
                     help="seconds of oracle force evaluations to time (at least one runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--batch", type=int, default=256, help="systems in the batched leg (0: skip it)")
+    ap.add_argument("--batch-reps", type=int, default=2, help="timed batched calls")
+    ap.add_argument("--plain-sample", type=int, default=8, help="systems timed one by one for the ratio")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -181,6 +241,13 @@ def main():
     kernels = {k: dict(launches=v["launches"], ms_per_step=v["ms"] / args.steps,
                        gflop_per_step=v["flops"] / args.steps / 1e9) for k, v in prof.items()}
 
+    batch = None
+    if args.batch > 0:
+        batch = batch_leg(pbsam_amd, synth, ctx, mols, mts, args.batch, args.batch_reps,
+                          min(args.plain_sample, args.batch))
+        print("batch of %d systems: %.1f evals/s (plain loop %.1f evals/s)" %
+              (args.batch, batch["evals_per_s"], batch["plain_loop_evals_per_s"]), file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(kappa, mols, mts, pos, quat, force, torque, args.cpu_budget)
@@ -206,6 +273,7 @@ def main():
             "separation_A": DIST, "inter_rcutoff_A": 100.0, "mutual_rounds": rounds,
         },
         "solves_per_s": solves_per_s,
+        "batch": batch,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,
