@@ -288,22 +288,36 @@ static void launch_trans(pbsam_ctx *c, int njobs, const TJob *jobs, int p, doubl
 template <int NPL>
 static void launch_xform_qp(pbsam_ctx *c, int n, const XArgs &a, int qmax) {
   const size_t lds = xform_lds(qmax, NPL, a.F != nullptr);
+  const dim3 grid(n, (NPL == 1 && a.split) ? a.planes : 1);
   if (qmax <= 8) {
     set_lds((const void *)k_xform<128, NPL>, lds);
-    hipLaunchKernelGGL((k_xform<128, NPL>), dim3(n), dim3(128), lds, c->stream, c->tabs, a);
+    hipLaunchKernelGGL((k_xform<128, NPL>), grid, dim3(128), lds, c->stream, c->tabs, a);
   } else if (qmax <= 16) {
     set_lds((const void *)k_xform<256, NPL>, lds);
-    hipLaunchKernelGGL((k_xform<256, NPL>), dim3(n), dim3(256), lds, c->stream, c->tabs, a);
+    hipLaunchKernelGGL((k_xform<256, NPL>), grid, dim3(256), lds, c->stream, c->tabs, a);
   } else {
     set_lds((const void *)k_xform<512, NPL>, lds);
-    hipLaunchKernelGGL((k_xform<512, NPL>), dim3(n), dim3(512), lds, c->stream, c->tabs, a);
+    hipLaunchKernelGGL((k_xform<512, NPL>), grid, dim3(512), lds, c->stream, c->tabs, a);
   }
 }
+// Multi-plane jobs share one rotation generation per workgroup; small launches (fewer
+// workgroups than the chip has CUs) instead run one plane per workgroup, trading that
+// sharing for parallelism.
+static constexpr int XSPLIT_MAX = 256;
 static void launch_xform_q(pbsam_ctx *c, int n, XArgs a, int qmax) {
   a.work = prof_begin(c, PBSAM_K_XFORM);
-  if (a.planes == 3) launch_xform_qp<3>(c, n, a, qmax);
-  else if (a.planes <= 1) launch_xform_qp<1>(c, n, a, qmax);
-  else fail(PBSAM_ERR_BADARG, "re-expansion planes must be 1 or 3");
+  if (a.planes == 3 && n < XSPLIT_MAX && !a.cls) {
+    a.split = 1;
+    launch_xform_qp<1>(c, n, a, qmax);
+  } else if (a.planes == 3) {
+    a.split = 0;
+    launch_xform_qp<3>(c, n, a, qmax);
+  } else if (a.planes <= 1) {
+    a.split = 0;
+    launch_xform_qp<1>(c, n, a, qmax);
+  } else {
+    fail(PBSAM_ERR_BADARG, "re-expansion planes must be 1 or 3");
+  }
   HIPCHK(hipGetLastError());
   prof_end(c, PBSAM_K_XFORM);
 }
@@ -387,7 +401,8 @@ static void launch_rotcoeff(pbsam_ctx *c, int n, RJob *jobs, int p, double *out)
 static void launch_rotapply(pbsam_ctx *c, int n, const RAJob *jobs, const RJob *rj, const double *Rst,
                             const double *in, double *out, int p, const int *sel = nullptr) {
   if (!n) return;
-  hipLaunchKernelGGL(k_rotapply, dim3(n), dim3(256), 0, c->stream, jobs, sel, rj, Rst, in, out, p);
+  const int chunks = (p * (p + 1) / 2 + RAT - 1) / RAT;
+  hipLaunchKernelGGL(k_rotapply, dim3(n, chunks), dim3(RAT), 0, c->stream, jobs, sel, rj, Rst, in, out, p);
   HIPCHK(hipGetLastError());
 }
 static void launch_sumslots(pbsam_ctx *c, int n, const LArgs &a) {

@@ -154,6 +154,7 @@ struct XArgs {
   int full;            // 1: use order p for both blocks (no adaptive truncation)
   unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
   int planes;          // consecutive source blocks / output slots per job (0 or 1: one)
+  int split;           // 1: one plane per workgroup, plane = blockIdx.y (single-plane kernel)
 };
 
 // Orders of a batch of re-expansion jobs (the same rule as k_xform), per batch position,
@@ -218,10 +219,15 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
   constexpr int NV = ((2 * QM - 2) * (2 * QM - 2) + XT - 1) / XT;  // staged recursion values per thread
   const int e = A.cls ? A.cls[blockIdx.x] : blockIdx.x;  // position in the batch
   const int job = A.sel ? A.sel[e] : e;
-  const XJob J = A.jobs[job];
+  XJob J = A.jobs[job];
   const int p = A.p, P2 = p * p;
   const int tid = threadIdx.x;
   const bool intra = (J.kind == 1);
+  const int tqsrc = J.src;  // totals index of the job's source sphere (its first plane)
+  if (NPL == 1 && A.split) {
+    J.src += blockIdx.y;
+    J.slot += blockIdx.y;
+  }
   double rho, th, ph;
   cart_to_sph(J.P[0], J.P[1], J.P[2], rho, th, ph);
   int pH, pF = 0;
@@ -229,7 +235,7 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
     pH = A.ordH[e];
     if (intra) pF = A.ordF[e];
   } else {
-    const int tqi = A.tqmap ? A.tqmap[J.src] : J.src;
+    const int tqi = A.tqmap ? A.tqmap[tqsrc] : tqsrc;
     pH = A.full ? p : compute_order(rho, A.TQH[tqi], J.a_src, p);
     if (intra) pF = A.full ? p : compute_order(rho, A.TQF[tqi], J.a_src, p);
   }
@@ -793,64 +799,100 @@ struct RAJob {
   int rot;           // rotation job index
   int fwd;
 };
-__global__ void __launch_bounds__(256) k_rotapply(const RAJob *jobs, const int *sel, const RJob *rj,
+// Grid (job, output chunk of RAT): the input block is staged in LDS and each thread forms
+// one output (n, k) as the reference's sum over s (or m) in order, with the rotation
+// coefficients loaded ahead.
+constexpr int RAT = 128;
+__global__ void __launch_bounds__(RAT) k_rotapply(const RAJob *jobs, const int *sel, const RJob *rj,
                                                   const double *Rst, const double *in, double *out, int p) {
+  __shared__ double mi_s[PMAX * PMAX];
   const RAJob J = jobs[sel ? sel[blockIdx.x] : blockIdx.x];
   const RJob R = rj[J.rot];
   const int P2 = p * p;
   const double *Min = in + (size_t)J.in * P2;
   double *Mout = out + (size_t)J.out * P2;
+  for (int i = threadIdx.x; i < P2; i += RAT) mi_s[i] = Min[i];
+  __syncthreads();
   const cd *C = reinterpret_cast<const cd *>(Rst + R.off);
   const int nout = p * (p + 1) / 2;
-  for (int e = threadIdx.x; e < nout; e += 256) {
-    int n = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-    while (n * (n + 1) / 2 > e) n--;
-    while ((n + 1) * (n + 1 + 1) / 2 <= e) n++;
-    const int k = e - n * (n + 1) / 2;  // m (forward) or s (transposed)
-    const double *mi = Min + n * n;
-    if (R.sing != 0) {
-      double xi = J.fwd ? R.xi : -R.xi;
-      if (k == 0) {
-        Mout[n * n] = (R.sing == -1 && (n % 2)) ? -mi[0] : mi[0];
-      } else {
-        cd inc = {mi[2 * k - 1], mi[2 * k]};
-        double sx, cx;
-        sincos(k * xi, &sx, &cx);
-        cd o = cmul(cd{cx, sx}, inc);
-        if (R.sing == -1) {
-          int s = (n % 2 == 0) ? 1 : -1;
-          Mout[n * n + 2 * k - 1] = s * o.x;
-          Mout[n * n + 2 * k] = -s * o.y;
-        } else {
-          Mout[n * n + 2 * k - 1] = o.x;
-          Mout[n * n + 2 * k] = o.y;
-        }
-      }
-      continue;
-    }
-    const cd *row = C + rot_off(n);  // row[m*(2n+1) + s + n]
-    const int W = 2 * n + 1;
-    cd o;
-    if (J.fwd) {
-      const int m = k;
-      o = cmul(cscale(1.0, row[m * W + n]), cd{mi[0], 0.0});
-      for (int s = 1; s <= n; s++) {
-        cd inc = {mi[2 * s - 1], mi[2 * s]};
-        o = cadd(o, cadd(cmul(row[m * W + n + s], inc), cmul(row[m * W + n - s], cconj(inc))));
-      }
+  const int e = blockIdx.y * RAT + threadIdx.x;
+  if (e >= nout) return;
+  int n = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+  while (n * (n + 1) / 2 > e) n--;
+  while ((n + 1) * (n + 1 + 1) / 2 <= e) n++;
+  const int k = e - n * (n + 1) / 2;  // m (forward) or s (transposed)
+  const double *mi = mi_s + n * n;
+  if (R.sing != 0) {
+    double xi = J.fwd ? R.xi : -R.xi;
+    if (k == 0) {
+      Mout[n * n] = (R.sing == -1 && (n % 2)) ? -mi[0] : mi[0];
     } else {
-      const int s = k;
-      o = cmul(cscale(1.0, row[0 * W + n - s]), cd{mi[0], 0.0});
-      for (int m = 1; m <= n; m++) {
-        cd inc = {mi[2 * m - 1], mi[2 * m]};
-        o = cadd(o, cadd(cmul(cconj(row[m * W + n + s]), inc), cmul(row[m * W + n - s], cconj(inc))));
+      cd inc = {mi[2 * k - 1], mi[2 * k]};
+      double sx, cx;
+      sincos(k * xi, &sx, &cx);
+      cd o = cmul(cd{cx, sx}, inc);
+      if (R.sing == -1) {
+        int sg = (n % 2 == 0) ? 1 : -1;
+        Mout[n * n + 2 * k - 1] = sg * o.x;
+        Mout[n * n + 2 * k] = -sg * o.y;
+      } else {
+        Mout[n * n + 2 * k - 1] = o.x;
+        Mout[n * n + 2 * k] = o.y;
       }
     }
-    if (k == 0) Mout[n * n] = o.x * 1.0;
-    else {
-      Mout[n * n + 2 * k - 1] = o.x;
-      Mout[n * n + 2 * k] = o.y;
+    return;
+  }
+  const cd *row = C + rot_off(n);  // row[m*(2n+1) + s + n]
+  const int W = 2 * n + 1;
+  cd o;
+  if (J.fwd) {
+    const int m = k;
+    const cd *rm = row + m * W + n;  // rm[s] = ROT(n, s, m)
+    o = cmul(cscale(1.0, rm[0]), cd{mi[0], 0.0});
+    int s = 1;
+    for (; s + 4 <= n + 1; s += 4) {
+      cd a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        a[u] = rm[s + u];
+        b[u] = rm[-(s + u)];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const cd inc = {mi[2 * (s + u) - 1], mi[2 * (s + u)]};
+        o = cadd(o, cadd(cmul(a[u], inc), cmul(b[u], cconj(inc))));
+      }
     }
+    for (; s <= n; s++) {
+      const cd inc = {mi[2 * s - 1], mi[2 * s]};
+      o = cadd(o, cadd(cmul(rm[s], inc), cmul(rm[-s], cconj(inc))));
+    }
+  } else {
+    const int s = k;
+    o = cmul(cscale(1.0, row[0 * W + n - s]), cd{mi[0], 0.0});
+    int m = 1;
+    for (; m + 4 <= n + 1; m += 4) {
+      cd a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        a[u] = row[(m + u) * W + n + s];
+        b[u] = row[(m + u) * W + n - s];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const cd inc = {mi[2 * (m + u) - 1], mi[2 * (m + u)]};
+        o = cadd(o, cadd(cmul(cconj(a[u]), inc), cmul(b[u], cconj(inc))));
+      }
+    }
+    for (; m <= n; m++) {
+      const cd inc = {mi[2 * m - 1], mi[2 * m]};
+      o = cadd(o, cadd(cmul(cconj(row[m * W + n + s]), inc), cmul(row[m * W + n - s], cconj(inc))));
+    }
+  }
+  if (k == 0) Mout[n * n] = o.x * 1.0;
+  else {
+    Mout[n * n + 2 * k - 1] = o.x;
+    Mout[n * n + 2 * k] = o.y;
   }
 }
 
@@ -985,27 +1027,96 @@ __device__ __forceinline__ double id_sweep(IdState<KW> &S, double (&oHr)[KW], do
   }
   return sum;
 }
+// One sweep as id_sweep without the computeDev sum: true when one of this lane's pairs
+// alone makes the sweep's computeDev exceed 1e-8 -- its term s (s >= 0 for every pair)
+// exceeds c = 4 p^2 1e-8 with a relative margin of 1e-6, tested as top > c bot without
+// the division.  The sum of the nonnegative terms is then above the threshold whatever
+// the summation order.
+template <int KW>
+__device__ __forceinline__ bool id_sweep_fast(IdState<KW> &S, double (&oHr)[KW], double (&oHi)[KW], double c) {
+  bool big = false;
+  for (int k = 0; k < KW; k++) {
+    if (S.nn[k] < 0) continue;
+    const bool mpos = (S.i0[k] != S.nn[k] * S.nn[k]);
+    double xr = S.fbr[k] + S.kH1[k] * S.Hr[k] + S.kF1[k] * S.Fr[k];
+    double xi = S.fbi[k] + S.kH1[k] * S.Hi[k] + S.kF1[k] * S.Fi[k];
+    S.Fr[k] = xr * S.kInv[k];
+    S.Fi[k] = mpos ? xi * S.kInv[k] : 0.0;
+    xr = S.hbr[k] + S.kH2[k] * S.Hr[k] + S.kF2[k] * S.Fr[k];
+    xi = S.hbi[k] + S.kH2[k] * S.Hi[k] + S.kF2[k] * S.Fi[k];
+    S.Hr[k] = xr * S.kIH[k];
+    S.Hi[k] = mpos ? xi * S.kIH[k] : 0.0;
+    const double ar = oHr[k], ai = mpos ? oHi[k] : 0.0, br = S.Hr[k], bi = mpos ? S.Hi[k] : 0.0;
+    const double PL = 1e-30;
+    const bool atiny = fabs(ar) < PL && fabs(ai) < PL, btiny = fabs(br) < PL && fabs(bi) < PL;
+    if (atiny != btiny) {
+      big = true;  // s = 1
+    } else if (!atiny) {
+      const double dr = ar - br, di = ai - bi;
+      const double top = (mpos ? 2.0 : 1.0) * (dr * dr + di * di);
+      const double bot = ar * ar + ai * ai + br * br + bi * bi;
+      big = big || (top > c * bot);
+    }
+    oHr[k] = S.Hr[k];
+    oHi[k] = S.Hi[k];
+  }
+  return big;
+}
+
 // Sweeps until the summed computeDev of all NW wavefronts' pairs <= 1e-8 (or 100 sweeps),
 // the reference's loop (expcenter.cpp:1097-1135).  The stopping test is evaluated in
 // chunks of IC sweeps: each chunk runs IC sweeps from a saved state, exchanges the IC
 // per-sweep partials with one barrier, and if the loop ends inside the chunk restores the
 // state and reruns the sweeps up to the stopping one -- the same arithmetic in the same
-// order as sweep-by-sweep testing.  red holds 2 * IC * NW doubles.  Returns the sweeps.
+// order as sweep-by-sweep testing.  A chunk first runs with id_sweep_fast: when every
+// sweep of it whose test can stop the loop is shown to continue by a single pair, the
+// exact sums are not needed; otherwise it is rerun from the saved state with them.
+// red holds 2 * IC * NW doubles, then 2 * NW doubles the caller uses after the loop,
+// then 2 * (NW + 1) ints of flags.  Returns the sweeps.
 constexpr int IC = 10;  // divides the 100-sweep cap
 template <int KW, int NW>
 __device__ int id_iterate(IdState<KW> &S, int p, double *red) {
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  int *flags = reinterpret_cast<int *>(red + 2 * IC * NW + 2 * NW);
   double oHr[KW], oHi[KW];
   for (int k = 0; k < KW; k++) {
     oHr[k] = S.Hr[k];
     oHi[k] = S.Hi[k];
   }
+  const double cthr = 4.0 * p * p * 1e-8 * (1.0 + 1e-6);
   int ct = 0;
   for (int chunk = 0;; chunk++) {
     double sFr[KW], sFi[KW], sHr[KW], sHi[KW], sOr[KW], sOi[KW];
     for (int k = 0; k < KW; k++) {
       sFr[k] = S.Fr[k]; sFi[k] = S.Fi[k]; sHr[k] = S.Hr[k]; sHi[k] = S.Hi[k];
       sOr[k] = oHr[k]; sOi[k] = oHi[k];
+    }
+    {
+      // fast pass: bit t of `mine` set when this lane shows sweep t continues
+      unsigned mine = 0;
+      for (int t = 0; t < IC; t++)
+        if (id_sweep_fast(S, oHr, oHi, cthr)) mine |= 1u << t;
+      unsigned wb = 0;
+      for (int t = 0; t < IC; t++)
+        if (__ballot((mine >> t) & 1u)) wb |= 1u << t;
+      int *fb = flags + (chunk & 1) * (NW + 1);  // double-buffered with the sums below
+      if (lane == 0) fb[w] = (int)wb;
+      __syncthreads();
+      unsigned all = 0;
+      for (int x = 0; x < NW; x++) all |= (unsigned)fb[x];
+      // sweeps whose test can stop the loop: done = ct + t + 1 < 100
+      unsigned need = 0;
+      for (int t = 0; t < IC; t++)
+        if (ct + t + 1 < 100) need |= 1u << t;
+      if ((all & need) == need) {
+        ct += IC;
+        if (ct >= 100) return ct;
+        continue;
+      }
+      for (int k = 0; k < KW; k++) {
+        S.Fr[k] = sFr[k]; S.Fi[k] = sFi[k]; S.Hr[k] = sHr[k]; S.Hi[k] = sHi[k];
+        oHr[k] = sOr[k]; oHi[k] = sOi[k];
+      }
     }
     double part[IC];
     for (int t = 0; t < IC; t++) part[t] = id_sweep(S, oHr, oHi);
@@ -1062,7 +1173,7 @@ __device__ __forceinline__ void id_dev2(const IdState<KW> &S, double &td, double
 template <int KW>
 __device__ void solve_identity(const SArgs &A, int g, int lst0, int lst1) {
   constexpr int NW = ST / 64;
-  __shared__ double red[2 * IC * NW + 2 * NW];
+  __shared__ double red[2 * IC * NW + 2 * NW + NW + 1];
   const int p = A.p, P2 = p * p;
   const double *cs = A.consts + (size_t)g * 10 * p;
   const double *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p;
@@ -1440,7 +1551,16 @@ __global__ void __launch_bounds__(256) k_sumslots(LArgs A) {
   const int go = A.obs ? A.obs[e0] : g;
   for (int i = blockIdx.y * 256 + threadIdx.x; i < P2; i += 256 * gridDim.y) {
     double s = A.base ? A.base[(size_t)g * P2 + i] : 0.0;
-    for (int e = l0; e < l1; e++) s += A.slot[(size_t)A.lst[e] * P2 + i];
+    int e = l0;
+    // eight loads in flight, added in list order
+    for (; e + 8 <= l1; e += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = A.slot[(size_t)A.lst[e + u] * P2 + i];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; e < l1; e++) s += A.slot[(size_t)A.lst[e] * P2 + i];
     A.out[(size_t)go * P2 + i] = s;
   }
 }

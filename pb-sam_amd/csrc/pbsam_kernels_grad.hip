@@ -540,7 +540,7 @@ __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
   const int g = A.sph[e0], e = A.ent[e0], lb = A.lfi ? A.lfi[e0] : blockIdx.x;
   const double *IM = A.imat[g];
   if (!IM) {  // uniform across the workgroup
-    __shared__ double red[2 * IC * 3 * WPC + 2 * 3 * WPC];
+    __shared__ double red[2 * IC * 3 * WPC + 2 * 3 * WPC + 3 * WPC + 1];
     if (p <= 22) solve3_identity<1>(A, g, e, lb, red);
     else if (p <= 31) solve3_identity<2>(A, g, e, lb, red);
     else solve3_identity<3>(A, g, e, lb, red);
@@ -930,8 +930,18 @@ __global__ void __launch_bounds__(256) k_p2lsum(const MJob *jobs, const int *sel
   const MJob J = jobs[sel ? sel[blockIdx.x] : blockIdx.x];
   const int P2 = p * p;
   for (int e = blockIdx.y * 256 + threadIdx.x; e < 2 * R * P2; e += 256 * gridDim.y) {
+    const double *pp = part + ((size_t)blockIdx.x * nsplit * 2) * R * P2 + e;
+    const size_t stride = (size_t)2 * R * P2;
     double s = 0.0;
-    for (int sp = 0; sp < nsplit; sp++) s += part[(((size_t)blockIdx.x * nsplit + sp) * 2) * R * P2 + e];
+    int sp = 0;
+    for (; sp + 8 <= nsplit; sp += 8) {  // eight loads in flight, added in split order
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = pp[(sp + u) * stride];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; sp < nsplit; sp++) s += pp[sp * stride];
     const int half = e / (R * P2), rc = e % (R * P2);
     (half == 0 ? slotF : slotH)[(size_t)J.slot * P2 + rc] = s;
   }
@@ -1005,7 +1015,7 @@ __global__ void __launch_bounds__(256) k_surfgemv(SGArgs A) {
   const double *Yg = A.Y[g];
   for (int e = tid; e < 6 * P2; e += 256) {
     const int b = e / P2, c = e % P2;
-    int n = (int)sqrt((double)c);
+    int n = (int)sqrtf((float)c);  // exact floor for c < 2^22 after the corrections
     while (n * n > c) n--;
     while ((n + 1) * (n + 1) <= c) n++;
     const double C = (b < 3) ? (2 * n + 1) / (4.0 * M_PI) : cChgH[n];
