@@ -79,6 +79,29 @@ struct DBuf {
   }
 };
 
+// pinned host buffer (grow-only; the caller synchronizes before it grows or is rewritten)
+template <class T>
+struct PinBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  PinBuf() = default;
+  PinBuf(const PinBuf &) = delete;
+  PinBuf &operator=(const PinBuf &) = delete;
+  ~PinBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  T *get(size_t count) {
+    if (count > n) {
+      if (p) HIPCHK(hipHostFree(p));
+      p = nullptr;
+      n = 0;
+      HIPCHK(hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault));
+      n = count;
+    }
+    return p;
+  }
+};
+
 inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 inline void cross3(const double *a, const double *b, double *o) {
   double x = a[1] * b[2] - a[2] * b[1], y = a[2] * b[0] - a[0] * b[2], z = a[0] * b[1] - a[1] * b[0];
@@ -138,7 +161,19 @@ struct pbsam_ctx {
   DBuf<unsigned long long> d_work;  // [kernel][flops, bytes]
   DBuf<double> d_qpart;             // per-workgroup partial surface-charge totals
   DBuf<int> d_xbin;                 // re-expansion orders, order classes and their counts
+  DBuf<int> d_xbin2;                // the same for launches on stream2
   KProf kp[PBSAM_K_COUNT];
+  // a second stream (default priority) for independent launch chains, joined by events
+  hipStream_t stream0 = nullptr, stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+};
+
+// Launch helpers use c->stream; StreamSwap points it at another stream for a scope.
+struct StreamSwap {
+  pbsam_ctx *c;
+  hipStream_t old;
+  StreamSwap(pbsam_ctx *c_, hipStream_t s) : c(c_), old(c_->stream) { c->stream = s; }
+  ~StreamSwap() { c->stream = old; }
 };
 
 static void prof_harvest(pbsam_ctx *c, int k) {
@@ -322,9 +357,11 @@ static void launch_xform_q(pbsam_ctx *c, int n, XArgs a, int qmax) {
   prof_end(c, PBSAM_K_XFORM);
 }
 // A batch of re-expansions.  Large batches with adaptive orders are first split by order
-// class (k_xorder) so that low-order pairs run in small workgroups with little LDS; this
-// costs one 16-byte read-back.  Results do not depend on the split.
+// class on the device (k_xorder) so that low-order pairs run in small workgroups with
+// little LDS; each class launch loops over its device-side count, so nothing is read back.
+// Results do not depend on the split.
 static constexpr int XBIN_MIN = 512;
+static constexpr int XBIN_GRID = 2048;  // workgroups of a class launch
 static void launch_xform(pbsam_ctx *c, int njobs, XArgs a) {
   if (!njobs) return;
   if (a.ordH || a.full || njobs < XBIN_MIN) {
@@ -332,26 +369,27 @@ static void launch_xform(pbsam_ctx *c, int njobs, XArgs a) {
     return;
   }
   const size_t need = (size_t)(2 + NXCLASS) * njobs + NXCLASS;
-  if (need > c->d_xbin.n) {
+  DBuf<int> &xb = (c->stream == c->stream2) ? c->d_xbin2 : c->d_xbin;  // one per stream
+  if (need > xb.n) {
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->d_xbin.alloc(std::max(need, (size_t)1 << 16));
+    xb.alloc(std::max(need, (size_t)1 << 16));
   }
-  int *ordH = c->d_xbin.p, *ordF = ordH + njobs, *lists = ordF + njobs, *cnt = lists + (size_t)NXCLASS * njobs;
+  int *ordH = xb.p, *ordF = ordH + njobs, *lists = ordF + njobs, *cnt = lists + (size_t)NXCLASS * njobs;
   HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int) * NXCLASS, c->stream));
   hipLaunchKernelGGL(k_xorder, dim3((njobs + 255) / 256), dim3(256), 0, c->stream, a.jobs, a.sel, njobs, a.TQH,
                      a.TQF, a.tqmap, a.p, a.full, ordH, ordF, cnt, lists);
   HIPCHK(hipGetLastError());
-  int h[NXCLASS];
-  HIPCHK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  // one launch per class, its workgroups looping over the class's device-side count (no
+  // read-back: the launch sequence does not wait for the host)
   static const int qmax[NXCLASS] = {8, 16, 24, PMAX};
   for (int k = 0; k < NXCLASS; k++) {
-    if (!h[k]) continue;
+    if (qmax[k] > a.p && k > 0 && qmax[k - 1] >= a.p) continue;  // no job has an order in this class
     XArgs b = a;
     b.ordH = ordH;
     b.ordF = ordF;
     b.cls = lists + (size_t)k * njobs;
-    launch_xform_q(c, h[k], b, std::min(qmax[k], a.p));
+    b.cnt = cnt + k;
+    launch_xform_q(c, std::min(njobs, XBIN_GRID), b, std::min(qmax[k], a.p));
   }
 }
 static void launch_p2l(pbsam_ctx *c, int njobs, PArgs a) {
@@ -1150,6 +1188,7 @@ struct GradState {
   DBuf<double> slot, slotF, slotH;         // [item][partner][3][p^2]
   DBuf<double> tGrot, LGF, LGH, LGH2;      // [item][3][p^2]
   DBuf<double> T, Tg, dev, devg, part;     // T: [item][partner][translation matrices]
+  DBuf<double> Es;                         // [item][p^2][6] scaled surface right-hand sides
   DBuf<XJob> xj, xj2, dxj;
   DBuf<TJob> tj, dtj;
   DBuf<RAJob> ra, ra2;
@@ -1158,6 +1197,8 @@ struct GradState {
   DBuf<MJob> mj;
   DBuf<double> mpart;
   DBuf<const double *> yptr;               // [sphere] harmonic rows of its exposed points
+  PinBuf<int> h_sb;                        // step lists staged for upload
+  PinBuf<double> h_dev;                    // per-step changes read back
 };
 
 struct pbsam_system {
@@ -1865,7 +1906,14 @@ int pbsam_ctx_create(double kappa, double eps_out, int device, pbsam_ctx **out) 
     HIPCHK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) fail(PBSAM_ERR_BADARG, "no such HIP device");
     HIPCHK(hipSetDevice(device));
+    // host waits spin rather than sleep: the Gauss-Seidel chains wait on short kernels
+    (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
+    (void)hipGetLastError();
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    c->stream0 = c->stream;
+    HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     int lds = 0;
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     c->lds_max = (size_t)lds;
@@ -1889,6 +1937,13 @@ void pbsam_ctx_destroy(pbsam_ctx *c) {
   c->d_work.release();
   c->d_qpart.release();
   c->d_xbin.release();
+  c->d_xbin2.release();
+  if (c->stream2) {
+    (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamDestroy(c->stream2);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -155,6 +155,7 @@ struct XArgs {
   unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
   int planes;          // consecutive source blocks / output slots per job (0 or 1: one)
   int split;           // 1: one plane per workgroup, plane = blockIdx.y (single-plane kernel)
+  const int *cnt;      // with cls: device count of the class; workgroups loop over it
 };
 
 // Orders of a batch of re-expansion jobs (the same rule as k_xform), per batch position,
@@ -212,22 +213,20 @@ __device__ __forceinline__ void sing_signs(int sing, int n, int m, double &sg, d
 
 // XT threads per workgroup: 128 for q <= 8, 256 for q <= 16, 512 above (each thread owns
 // up to two of the q(q+1)/2 back-rotated outputs)
+// one job: batch position e, plane offset pl (split launches)
 template <int XT, int NPL>
-__global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
+__device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e, int pl) {
   extern __shared__ double sh[];
   constexpr int QM = (XT == 128) ? 8 : ((XT == 256) ? 16 : PMAX);
   constexpr int NV = ((2 * QM - 2) * (2 * QM - 2) + XT - 1) / XT;  // staged recursion values per thread
-  const int e = A.cls ? A.cls[blockIdx.x] : blockIdx.x;  // position in the batch
   const int job = A.sel ? A.sel[e] : e;
   XJob J = A.jobs[job];
   const int p = A.p, P2 = p * p;
   const int tid = threadIdx.x;
   const bool intra = (J.kind == 1);
   const int tqsrc = J.src;  // totals index of the job's source sphere (its first plane)
-  if (NPL == 1 && A.split) {
-    J.src += blockIdx.y;
-    J.slot += blockIdx.y;
-  }
+  J.src += pl;
+  J.slot += pl;
   double rho, th, ph;
   cart_to_sph(J.P[0], J.P[1], J.P[2], rho, th, ph);
   int pH, pF = 0;
@@ -546,6 +545,18 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
         else { oF[n * n + 2 * s - 1] = accF[k][r].x; oF[n * n + 2 * s] = accF[k][r].y; }
       }
     }
+  }
+}
+template <int XT, int NPL>
+__global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
+  if (A.cnt) {  // a class of a binned batch: loop over its device-side count
+    const int n = *A.cnt;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+      __syncthreads();  // LDS of the previous job
+      xform_job<XT, NPL>(tb, A, A.cls[it], 0);
+    }
+  } else {
+    xform_job<XT, NPL>(tb, A, A.cls ? A.cls[blockIdx.x] : blockIdx.x, (NPL == 1 && A.split) ? blockIdx.y : 0);
   }
 }
 

@@ -6,7 +6,7 @@
 //                    (CRotCoeff::computeGradCoeff rotcoeff.cpp:818-850, dRotateT/P(+Sing)
 //                     :450-771, CTransCoeff dT :372-395) -- fused per pair
 //   k_solve3         CSolExpCenter::solveFH with 3 right-hand sides   expcenter.cpp:1080-1370
-//   k_surfgemv       computeExposedSurfaceGradientFH (stored harmonics) expcenter.cpp:1378-1412
+//   k_surfgemm       computeExposedSurfaceGradientFH (stored harmonics) expcenter.cpp:1378-1412
 //   k_p2lgemv        CXFormNIntra::xformGFH (stored maps)             xforms.cpp:514-582
 //   k_force          computeForceOn_0 + the per-sphere force terms    expcenter.cpp:124-149,
 //                                                                     moldynamics.cpp:525-550
@@ -923,6 +923,127 @@ __global__ void __launch_bounds__(MT) k_p2lgemv(MArgs A) {
     atomicAdd(&A.work[1], (unsigned long long)(8.0 * (nF + nH) * np));
   }
 }
+// The stored maps of a bundle of jobs that share one map (the same target and source
+// sphere of one molecule type, in different systems of a batch) as an FP64 matrix-core
+// product: out[c][3j + r] = sum_h M[h][c] q_j,r[h] for the F map (blockIdx.z even) or the H
+// map (odd).  Workgroup tile: 64 coefficients (one wavefront per 16) x 48 columns (16 jobs
+// x 3 planes); the charges of 64 points at a time are staged in LDS with the skip test
+// applied, the map is read straight from cache.  Every workgroup writes its rows of the
+// jobs' slots, zero at and above each job's own order squared.
+typedef double v4d __attribute__((ext_vector_type(4)));
+struct PGArgs {
+  const MJob *jobs;
+  const int *bent, *boff;    // job indices bundle by bundle, [bundle + 1] offsets
+  const double *qF, *qH;     // charges, 3 planes of `plane` doubles
+  long long plane;
+  const double *nF, *nH;     // squared norms for the skip test
+  const double *TQF, *TQH;
+  double *slotF, *slotH;
+  int p;
+  unsigned long long *work;
+};
+constexpr int PGJ = 16;   // jobs per workgroup (48 columns)
+constexpr int PGK = 64;   // points per staged chunk
+// grid (64-coefficient tile, job tile of the bundle, 2 * bundle + F/H); 256 threads
+__global__ void __launch_bounds__(256) k_p2lgemm(PGArgs A) {
+  __shared__ double Bs[PGK * 3 * PGJ];
+  __shared__ int job_s[PGJ], nq_s[PGJ], pto_s[PGJ];
+  __shared__ int maxnq_s;
+  const int p = A.p, P2 = p * p, tid = threadIdx.x, w = tid / 64, lane = tid % 64;
+  const int hf = blockIdx.z & 1, bund = blockIdx.z >> 1;
+  const int b0 = A.boff[bund], nb = A.boff[bund + 1] - b0;
+  const int j0 = blockIdx.y * PGJ;
+  if (j0 >= nb) return;  // uniform
+  const int nj = min(PGJ, nb - j0);
+  if (tid < PGJ) {
+    int q = 0, jb = -1, po = 0;
+    if (tid < nj) {
+      jb = A.bent[b0 + j0 + tid];
+      const MJob J = A.jobs[jb];
+      const int o = compute_order(J.rho, (hf ? A.TQH : A.TQF)[J.src], J.a_src, p);
+      q = o * o;
+      po = J.pt_off;
+    }
+    job_s[tid] = jb;
+    nq_s[tid] = q;
+    pto_s[tid] = po;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int m = 0;
+    for (int t = 0; t < nj; t++) m = max(m, nq_s[t]);
+    maxnq_s = m;
+  }
+  __syncthreads();
+  const MJob J0 = A.jobs[job_s[0]];
+  const int npts = J0.npts, c0 = blockIdx.x * 64;
+  double *slot = hf ? A.slotH : A.slotF;
+  const double *q = hf ? A.qH : A.qF;
+  const double *nrm = hf ? A.nH : A.nF;
+  v4d acc[3];
+#pragma unroll
+  for (int t = 0; t < 3; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  const int ca = c0 + w * 16 + lane % 16, kq = lane / 16;
+  if (c0 < maxnq_s) {  // uniform: rows at or above every job's order stay zero
+    const bool cok = ca < P2;
+    const double *ma = J0.M + (size_t)hf * npts * P2 + (cok ? ca : 0);
+    for (int h0 = 0; h0 < npts; h0 += PGK) {
+      __syncthreads();
+      // stage the chunk's charges: column col = 3 j + r, 64 points each (coalesced)
+      for (int i = tid; i < PGK * 3 * PGJ; i += 256) {
+        const int kk = i % PGK, col = i / PGK, jj = col / 3, r = col % 3, h = h0 + kk;
+        double v = 0.0;
+        if (jj < nj && h < npts) {
+          const size_t pt = (size_t)pto_s[jj] + h;
+          v = (nrm[pt] < 1e-15) ? 0.0 : q[(size_t)r * A.plane + pt];
+        }
+        Bs[kk * 3 * PGJ + col] = v;
+      }
+      __syncthreads();
+      const int kn = min(PGK, npts - h0);
+      if (kn == PGK) {
+        double av[PGK / 4];
+#pragma unroll
+        for (int u = 0; u < PGK / 4; u++) av[u] = cok ? ma[(size_t)(h0 + 4 * u + kq) * P2] : 0.0;
+#pragma unroll
+        for (int u = 0; u < PGK / 4; u++)
+#pragma unroll
+          for (int t = 0; t < 3; t++)
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], Bs[(4 * u + kq) * 3 * PGJ + t * 16 + lane % 16],
+                                                         acc[t], 0, 0, 0);
+      } else {
+        for (int kk = 0; kk < kn; kk += 4) {  // points past the end are zero in Bs
+          const bool hok = kk + kq < kn;
+          const double av = (cok && hok) ? ma[(size_t)(h0 + kk + kq) * P2] : 0.0;
+#pragma unroll
+          for (int t = 0; t < 3; t++)
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Bs[(kk + kq) * 3 * PGJ + t * 16 + lane % 16], acc[t],
+                                                         0, 0, 0);
+        }
+      }
+    }
+  }
+  // lane l, register r holds row l/16 + 4 r of the wavefront's 16, column l % 16 of tile t
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    const int col = t * 16 + lane % 16, jj = col / 3, rr = col % 3;
+    if (jj >= nj) continue;
+    const MJob J = A.jobs[job_s[jj]];
+    double *o = slot + (size_t)(J.slot + rr) * P2;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int c = c0 + w * 16 + lane / 16 + 4 * r;
+      if (c < P2) o[c] = (c < nq_s[jj]) ? acc[t][r] : 0.0;
+    }
+  }
+  if (A.work && tid == 0 && c0 < maxnq_s) {
+    double nsum = 0.0;
+    for (int t = 0; t < nj; t++) nsum += min(64, max(0, nq_s[t] - c0));
+    atomicAdd(&A.work[0], (unsigned long long)(2.0 * 3 * nsum * npts));
+    atomicAdd(&A.work[1], (unsigned long long)(8.0 * min(64, maxnq_s - c0) * npts));
+  }
+}
+
 // slotF/slotH[J.slot + r][c] = sum over splits of the partials, in split order
 // grid (job row, output chunks of 256)
 __global__ void __launch_bounds__(256) k_p2lsum(const MJob *jobs, const int *sel, int nsplit, int R, int p,
@@ -951,7 +1072,7 @@ __global__ void __launch_bounds__(256) k_p2lsum(const MJob *jobs, const int *sel
 // Exposed-surface gradient charges as a stored linear map.  The harmonics of each exposed
 // point about its own sphere centre are fixed by the molecule type; k_surfmat stores them
 // point-major (Y[h][c], one row per point, rows in the molecule type's point order), and
-// k_surfgemv forms the six charges of each point (F and H gradient along x, y, z) as
+// k_surfgemm forms the six charges of each point (F and H gradient along x, y, z) as
 //   q[h] = dA * sum_c Y[h][c] w_c E[c],  w_c = (2 if m > 0 else 1) * C_n,
 // C_n = (2n+1)/4pi for F and 1/(i^_n(ka)) (2n+1)/4pi for H (expcenter.cpp:1378-1412).
 // ----------------------------------------------------------------------------
@@ -980,86 +1101,161 @@ __global__ void __launch_bounds__(256) k_surfmat(Tabs tb, const double *xyz, lon
   for (int c = lane; c < P2; c += 64) Y[h * P2 + c] = Ys[w][c];
 }
 
-constexpr int SGR = 16;  // points (rows) per workgroup
+// ----------------------------------------------------------------------------
+// Exposed-surface gradient charges of a bundle of entries whose spheres share one molecule
+// type's stored harmonics Y (npts x p^2, point-major) as an FP64 matrix-core product
+//   C = Y E,  E[c][6t + b] = w_c,b * (gF_t | gH_t)[b][c],
+// w = (2 if m > 0 else 1) * C_n, C_n = (2n+1)/4pi (F) or cChgH[n] (H): the six charges of
+// every point and entry (expcenter.cpp:1378-1412).  k_surfE writes each entry's scaled E
+// ([c][6], per item); k_surfgemm: one wavefront per 16 points, NT 16-column tiles (entries
+// of 6 columns), both operands read straight from cache, K = p^2 in steps of 4.  The
+// epilogue stores the charges ([3][all points]), the squared norms of the F and H
+// 3-vectors, and per (entry, 64-point tile) the partial sums of |charge| in point order
+// (k_surfgrad_totals adds the tiles in order).
+// ----------------------------------------------------------------------------
+constexpr int SGM_ROWS = 64;  // points per workgroup (4 wavefronts x 16)
+
+// grid (entry of the list, chunk of 256 coefficients)
+__global__ void __launch_bounds__(256) k_surfE(const int *sel, const int *eitem, const int *esph,
+                                               const double *consts, const double *gF, const double *gH, double *E,
+                                               int p) {
+  const int e = sel[blockIdx.x], a = eitem[e], g = esph[e], P2 = p * p;
+  const int c = blockIdx.y * 256 + threadIdx.x;
+  if (c >= P2) return;
+  int n = (int)sqrtf((float)c);  // exact floor for c < 2^22 after the corrections
+  while (n * n > c) n--;
+  while ((n + 1) * (n + 1) <= c) n++;
+  const double two = (c == n * n) ? 1.0 : 2.0;
+  const double CF = (2 * n + 1) / (4.0 * M_PI), CH = consts[(size_t)g * 10 * p + 9 * p + n];
+  double *o = E + ((size_t)a * P2 + c) * 6;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    o[d] = two * (CF * gF[((size_t)e * 3 + d) * P2 + c]);
+    o[3 + d] = two * (CH * gH[((size_t)e * 3 + d) * P2 + c]);
+  }
+}
+
 struct SGArgs {
-  const int *sel;            // entries to run, one per grid row y
+  const int *bent;           // entries of the launch, bundle by bundle
+  const int *boff;           // [bundle + 1] offsets into bent
   const int *sph;            // [entry] global sphere
-  const int *pbuf;           // [entry] block of `part`
+  const int *pbuf;           // [entry] item: block of E and of `part`
   const double *const *Y;    // [global sphere] harmonic rows of its exposed points
   const int *pt_off, *pt_cnt, *nspx;  // [global sphere] first point, exposed and spiral counts
-  const double *consts;      // [sphere][10][p] (cChgH at row 9)
-  const double *gF, *gH;     // [entry][3][p^2]
+  const double *E;           // [item][p^2][6] scaled right-hand sides (k_surfE)
   double *gSF, *gSH;         // [3][all points]
   double *nF, *nH;
   long long npts_all;
-  double *part;              // [pbuf][chunk][6]
-  int nchunk;                // chunks per entry (grid x)
+  double *part;              // [item][chunk][6], chunk = 64-point tile
+  int nchunk;                // 64-point tiles per entry (grid x)
   int p;
   unsigned long long *work;
 };
-// grid (chunk of SGR points, entry)
-__global__ void __launch_bounds__(256) k_surfgemv(SGArgs A) {
-  __shared__ double E[6][PMAX * PMAX];
-  __shared__ double red[8];
+// grid (64-point tile, entry tile of the bundle, bundle); 256 threads; NT * 16 / 6 entries
+// per workgroup
+template <int NT>
+__global__ void __launch_bounds__(256) k_surfgemm(SGArgs A) {
+  constexpr int NC = NT * 16, NENT = NC / 6;
+  __shared__ double Cs[SGM_ROWS * NC];
+  __shared__ int ent_s[NENT];
+  __shared__ double dA_s[NENT];
   const int p = A.p, P2 = p * p, tid = threadIdx.x, w = tid / 64, lane = tid % 64;
-  const int ent = A.sel[blockIdx.y], g = A.sph[ent];
-  const int npts = A.pt_cnt[g];
-  double *part = A.part + ((size_t)A.pbuf[ent] * A.nchunk + blockIdx.x) * 6;
-  if ((int)blockIdx.x * SGR >= npts) {  // uniform: a chunk past the sphere's points
-    if (tid < 6) part[tid] = 0.0;
-    return;
-  }
-  const double *cChgH = A.consts + (size_t)g * 10 * p + 9 * p;
-  const double dA = 4 * M_PI / double(A.nspx[g]);
-  const long long pt_glob = A.pt_off[g];
-  const double *Yg = A.Y[g];
-  for (int e = tid; e < 6 * P2; e += 256) {
-    const int b = e / P2, c = e % P2;
-    int n = (int)sqrtf((float)c);  // exact floor for c < 2^22 after the corrections
-    while (n * n > c) n--;
-    while ((n + 1) * (n + 1) <= c) n++;
-    const double C = (b < 3) ? (2 * n + 1) / (4.0 * M_PI) : cChgH[n];
-    const double v = (b < 3) ? A.gF[((size_t)ent * 3 + b) * P2 + c] : A.gH[((size_t)ent * 3 + b - 3) * P2 + c];
-    E[b][c] = (c == n * n ? 1.0 : 2.0) * (C * v);
+  const int b0 = A.boff[blockIdx.z], nb = A.boff[blockIdx.z + 1] - b0;
+  const int t0 = blockIdx.y * NENT;
+  if (t0 >= nb) return;  // uniform
+  const int nent = min(NENT, nb - t0);
+  const int g0 = A.sph[A.bent[b0 + t0]];
+  const int npts = A.pt_cnt[g0];
+  const int h0 = blockIdx.x * SGM_ROWS;
+  if (tid < NENT) {
+    const int e = (tid < nent) ? A.bent[b0 + t0 + tid] : -1;
+    ent_s[tid] = e;
+    dA_s[tid] = (e >= 0) ? 4 * M_PI / double(A.nspx[A.sph[e]]) : 0.0;
   }
   __syncthreads();
-  double t[6] = {0, 0, 0, 0, 0, 0};
-  for (int rr = 0; rr < SGR / 4; rr++) {
-    const int h = blockIdx.x * SGR + w * (SGR / 4) + rr;
-    if (h >= npts) break;  // rows left to this wavefront are all past the end
-    const double *row = Yg + (size_t)h * P2;
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    for (int c = lane; c < P2; c += 64) {
-      const double y = row[c];
-      for (int b = 0; b < 6; b++) s[b] += y * E[b][c];
+  if (h0 >= npts) {  // a tile past this sphere's points: zero partials
+    if (tid < nent * 6) A.part[((size_t)A.pbuf[ent_s[tid / 6]] * A.nchunk + blockIdx.x) * 6 + tid % 6] = 0.0;
+    return;
+  }
+  const int nrow = min(SGM_ROWS, npts - h0);
+  // A operand: lane l supplies Y[row l%16][k + l/16]; B operand: E[k + l/16][column l%16]
+  const int ra = w * 16 + lane % 16, kq = lane / 16;
+  const bool rok = ra < nrow;
+  const double *ya = A.Y[g0] + (size_t)(h0 + (rok ? ra : 0)) * P2 + kq;
+  const double *eb[NT];
+  bool cok[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int col = t * 16 + lane % 16, es = col / 6;
+    cok[t] = es < nent;
+    eb[t] = A.E + ((size_t)A.pbuf[ent_s[cok[t] ? es : 0]] * P2 + kq) * 6 + col % 6;
+  }
+  v4d acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  constexpr int UNR = (NT == 1) ? 16 : 8;  // k-steps per iteration: their loads are issued first
+  const int K4 = P2 & ~3;
+  int k = 0;
+  for (; k + 4 * UNR <= K4; k += 4 * UNR) {
+    double av[UNR], bv[UNR][NT];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      av[u] = rok ? ya[k + 4 * u] : 0.0;
+#pragma unroll
+      for (int t = 0; t < NT; t++) bv[u][t] = cok[t] ? eb[t][(size_t)(k + 4 * u) * 6] : 0.0;
     }
-    for (int b = 0; b < 6; b++) s[b] = wave_sum(s[b]);
-    if (lane == 0) {
-      const size_t pt = (size_t)pt_glob + h;
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u][t], acc[t], 0, 0, 0);
+  }
+  for (; k < P2; k += 4) {  // remaining steps; k + l/16 past p^2 reads as zero
+    const bool kok = k + kq < P2;
+    const double av = (rok && kok) ? ya[k] : 0.0;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const double bv = (cok[t] && kok) ? eb[t][(size_t)k * 6] : 0.0;
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+    }
+  }
+  // C tile to LDS: lane l, register r holds row l/16 + 4 r, column l % 16 of its tile
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) Cs[(w * 16 + lane / 16 + 4 * r) * NC + t * 16 + lane % 16] = acc[t][r];
+  __syncthreads();
+  // epilogue: wavefront w handles entries w, w + 4, ...; lane = point
+  for (int t = w; t < nent; t += 4) {  // uniform per wavefront
+    const int e = ent_s[t], g = A.sph[e];
+    const double dA = dA_s[t];
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    if (lane < nrow) {
+      const size_t pt = (size_t)A.pt_off[g] + h0 + lane;
       double nf = 0.0, nh = 0.0;
+#pragma unroll
       for (int d = 0; d < 3; d++) {
-        const double gf = s[d] * dA, gh = s[3 + d] * dA;
+        const double gf = Cs[lane * NC + 6 * t + d] * dA, gh = Cs[lane * NC + 6 * t + 3 + d] * dA;
         A.gSF[(size_t)d * A.npts_all + pt] = gf;
         A.gSH[(size_t)d * A.npts_all + pt] = gh;
-        t[d] += fabs(gf);
-        t[3 + d] += fabs(gh);
+        q[d] = fabs(gf);
+        q[3 + d] = fabs(gh);
         nf += gf * gf;
         nh += gh * gh;
       }
       A.nF[pt] = nf;
       A.nH[pt] = nh;
     }
+#pragma unroll
+    for (int b = 0; b < 6; b++) q[b] = wave_sum(q[b]);
+    if (lane == 0) {
+      double *pp = A.part + ((size_t)A.pbuf[e] * A.nchunk + blockIdx.x) * 6;
+#pragma unroll
+      for (int b = 0; b < 6; b++) pp[b] = q[b];
+    }
   }
-  // per-workgroup totals over its rows, in row order within each wavefront
-  for (int b = 0; b < 6; b++) {
-    __syncthreads();
-    if (lane == 0) red[w] = t[b];
-    __syncthreads();
-    if (tid == 0) part[b] = red[0] + red[1] + red[2] + red[3];
-  }
-  if (A.work && tid == 0 && blockIdx.x == 0) {
-    atomicAdd(&A.work[0], (unsigned long long)(12.0 * P2 * npts));
-    atomicAdd(&A.work[1], (unsigned long long)(8.0 * P2 * npts + 64.0 * npts));
+  if (A.work && tid == 0) {
+    atomicAdd(&A.work[0], (unsigned long long)(12.0 * P2 * nrow * nent));
+    atomicAdd(&A.work[1], (unsigned long long)(8.0 * P2 * nrow + 64.0 * nrow * nent));
   }
 }
 

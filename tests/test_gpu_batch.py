@@ -112,3 +112,36 @@ def test_batch_equals_single_systems(gpu, orc, n_sph, p, npts):
     assert rco == 0
     assert relerr(fb[0:2], fo) < 1e-8
     assert relerr(tb[0:2], to) < 1e-8
+
+
+@pytest.mark.parametrize("n_sph,p,npts", [(6, 10, 400), (5, 30, 1800)])
+def test_batch_of_snapshots(gpu, orc, n_sph, p, npts):
+    """Six poses of the same two molecule types: every step's stored maps and surface
+    harmonics are shared by all groups (the bundled matrix-core products)."""
+    import pbsam_amd
+    import pyoracle
+    from pbsam_amd import synth
+    mols, gs, os_ = moltypes(gpu, orc, n_sph, p, npts, seeds=(4, 5))
+    d0 = 2 * synth.ball_radius(n_sph) + 4.0
+    poses = [synth.place_pair(mols[0], mols[1], d0 + 1.5 * b, seed=40 + b) for b in range(6)]
+    pos = np.vstack([pp for pp, _ in poses])
+    quat = np.vstack([qq for _, qq in poses])
+    batch = pbsam_amd.System(gpu, [gs[0], gs[1]] * 6, pos, quat, groups=[2] * 6)
+    batch.set_cutoffs(100.0, 100.0)
+    _, rc, fb, tb = batch.compute_forces(tol=0, max_rounds=0)
+    assert rc == 0
+    st, br, bp = batch.status()
+    for b, (pp, qq) in enumerate(poses):
+        single = pbsam_amd.System(gpu, [gs[0], gs[1]], pp, qq)
+        single.set_cutoffs(100.0, 100.0)
+        _, rc1, f1, t1 = single.compute_forces(tol=0, max_rounds=0)
+        _, sr, sp = single.status()
+        assert rc1 == 0 and (br[b], bp[b]) == (sr[0], sp[0]), b
+        assert relerr(fb[2 * b:2 * b + 2], f1) < 1e-10, (b, fb[2 * b:2 * b + 2], f1)
+        assert relerr(tb[2 * b:2 * b + 2], t1) < 1e-10, b
+    so = pyoracle.System(orc, [os_[0], os_[1]], poses[3][0], poses[3][1])
+    so.set_cutoffs(100.0, 100.0)
+    rco, fo, to = so.compute_forces(0, 0.0, 0)
+    assert rco == 0
+    assert relerr(fb[6:8], fo) < 1e-8
+    assert relerr(tb[6:8], to) < 1e-8
