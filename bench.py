@@ -167,6 +167,26 @@ def cpu_baseline(kappa, mols, mts, pos, quat, gpu_force, gpu_torque, budget_s):
                 max_rel_err_force_device_vs_oracle=err_f, max_rel_err_torque_device_vs_oracle=err_t)
 
 
+def copy_calibration(mib):
+    """One device-to-device hipMemcpy of `mib` MiB (the HIP runtime's copy kernel), for
+    scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters: the copy reads and writes exactly
+    that many bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for buf in (a, b):
+        if hip.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "rc": rc}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,7 +198,13 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="systems in the batched leg (0: skip it)")
     ap.add_argument("--batch-reps", type=int, default=2, help="timed batched calls")
     ap.add_argument("--plain-sample", type=int, default=8, help="systems timed one by one for the ratio")
+    ap.add_argument("--solves", type=int, default=20, help="mutual solves timed for solves_per_s")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the solve of one system runs on one device; "
@@ -217,12 +243,14 @@ def main():
     print("timed %d force+torque evaluations in %.3f s (mutual rounds %d)" % (args.steps, elapsed, rounds),
           file=sys.stderr, flush=True)
     # the solve alone (outside the timed region above)
-    nsol = 20
-    t0 = time.perf_counter()
-    for _ in range(nsol):
-        system.solve(tol=0, max_rounds=0)
-    ctx.sync()
-    solves_per_s = nsol / (time.perf_counter() - t0)
+    nsol = args.solves
+    solves_per_s = None
+    if nsol > 0:
+        t0 = time.perf_counter()
+        for _ in range(nsol):
+            system.solve(tol=0, max_rounds=0)
+        ctx.sync()
+        solves_per_s = nsol / (time.perf_counter() - t0)
 
     dom = max(prof, key=lambda k: prof[k]["ms"])
     d = prof[dom]

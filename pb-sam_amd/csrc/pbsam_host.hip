@@ -358,10 +358,10 @@ static void launch_xform_q(pbsam_ctx *c, int n, XArgs a, int qmax) {
 }
 // A batch of re-expansions.  Large batches with adaptive orders are first split by order
 // class on the device (k_xorder) so that low-order pairs run in small workgroups with
-// little LDS; each class launch loops over its device-side count, so nothing is read back.
+// little LDS; each class launch covers the whole batch and its workgroups past the class's
+// device-side count leave at once, so nothing is read back.
 // Results do not depend on the split.
 static constexpr int XBIN_MIN = 512;
-static constexpr int XBIN_GRID = 2048;  // workgroups of a class launch
 static void launch_xform(pbsam_ctx *c, int njobs, XArgs a) {
   if (!njobs) return;
   if (a.ordH || a.full || njobs < XBIN_MIN) {
@@ -389,7 +389,7 @@ static void launch_xform(pbsam_ctx *c, int njobs, XArgs a) {
     b.ordF = ordF;
     b.cls = lists + (size_t)k * njobs;
     b.cnt = cnt + k;
-    launch_xform_q(c, std::min(njobs, XBIN_GRID), b, std::min(qmax[k], a.p));
+    launch_xform_q(c, njobs, b, std::min(qmax[k], a.p));
   }
 }
 static void launch_p2l(pbsam_ctx *c, int njobs, PArgs a) {

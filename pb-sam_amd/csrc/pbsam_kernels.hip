@@ -155,7 +155,7 @@ struct XArgs {
   unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
   int planes;          // consecutive source blocks / output slots per job (0 or 1: one)
   int split;           // 1: one plane per workgroup, plane = blockIdx.y (single-plane kernel)
-  const int *cnt;      // with cls: device count of the class; workgroups loop over it
+  const int *cnt;      // with cls: device count of the class (workgroups past it leave)
 };
 
 // Orders of a batch of re-expansion jobs (the same rule as k_xform), per batch position,
@@ -219,6 +219,7 @@ __device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e,
   extern __shared__ double sh[];
   constexpr int QM = (XT == 128) ? 8 : ((XT == 256) ? 16 : PMAX);
   constexpr int NV = ((2 * QM - 2) * (2 * QM - 2) + XT - 1) / XT;  // staged recursion values per thread
+  constexpr int NO = (QM * (QM + 1) / 2 + XT - 1) / XT;            // back-rotated outputs per thread
   const int job = A.sel ? A.sel[e] : e;
   XJob J = A.jobs[job];
   const int p = A.p, P2 = p * p;
@@ -284,10 +285,10 @@ __device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e,
   }
   // back-rotated accumulators: each thread owns up to two (n,s) outputs
   const int nout = q * (q + 1) / 2;
-  int on[2], os[2];
-  cd accH[2][NPL], accF[2][NPL];
+  int on[NO], os[NO];
+  cd accH[NO][NPL], accF[NO][NPL];
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < NO; k++) {
     int ek = tid + k * XT;
     on[k] = -1;
     os[k] = 0;
@@ -341,7 +342,7 @@ __device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e,
       }
       __syncthreads();
 #pragma unroll
-      for (int k = 0; k < 2; k++) {
+      for (int k = 0; k < NO; k++) {
         const int n = on[k], s = os[k];
         if (n < 0 || s != m) continue;
         double sg, sgi;
@@ -461,7 +462,7 @@ __device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e,
       __syncthreads();
       // (c) back-rotation contribution of level m (rotcoeff.cpp:420-438)
 #pragma unroll
-      for (int k = 0; k < 2; k++) {
+      for (int k = 0; k < NO; k++) {
         const int n = on[k], s = os[k];
         if (n < m) continue;
         const cd *row = R + n * n + n;
@@ -529,7 +530,7 @@ __device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e,
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < NO; k++) {
     const int n = on[k], s = os[k];
     if (n < 0) continue;
 #pragma unroll
@@ -549,15 +550,11 @@ __device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e,
 }
 template <int XT, int NPL>
 __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
-  if (A.cnt) {  // a class of a binned batch: loop over its device-side count
-    const int n = *A.cnt;
-    for (int it = blockIdx.x; it < n; it += gridDim.x) {
-      __syncthreads();  // LDS of the previous job
-      xform_job<XT, NPL>(tb, A, A.cls[it], 0);
-    }
-  } else {
-    xform_job<XT, NPL>(tb, A, A.cls ? A.cls[blockIdx.x] : blockIdx.x, (NPL == 1 && A.split) ? blockIdx.y : 0);
-  }
+  // one job per workgroup; a class of a binned batch is launched for the whole batch and
+  // its workgroups past the class's device-side count leave at once
+  if (A.cnt && (int)blockIdx.x >= *A.cnt) return;
+  const int e = A.cls ? A.cls[blockIdx.x] : blockIdx.x;
+  xform_job<XT, NPL>(tb, A, e, (NPL == 1 && A.split) ? blockIdx.y : 0);
 }
 
 

@@ -6,6 +6,12 @@ Prints a Markdown table of per-kernel dispatch statistics over the whole trace a
 the marker trace holds bench.py's "timed_region" range, over the dispatches that ran
 inside it; with bench.py's JSON line it also lists bench.py's own HIP-event averages of
 the instrumented kernels beside rocprofv3's.
+
+    python -m pbsam_amd.rocprof pmc <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv> <copy_bytes>
+
+Per-kernel HBM traffic from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` counter
+collections (KiB per dispatch), each scaled by what the same counter reads for one
+device-to-device copy of copy_bytes (bench.py --copy-calibration).
 """
 import csv
 import json
@@ -83,5 +89,49 @@ def main(argv):
                                                  k["launches"], ev))
 
 
+def load_counters(path, counter):
+    """{kernel: [value per dispatch]} of one counter (KiB) from a counter-collection CSV"""
+    per = defaultdict(float)
+    names = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r.get("Counter_Name") != counter:
+                continue
+            d = r.get("Dispatch_Id") or r.get("Correlation_Id")
+            per[d] += float(r["Counter_Value"])
+            names[d] = short(r["Kernel_Name"])
+    out = defaultdict(list)
+    for d, v in per.items():
+        out[names[d]].append(v)
+    return out
+
+
+def pmc(argv):
+    fetch, write = load_counters(argv[0], "FETCH_SIZE"), load_counters(argv[1], "WRITE_SIZE")
+    cf, cw = load_counters(argv[2], "FETCH_SIZE"), load_counters(argv[3], "WRITE_SIZE")
+    nbytes = float(argv[4])
+    copy = [k for k in cf if "copy" in k.lower()]
+    if not copy:
+        print("no copy kernel in the calibration run")
+        return
+    kf, kw = max(cf[copy[0]]), max(cw[copy[0]])
+    sf, sw = nbytes / (kf * 1024.0), nbytes / (kw * 1024.0)
+    print("Calibration: %s of %d bytes reads FETCH_SIZE %.1f KiB, WRITE_SIZE %.1f KiB; scale %.4f (fetch), %.4f "
+          "(write)" % (copy[0], nbytes, kf, kw, sf, sw))
+    print()
+    print("| kernel | dispatches | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch | HBM bytes/dispatch (scaled) |")
+    print("|---|---|---|---|---|")
+    rows = []
+    for k in fetch:
+        f_ = sum(fetch[k]) / len(fetch[k])
+        w_ = sum(write.get(k, [0.0])) / max(len(write.get(k, [])), 1)
+        rows.append((f_ * 1024 * sf + w_ * 1024 * sw, k, len(fetch[k]), f_, w_))
+    for b, k, n, f_, w_ in sorted(rows, reverse=True):
+        print("| %s | %d | %.1f | %.1f | %.0f |" % (k, n, f_, w_, b))
+
+
 if __name__ == "__main__":
-    main(sys.argv)
+    if len(sys.argv) > 1 and sys.argv[1] == "pmc":
+        pmc(sys.argv[2:])
+    else:
+        main(sys.argv)
