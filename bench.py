@@ -38,6 +38,12 @@ P_ORD = P
 SEEDS, POSE_SEED, BATCH_SEED = (101, 102), 7, 5000
 PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix), flop/s
 PEAK_HBM = 8.0e12        # MI355X HBM3E, B/s
+# HBM bytes per launch of each instrumented kernel group at this workload: two rocprofv3
+# counter passes over one evaluation (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, the
+# dispatches from the system's pair lists on), each scaled by the same counter for one
+# 256 MiB device-to-device copy (FETCH_SIZE reads half the bytes: x2.00; WRITE_SIZE x1.00),
+# averaged over the group's dispatches in one evaluation.
+TRAFFIC = {"solve": 185587, "xform": 2402283, "surfgrad": 13526132, "p2l": 36640290}
 
 
 def roctx():
@@ -89,11 +95,16 @@ def batch_leg(pbsam_amd, synth, ctx, mols, mts, nsys, reps, nplain):
     batch.compute_forces(tol=0, max_rounds=0)
     ctx.sync()
     ctx.profile(True)
+    mark = roctx()
+    if mark:
+        mark.roctxRangePushA(b"batch_region")
     t0 = time.perf_counter()
     for _ in range(reps):
         _, rc, fb, tb = batch.compute_forces(tol=0, max_rounds=0)
     ctx.sync()
     el = time.perf_counter() - t0
+    if mark:
+        mark.roctxRangePop()
     prof = {k: ctx.profile_read(k) for k in pbsam_amd.KERNELS}
     ctx.profile(False)
     st, rounds, passes = batch.status()
@@ -265,7 +276,8 @@ def main():
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s", frac=achieved / PEAK_HBM)
     roof.update(kernel=dom, launches=d["launches"], avg_launch_us=avg_s * 1e6,
                 algorithmic_per_launch=(d["flops"] if compute_bound else d["bytes"]) / max(d["launches"], 1),
-                traffic=None)
+                traffic=TRAFFIC.get(dom), traffic_unit="HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                                        "copy-calibrated)")
     kernels = {k: dict(launches=v["launches"], ms_per_step=v["ms"] / args.steps,
                        gflop_per_step=v["flops"] / args.steps / 1e9) for k, v in prof.items()}
 

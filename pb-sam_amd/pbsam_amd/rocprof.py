@@ -3,15 +3,18 @@
     python -m pbsam_amd.rocprof <run_kernel_trace.csv> [run_marker_api_trace.csv [bench.json]]
 
 Prints a Markdown table of per-kernel dispatch statistics over the whole trace and, when
-the marker trace holds bench.py's "timed_region" range, over the dispatches that ran
-inside it; with bench.py's JSON line it also lists bench.py's own HIP-event averages of
+the marker trace holds bench.py's "timed_region" (and "batch_region") range, over the
+dispatches that ran inside it; with bench.py's JSON line it also lists bench.py's own HIP-event averages of
 the instrumented kernels beside rocprofv3's.
 
     python -m pbsam_amd.rocprof pmc <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv> <copy_bytes>
+        [--after KERNEL]
 
 Per-kernel HBM traffic from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` counter
 collections (KiB per dispatch), each scaled by what the same counter reads for one
-device-to-device copy of copy_bytes (bench.py --copy-calibration).
+device-to-device copy of copy_bytes (bench.py --copy-calibration); with --after, only the
+dispatches from the first dispatch of KERNEL on (k_rotcoeff: bench.py's evaluation, past
+the molecule types' set-up).
 """
 import csv
 import json
@@ -66,6 +69,10 @@ def main(argv):
     table(rows, "All dispatches of the run")
     if len(argv) < 3:
         return
+    brng = find_range(argv[2], "batch_region")
+    if brng is not None:
+        inb = [r for r in rows if r[1] >= brng[0] and r[2] <= brng[1]]
+        table(inb, "Dispatches inside bench.py's batched leg (%.3f ms of wall time)" % ((brng[1] - brng[0]) / 1e6))
     rng = find_range(argv[2], "timed_region")
     if rng is None:
         print("no timed_region range in the marker trace")
@@ -89,25 +96,33 @@ def main(argv):
                                                  k["launches"], ev))
 
 
-def load_counters(path, counter):
-    """{kernel: [value per dispatch]} of one counter (KiB) from a counter-collection CSV"""
+def load_counters(path, counter, after=None):
+    """{kernel: [value per dispatch]} of one counter (KiB) from a counter-collection CSV;
+    with `after`, only dispatches from the first dispatch of that kernel on"""
     per = defaultdict(float)
     names = {}
     with open(path) as f:
         for r in csv.DictReader(f):
             if r.get("Counter_Name") != counter:
                 continue
-            d = r.get("Dispatch_Id") or r.get("Correlation_Id")
+            d = int(r.get("Dispatch_Id") or r.get("Correlation_Id"))
             per[d] += float(r["Counter_Value"])
             names[d] = short(r["Kernel_Name"])
+    first = min((d for d, n in names.items() if n == after), default=None) if after else None
     out = defaultdict(list)
     for d, v in per.items():
-        out[names[d]].append(v)
+        if first is None or d >= first:
+            out[names[d]].append(v)
     return out
 
 
 def pmc(argv):
-    fetch, write = load_counters(argv[0], "FETCH_SIZE"), load_counters(argv[1], "WRITE_SIZE")
+    after = None
+    if "--after" in argv:
+        i = argv.index("--after")
+        after = argv[i + 1]
+        argv = argv[:i] + argv[i + 2:]
+    fetch, write = load_counters(argv[0], "FETCH_SIZE", after), load_counters(argv[1], "WRITE_SIZE", after)
     cf, cw = load_counters(argv[2], "FETCH_SIZE"), load_counters(argv[3], "WRITE_SIZE")
     nbytes = float(argv[4])
     copy = [k for k in cf if "copy" in k.lower()]
