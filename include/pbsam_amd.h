@@ -109,7 +109,8 @@ void pbsam_system_destroy(pbsam_system *sys);
  * systems 0..g-1 in mts/pos/quat, and interacts only with itself.  Every call on a system
  * handle works on a batch with molecule indices running over all groups; each group
  * follows its own reference schedule exactly (its results equal those of a system made
- * of that group alone), while the groups' Gauss-Seidel chains share kernel launches.
+ * of that group alone up to the rounding of different launch shapes), while the groups'
+ * Gauss-Seidel chains share kernel launches.
  * The batched counterpart of calling CMolecule::computeForces (src/moldynamics.cpp:554-587)
  * once per independent system; there is no single reference entry point for it. */
 int pbsam_batch_create(pbsam_ctx *ctx, int n_sys, const int *n_mol, pbsam_moltype *const *mts,

@@ -183,9 +183,9 @@ __host__ __device__ inline double work_surface_point(int p) { return 7.0 * p * (
 
 // offset of translation matrix of level m (dense (p-m)^2) in a pair's block
 __host__ __device__ __forceinline__ long long tmat_off(int p, int m) {
-  long long o = 0;
-  for (int k = 0; k < m; k++) o += (long long)(p - k) * (p - k);
-  return o;
+  // sum_{k<m} (p-k)^2
+  const long long P = p, M = m;
+  return M * P * P - P * M * (M - 1) + (M - 1) * M * (2 * M - 1) / 6;
 }
 __host__ __device__ __forceinline__ long long tmat_size(int p) { return tmat_off(p, p); }
 // stored rotation coefficients: row n holds (n+1) x (2n+1) complex, index m*(2n+1)+(s+n)

@@ -325,8 +325,8 @@ static void launch_xform_qp(pbsam_ctx *c, int n, const XArgs &a, int qmax) {
   const size_t lds = xform_lds(qmax, NPL, a.F != nullptr);
   const dim3 grid(n, (NPL == 1 && a.split) ? a.planes : 1);
   if (qmax <= 8) {
-    set_lds((const void *)k_xform<128, NPL>, lds);
-    hipLaunchKernelGGL((k_xform<128, NPL>), grid, dim3(128), lds, c->stream, c->tabs, a);
+    set_lds((const void *)k_xform<64, NPL>, lds);
+    hipLaunchKernelGGL((k_xform<64, NPL>), grid, dim3(64), lds, c->stream, c->tabs, a);
   } else if (qmax <= 16) {
     set_lds((const void *)k_xform<256, NPL>, lds);
     hipLaunchKernelGGL((k_xform<256, NPL>), grid, dim3(256), lds, c->stream, c->tabs, a);

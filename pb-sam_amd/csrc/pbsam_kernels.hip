@@ -211,13 +211,13 @@ __device__ __forceinline__ void sing_signs(int sing, int n, int m, double &sg, d
   }
 }
 
-// XT threads per workgroup: 128 for q <= 8, 256 for q <= 16, 512 above (each thread owns
-// up to two of the q(q+1)/2 back-rotated outputs)
+// XT threads per workgroup: 64 (one wavefront) for q <= 8, 256 for q <= 16, 512 above
+// (each thread owns at most NO of the q(q+1)/2 back-rotated outputs)
 // one job: batch position e, plane offset pl (split launches)
 template <int XT, int NPL>
 __device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e, int pl) {
   extern __shared__ double sh[];
-  constexpr int QM = (XT == 128) ? 8 : ((XT == 256) ? 16 : PMAX);
+  constexpr int QM = (XT <= 128) ? 8 : ((XT == 256) ? 16 : PMAX);
   constexpr int NV = ((2 * QM - 2) * (2 * QM - 2) + XT - 1) / XT;  // staged recursion values per thread
   constexpr int NO = (QM * (QM + 1) / 2 + XT - 1) / XT;            // back-rotated outputs per thread
   const int job = A.sel ? A.sel[e] : e;
@@ -295,7 +295,7 @@ __device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e,
 #pragma unroll
     for (int r = 0; r < NPL; r++) accH[k][r] = accF[k][r] = {0.0, 0.0};
     if (ek < nout) {
-      int n = (int)((sqrt(8.0 * ek + 1.0) - 1.0) * 0.5);
+      int n = (int)((sqrtf(8.0f * ek + 1.0f) - 1.0f) * 0.5f);
       while (n * (n + 1) / 2 > ek) n--;
       while ((n + 1) * (n + 2) / 2 <= ek) n++;
       on[k] = n;
@@ -494,7 +494,7 @@ __device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e,
           const int ee = tid + t0 * XT;
           if (ee >= total) break;
           int t = ee + base;
-          int n = (int)sqrt((double)t);
+          int n = (int)sqrtf((float)t);  // exact floor for t < 2^22 after the corrections
           while (n * n > t) n--;
           while ((n + 1) * (n + 1) <= t) n++;
           n += 1;  // element t belongs to row n: start (n-1)^2
