@@ -1186,7 +1186,7 @@ struct GradState {
   DBuf<double> maxgF, maxgH;               // [sphere]
   DBuf<double> gLHN, gH;                   // [sphere][3][p^2]
   DBuf<double> slot, slotF, slotH;         // [item][partner][3][p^2]
-  DBuf<double> tGrot, LGF, LGH, LGH2;      // [item][3][p^2]
+  DBuf<double> tGrot, LGF, LGH;            // [item][3][p^2]
   DBuf<double> T, Tg, dev, devg, part;     // T: [item][partner][translation matrices]
   DBuf<double> Es;                         // [item][p^2][6] scaled surface right-hand sides
   DBuf<XJob> xj, xj2, dxj;

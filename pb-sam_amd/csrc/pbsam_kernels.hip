@@ -9,7 +9,8 @@
 //                  expansion.cpp:203-340
 //   k_rotcoeff     CRotCoeff::reset(CQuat) rotcoeff.cpp:181-201 (coefficients stored)
 //   k_rotapply     CRotCoeff::rotateWithXi rotcoeff.cpp:250-350
-//   k_gather       LS / LHS / LFS assembly molecule.cpp:1214-1297,1414-1473
+//   k_sumslots     LS / LHS / LFS assembly molecule.cpp:1214-1297,1414-1473
+//   k_xorder       CXFormBase::computeOrder per pair xforms.cpp:402-414 (order classes)
 //   k_solve        CSolExpCenter::solveFH expcenter.cpp:1080-1370
 //   k_surface      computeExposedSurfaceChargesFH expcenter.cpp:1420-1440
 #include "pbsam_device.h"

@@ -463,6 +463,8 @@ struct S3Args {
   unsigned long long *work;
   const int *sel;          // entries of sph/ent to run (null: the workgroup index)
   const int *lfi;          // right-hand-side block of each entry (null: the workgroup index)
+  double *E;               // [block][p^2][6]: the scaled surface right-hand sides of the
+                           // result (k_surfgemm's operand), or null
 };
 
 // three right-hand sides: WPC wavefronts per column, column d = wave / WPC
@@ -503,12 +505,22 @@ __device__ void solve3_identity(const S3Args &A, int g, int e, int lb, double *r
   id_dev2(S, td, ts);
   for (int k = 0; k < KW; k++) {
     if (S.nn[k] < 0) continue;
-    const int i = S.i0[k];
+    const int i = S.i0[k], n = S.nn[k];
     A.F[o + i] = S.Fr[k];
     A.H[o + i] = S.Hr[k];
-    if (i != S.nn[k] * S.nn[k]) {
+    if (i != n * n) {
       A.F[o + i + 1] = S.Fi[k];
       A.H[o + i + 1] = S.Hi[k];
+    }
+    if (A.E) {  // w (2n+1)/4pi F and w cChgH[n] H, w = 2 for m > 0
+      const double two = (i == n * n) ? 1.0 : 2.0, CF = (2 * n + 1) / (4.0 * M_PI), CH = cs[9 * p + n];
+      double *Eo = A.E + ((size_t)lb * P2 + i) * 6;
+      Eo[d] = two * (CF * S.Fr[k]);
+      Eo[3 + d] = two * (CH * S.Hr[k]);
+      if (i != n * n) {
+        Eo[6 + d] = two * (CF * S.Fi[k]);
+        Eo[6 + 3 + d] = two * (CH * S.Hi[k]);
+      }
     }
   }
   td = wave_sum(td);
@@ -635,6 +647,13 @@ __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
     ts += w * (a * a + b * b);
     A.F[o + i] = F[i];
     A.H[o + i] = H[i];
+    if (A.E) {
+      const int d = i / P2, c = i % P2, n = nn[k];
+      const double CF = (2 * n + 1) / (4.0 * M_PI), CH = cs[9 * p + n];
+      double *Eo = A.E + ((size_t)lb * P2 + c) * 6;
+      Eo[d] = w * (CF * F[i]);
+      Eo[3 + d] = w * (CH * H[i]);
+    }
   }
   td = block_sum(td, red);
   ts = block_sum(ts, red);
@@ -652,8 +671,10 @@ __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
 // directions of sum |charge|, expcenter.cpp:1378-1412) from k_surfgemv's per-workgroup
 // partials: one wavefront per entry; lane l sums chunks l, l+64, ..., then the lanes are added.
 __global__ void __launch_bounds__(64) k_surfgrad_totals(const int *sel, const int *sph, const int *pbuf, int nchunk,
-                                                        const double *part, double *maxgF, double *maxgH) {
+                                                        const double *part, double *maxgF, double *maxgH,
+                                                        const double *dev, double *devg) {
   const int e = sel[blockIdx.x];
+  if (devg && threadIdx.x == 0) devg[blockIdx.x] = dev[e];  // the entry's change, gathered
   const double *pp = part + (size_t)pbuf[e] * nchunk * 6;
   double t[6] = {0, 0, 0, 0, 0, 0};
   for (int c = threadIdx.x; c < nchunk; c += 64)
@@ -668,11 +689,40 @@ __global__ void __launch_bounds__(64) k_surfgrad_totals(const int *sel, const in
   maxgH[sph[e]] = mh;
 }
 
-// dst[i] = src[sel[i]]
-__global__ void __launch_bounds__(256) k_gather(const int *sel, int n, const double *src, double *dst) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) dst[i] = src[sel[i]];
+// The intra-molecular sums of a gradient entry in one pass: LGF = sum of its F slots and
+// LGH = LGH + (sum of its H slots), both in list order; grid (entry, chunk of 256)
+__global__ void __launch_bounds__(256) k_sumfh(const int *sel, const int *obs, const int *lst_off, const int *lst,
+                                               const double *slotF, const double *slotH, double *LGF, double *LGH,
+                                               int bs) {
+  const int e0 = sel[blockIdx.x], go = obs[e0];
+  const int l0 = lst_off[e0], l1 = lst_off[e0 + 1];
+  for (int i = blockIdx.y * 256 + threadIdx.x; i < bs; i += 256 * gridDim.y) {
+    double sF = 0.0, sH = 0.0;
+    int e = l0;
+    for (; e + 8 <= l1; e += 8) {  // eight loads per array in flight, added in list order
+      double vF[8], vH[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const size_t o = (size_t)lst[e + u] * bs + i;
+        vF[u] = slotF[o];
+        vH[u] = slotH[o];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        sF += vF[u];
+        sH += vH[u];
+      }
+    }
+    for (; e < l1; e++) {
+      const size_t o = (size_t)lst[e] * bs + i;
+      sF += slotF[o];
+      sH += slotH[o];
+    }
+    LGF[(size_t)go * bs + i] = sF;
+    LGH[(size_t)go * bs + i] = LGH[(size_t)go * bs + i] + sH;
+  }
 }
+
 // blocks of bs doubles: dst[didx[e]] = src[sidx[e]] for the listed entries e; grid (entry, chunk)
 __global__ void __launch_bounds__(256) k_copyblk(const int *sel, const int *sidx, const int *didx, int bs,
                                                  const double *src, double *dst) {
@@ -681,18 +731,7 @@ __global__ void __launch_bounds__(256) k_copyblk(const int *sel, const int *sidx
   double *d = dst + (size_t)didx[e] * bs;
   for (int i = blockIdx.y * 256 + threadIdx.x; i < bs; i += 256 * gridDim.y) d[i] = s[i];
 }
-// blocks of bs doubles: out[idx[e]] = a[idx[e]] + b[idx[e]] for the listed entries e
-__global__ void __launch_bounds__(256) k_addblk(const int *sel, const int *idx, int bs, const double *a,
-                                                const double *b, double *out) {
-  const size_t o = (size_t)idx[sel[blockIdx.x]] * bs;
-  for (int i = blockIdx.y * 256 + threadIdx.x; i < bs; i += 256 * gridDim.y) out[o + i] = a[o + i] + b[o + i];
-}
 
-// out = a + b over n doubles
-__global__ void __launch_bounds__(256) k_add(const double *a, const double *b, double *out, long long n) {
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) out[i] = a[i] + b[i];
-}
 
 // ----------------------------------------------------------------------------
 // per-sphere force (computeMol_Force_and_Torque moldynamics.cpp:525-550 with
@@ -1106,34 +1145,14 @@ __global__ void __launch_bounds__(256) k_surfmat(Tabs tb, const double *xyz, lon
 // type's stored harmonics Y (npts x p^2, point-major) as an FP64 matrix-core product
 //   C = Y E,  E[c][6t + b] = w_c,b * (gF_t | gH_t)[b][c],
 // w = (2 if m > 0 else 1) * C_n, C_n = (2n+1)/4pi (F) or cChgH[n] (H): the six charges of
-// every point and entry (expcenter.cpp:1378-1412).  k_surfE writes each entry's scaled E
-// ([c][6], per item); k_surfgemm: one wavefront per 16 points, NT 16-column tiles (entries
+// every point and entry (expcenter.cpp:1378-1412).  k_solve3 writes each entry's scaled E
+// ([c][6], per item) as it finishes; k_surfgemm: one wavefront per 16 points, NT 16-column tiles (entries
 // of 6 columns), both operands read straight from cache, K = p^2 in steps of 4.  The
 // epilogue stores the charges ([3][all points]), the squared norms of the F and H
 // 3-vectors, and per (entry, 64-point tile) the partial sums of |charge| in point order
 // (k_surfgrad_totals adds the tiles in order).
 // ----------------------------------------------------------------------------
 constexpr int SGM_ROWS = 64;  // points per workgroup (4 wavefronts x 16)
-
-// grid (entry of the list, chunk of 256 coefficients)
-__global__ void __launch_bounds__(256) k_surfE(const int *sel, const int *eitem, const int *esph,
-                                               const double *consts, const double *gF, const double *gH, double *E,
-                                               int p) {
-  const int e = sel[blockIdx.x], a = eitem[e], g = esph[e], P2 = p * p;
-  const int c = blockIdx.y * 256 + threadIdx.x;
-  if (c >= P2) return;
-  int n = (int)sqrtf((float)c);  // exact floor for c < 2^22 after the corrections
-  while (n * n > c) n--;
-  while ((n + 1) * (n + 1) <= c) n++;
-  const double two = (c == n * n) ? 1.0 : 2.0;
-  const double CF = (2 * n + 1) / (4.0 * M_PI), CH = consts[(size_t)g * 10 * p + 9 * p + n];
-  double *o = E + ((size_t)a * P2 + c) * 6;
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    o[d] = two * (CF * gF[((size_t)e * 3 + d) * P2 + c]);
-    o[3 + d] = two * (CH * gH[((size_t)e * 3 + d) * P2 + c]);
-  }
-}
 
 struct SGArgs {
   const int *bent;           // entries of the launch, bundle by bundle
@@ -1142,7 +1161,7 @@ struct SGArgs {
   const int *pbuf;           // [entry] item: block of E and of `part`
   const double *const *Y;    // [global sphere] harmonic rows of its exposed points
   const int *pt_off, *pt_cnt, *nspx;  // [global sphere] first point, exposed and spiral counts
-  const double *E;           // [item][p^2][6] scaled right-hand sides (k_surfE)
+  const double *E;           // [item][p^2][6] scaled right-hand sides (written by k_solve3)
   double *gSF, *gSH;         // [3][all points]
   double *nF, *nH;
   long long npts_all;
@@ -1235,6 +1254,103 @@ __global__ void __launch_bounds__(256) k_surfgemm(SGArgs A) {
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         const double gf = Cs[lane * NC + 6 * t + d] * dA, gh = Cs[lane * NC + 6 * t + 3 + d] * dA;
+        A.gSF[(size_t)d * A.npts_all + pt] = gf;
+        A.gSH[(size_t)d * A.npts_all + pt] = gh;
+        q[d] = fabs(gf);
+        q[3 + d] = fabs(gh);
+        nf += gf * gf;
+        nh += gh * gh;
+      }
+      A.nF[pt] = nf;
+      A.nH[pt] = nh;
+    }
+#pragma unroll
+    for (int b = 0; b < 6; b++) q[b] = wave_sum(q[b]);
+    if (lane == 0) {
+      double *pp = A.part + ((size_t)A.pbuf[e] * A.nchunk + blockIdx.x) * 6;
+#pragma unroll
+      for (int b = 0; b < 6; b++) pp[b] = q[b];
+    }
+  }
+  if (A.work && tid == 0) {
+    atomicAdd(&A.work[0], (unsigned long long)(12.0 * P2 * nrow * nent));
+    atomicAdd(&A.work[1], (unsigned long long)(8.0 * P2 * nrow + 64.0 * nrow * nent));
+  }
+}
+
+// The same product for bundles of one or two entries (a single system): one 16-point tile
+// per workgroup, its four wavefronts splitting K = p^2 in four ranges whose partial tiles
+// are added in order, so each wavefront's dependent chain is a quarter as long.
+constexpr int SG1_ROWS = 16;
+// grid (16-point tile, entry pair of the bundle, bundle); 256 threads
+__global__ void __launch_bounds__(256) k_surfgemm_k4(SGArgs A) {
+  __shared__ double Cp[4][16 * 16];
+  __shared__ int ent_s[2];
+  __shared__ double dA_s[2];
+  const int p = A.p, P2 = p * p, tid = threadIdx.x, w = tid / 64, lane = tid % 64;
+  const int b0 = A.boff[blockIdx.z], nb = A.boff[blockIdx.z + 1] - b0;
+  const int t0 = blockIdx.y * 2;
+  if (t0 >= nb) return;  // uniform
+  const int nent = min(2, nb - t0);
+  const int g0 = A.sph[A.bent[b0 + t0]];
+  const int npts = A.pt_cnt[g0];
+  const int h0 = blockIdx.x * SG1_ROWS;
+  if (tid < 2) {
+    const int e = (tid < nent) ? A.bent[b0 + t0 + tid] : -1;
+    ent_s[tid] = e;
+    dA_s[tid] = (e >= 0) ? 4 * M_PI / double(A.nspx[A.sph[e]]) : 0.0;
+  }
+  __syncthreads();
+  if (h0 >= npts) {  // a tile past this sphere's points: zero partials
+    if (tid < nent * 6) A.part[((size_t)A.pbuf[ent_s[tid / 6]] * A.nchunk + blockIdx.x) * 6 + tid % 6] = 0.0;
+    return;
+  }
+  const int nrow = min(SG1_ROWS, npts - h0);
+  const int ra = lane % 16, kq = lane / 16;
+  const bool rok = ra < nrow;
+  const double *ya = A.Y[g0] + (size_t)(h0 + (rok ? ra : 0)) * P2 + kq;
+  const int col = lane % 16, es = col / 6;
+  const bool cok = es < nent;
+  const double *eb = A.E + ((size_t)A.pbuf[ent_s[cok ? es : 0]] * P2 + kq) * 6 + col % 6;
+  // this wavefront's K range, in steps of 4
+  const int ksteps = (P2 + 3) / 4, kper = (ksteps + 3) / 4;
+  const int k0 = 4 * w * kper, k1 = min(P2, 4 * (w + 1) * kper);
+  v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+  constexpr int UNR = 8;
+  int k = k0;
+  for (; k + 4 * UNR <= k1 && k + 4 * UNR <= (P2 & ~3); k += 4 * UNR) {
+    double av[UNR], bv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      av[u] = rok ? ya[k + 4 * u] : 0.0;
+      bv[u] = cok ? eb[(size_t)(k + 4 * u) * 6] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+  }
+  for (; k < k1; k += 4) {  // remaining steps; k + l/16 past p^2 reads as zero
+    const bool kok = k + kq < P2;
+    const double av = (rok && kok) ? ya[k] : 0.0;
+    const double bv = (cok && kok) ? eb[(size_t)k * 6] : 0.0;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+  // lane l, register r holds row l/16 + 4 r, column l % 16
+#pragma unroll
+  for (int r = 0; r < 4; r++) Cp[w][(lane / 16 + 4 * r) * 16 + lane % 16] = acc[r];
+  __syncthreads();
+  // epilogue: wavefront t < nent handles entry t, lane = point
+  if (w < nent) {
+    const int t = w, e = ent_s[t], g = A.sph[e];
+    const double dA = dA_s[t];
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    if (lane < nrow) {
+      const size_t pt = (size_t)A.pt_off[g] + h0 + lane;
+      double nf = 0.0, nh = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const int cf = lane * 16 + 6 * t + d, ch = cf + 3;
+        const double gf = (((Cp[0][cf] + Cp[1][cf]) + Cp[2][cf]) + Cp[3][cf]) * dA;
+        const double gh = (((Cp[0][ch] + Cp[1][ch]) + Cp[2][ch]) + Cp[3][ch]) * dA;
         A.gSF[(size_t)d * A.npts_all + pt] = gf;
         A.gSH[(size_t)d * A.npts_all + pt] = gh;
         q[d] = fabs(gf);
