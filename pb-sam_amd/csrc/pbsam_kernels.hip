@@ -995,6 +995,7 @@ template <int KW>
 struct IdState {
   double Fr[KW], Fi[KW], Hr[KW], Hi[KW], fbr[KW], fbi[KW], hbr[KW], hbi[KW], uHr[KW], uHi[KW];
   double kH1[KW], kF1[KW], kH2[KW], kF2[KW], kIH[KW], kInv[KW];
+  double c2[KW];  // fast stopping test: the pair alone exceeds the threshold when top > c2 bot
   int nn[KW], i0[KW];
 };
 // (n, m) pair index ep -> n, and the real index of c(n, m)
@@ -1014,13 +1015,15 @@ __device__ __forceinline__ void id_consts(IdState<KW> &S, int k, int n, const do
   S.kIH[k] = cs[8 * p + n];
   S.kInv[k] = 1 / (double)(2 * n + 1);
 }
-// one sweep of this lane's pairs; returns the lane's computeDev partial
+// one sweep of this lane's pairs; returns the lane's computeDev partial (old values: H at
+// the start of the sweep)
 template <int KW>
-__device__ __forceinline__ double id_sweep(IdState<KW> &S, double (&oHr)[KW], double (&oHi)[KW]) {
+__device__ __forceinline__ double id_sweep(IdState<KW> &S) {
   double sum = 0.0;
   for (int k = 0; k < KW; k++) {
     if (S.nn[k] < 0) continue;
     const bool mpos = (S.i0[k] != S.nn[k] * S.nn[k]);
+    const double oHr = S.Hr[k], oHi = S.Hi[k];
     double xr = S.fbr[k] + S.kH1[k] * S.Hr[k] + S.kF1[k] * S.Fr[k];
     double xi = S.fbi[k] + S.kH1[k] * S.Hi[k] + S.kF1[k] * S.Fi[k];
     S.Fr[k] = xr * S.kInv[k];
@@ -1030,44 +1033,39 @@ __device__ __forceinline__ double id_sweep(IdState<KW> &S, double (&oHr)[KW], do
     S.Hr[k] = xr * S.kIH[k];
     S.Hi[k] = mpos ? xi * S.kIH[k] : 0.0;
     double sv;
-    if (dev_term(oHr[k], mpos ? oHi[k] : 0.0, S.Hr[k], mpos ? S.Hi[k] : 0.0, mpos, sv)) sum += sv;
-    oHr[k] = S.Hr[k];
-    oHi[k] = S.Hi[k];
+    if (dev_term(oHr, mpos ? oHi : 0.0, S.Hr[k], mpos ? S.Hi[k] : 0.0, mpos, sv)) sum += sv;
   }
   return sum;
 }
 // One sweep as id_sweep without the computeDev sum: true when one of this lane's pairs
 // alone makes the sweep's computeDev exceed 1e-8 -- its term s (s >= 0 for every pair)
-// exceeds c = 4 p^2 1e-8 with a relative margin of 1e-6, tested as top > c bot without
-// the division.  The sum of the nonnegative terms is then above the threshold whatever
-// the summation order.
+// exceeds c = 4 p^2 1e-8 with a relative margin of 1e-6, tested as top > c2 bot (c2 = c, or
+// c/2 where the term carries the reference's factor 2) without the division.  Pairs whose
+// old and new values are all below 1e-30 are skipped as in the reference; a pair with only
+// one side that small (the reference's term 1) certifies only through the same test, which
+// can only leave a sweep uncertified.  The sum of the nonnegative terms is then above the
+// threshold whatever the summation order.  m = 0 pairs keep zero imaginary parts by
+// themselves (zero right-hand side and start), so no selects are needed here.
 template <int KW>
-__device__ __forceinline__ bool id_sweep_fast(IdState<KW> &S, double (&oHr)[KW], double (&oHi)[KW], double c) {
+__device__ __forceinline__ bool id_sweep_fast(IdState<KW> &S) {
   bool big = false;
   for (int k = 0; k < KW; k++) {
     if (S.nn[k] < 0) continue;
-    const bool mpos = (S.i0[k] != S.nn[k] * S.nn[k]);
+    const double ar = S.Hr[k], ai = S.Hi[k];
     double xr = S.fbr[k] + S.kH1[k] * S.Hr[k] + S.kF1[k] * S.Fr[k];
     double xi = S.fbi[k] + S.kH1[k] * S.Hi[k] + S.kF1[k] * S.Fi[k];
     S.Fr[k] = xr * S.kInv[k];
-    S.Fi[k] = mpos ? xi * S.kInv[k] : 0.0;
+    S.Fi[k] = xi * S.kInv[k];
     xr = S.hbr[k] + S.kH2[k] * S.Hr[k] + S.kF2[k] * S.Fr[k];
     xi = S.hbi[k] + S.kH2[k] * S.Hi[k] + S.kF2[k] * S.Fi[k];
-    S.Hr[k] = xr * S.kIH[k];
-    S.Hi[k] = mpos ? xi * S.kIH[k] : 0.0;
-    const double ar = oHr[k], ai = mpos ? oHi[k] : 0.0, br = S.Hr[k], bi = mpos ? S.Hi[k] : 0.0;
-    const double PL = 1e-30;
-    const bool atiny = fabs(ar) < PL && fabs(ai) < PL, btiny = fabs(br) < PL && fabs(bi) < PL;
-    if (atiny != btiny) {
-      big = true;  // s = 1
-    } else if (!atiny) {
-      const double dr = ar - br, di = ai - bi;
-      const double top = (mpos ? 2.0 : 1.0) * (dr * dr + di * di);
-      const double bot = ar * ar + ai * ai + br * br + bi * bi;
-      big = big || (top > c * bot);
-    }
-    oHr[k] = S.Hr[k];
-    oHi[k] = S.Hi[k];
+    const double br = xr * S.kIH[k], bi = xi * S.kIH[k];
+    S.Hr[k] = br;
+    S.Hi[k] = bi;
+    const double mx = fmax(fmax(fabs(ar), fabs(ai)), fmax(fabs(br), fabs(bi)));
+    const double dr = ar - br, di = ai - bi;
+    const double top = dr * dr + di * di;
+    const double bot = ar * ar + ai * ai + br * br + bi * bi;
+    big = big || (mx >= 1e-30 && top > S.c2[k] * bot);
   }
   return big;
 }
@@ -1087,24 +1085,19 @@ template <int KW, int NW>
 __device__ int id_iterate(IdState<KW> &S, int p, double *red) {
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
   int *flags = reinterpret_cast<int *>(red + 2 * IC * NW + 2 * NW);
-  double oHr[KW], oHi[KW];
-  for (int k = 0; k < KW; k++) {
-    oHr[k] = S.Hr[k];
-    oHi[k] = S.Hi[k];
-  }
   const double cthr = 4.0 * p * p * 1e-8 * (1.0 + 1e-6);
+  for (int k = 0; k < KW; k++) S.c2[k] = (S.nn[k] >= 0 && S.i0[k] != S.nn[k] * S.nn[k]) ? 0.5 * cthr : cthr;
   int ct = 0;
   for (int chunk = 0;; chunk++) {
-    double sFr[KW], sFi[KW], sHr[KW], sHi[KW], sOr[KW], sOi[KW];
+    double sFr[KW], sFi[KW], sHr[KW], sHi[KW];
     for (int k = 0; k < KW; k++) {
       sFr[k] = S.Fr[k]; sFi[k] = S.Fi[k]; sHr[k] = S.Hr[k]; sHi[k] = S.Hi[k];
-      sOr[k] = oHr[k]; sOi[k] = oHi[k];
     }
     {
       // fast pass: bit t of `mine` set when this lane shows sweep t continues
       unsigned mine = 0;
       for (int t = 0; t < IC; t++)
-        if (id_sweep_fast(S, oHr, oHi, cthr)) mine |= 1u << t;
+        if (id_sweep_fast(S)) mine |= 1u << t;
       unsigned wb = 0;
       for (int t = 0; t < IC; t++)
         if (__ballot((mine >> t) & 1u)) wb |= 1u << t;
@@ -1124,11 +1117,10 @@ __device__ int id_iterate(IdState<KW> &S, int p, double *red) {
       }
       for (int k = 0; k < KW; k++) {
         S.Fr[k] = sFr[k]; S.Fi[k] = sFi[k]; S.Hr[k] = sHr[k]; S.Hi[k] = sHi[k];
-        oHr[k] = sOr[k]; oHi[k] = sOi[k];
       }
     }
     double part[IC];
-    for (int t = 0; t < IC; t++) part[t] = id_sweep(S, oHr, oHi);
+    for (int t = 0; t < IC; t++) part[t] = id_sweep(S);
     for (int t = 0; t < IC; t++) part[t] = wave_sum_dpp(part[t]);
     double *rb = red + (chunk & 1) * IC * NW;  // double-buffered: one barrier per chunk
     if (lane == 0)
@@ -1152,9 +1144,8 @@ __device__ int id_iterate(IdState<KW> &S, int p, double *red) {
     if (stop < IC - 1) {
       for (int k = 0; k < KW; k++) {
         S.Fr[k] = sFr[k]; S.Fi[k] = sFi[k]; S.Hr[k] = sHr[k]; S.Hi[k] = sHi[k];
-        oHr[k] = sOr[k]; oHi[k] = sOi[k];
       }
-      for (int t = 0; t <= stop; t++) (void)id_sweep(S, oHr, oHi);
+      for (int t = 0; t <= stop; t++) (void)id_sweep(S);
     }
     ct += stop + 1;
     return ct;
