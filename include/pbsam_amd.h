@@ -65,8 +65,10 @@ typedef struct pbsam_moltype_desc {
   int npts_cap;          /* cap of the radius rule (reference: 10000) */
   double intra_rcutoff;  /* near/far split (reference: 30 A) */
   const double *imats;   /* n_sph*p^4 column-major surface-integral matrices, or NULL */
-  int imat_npts;         /* with imats NULL: spiral size used to detect buried points;
-                            spheres with none get the exact identity matrix */
+  int imat_npts;         /* with imats NULL: spiral size of the matrix build (on the device,
+                            as pbsam_imat_build) for spheres with buried points; the others get
+                            the exact identity matrix.  0 with overlapping spheres and no
+                            imats: PBSAM_ERR_UNSUPPORTED */
 } pbsam_moltype_desc;
 
 int pbsam_moltype_create(pbsam_ctx *ctx, const pbsam_moltype_desc *desc, pbsam_moltype **out);
@@ -166,6 +168,13 @@ int pbsam_compute_forces(pbsam_system *sys, double tol, int max_rounds, double *
 int pbsam_system_get_grad(const pbsam_system *sys, int mol, int k, double *gH, double *gLHN);
 
 /* ---- kernel-level entry points (one batch of independent jobs; host arrays in/out) ---- */
+/* CSolExpCenter::build_IMat (src/expcenter.h:207, src/expcenter.cpp:667-902), the `pbsam mat`
+ * stage for one sphere: from its exposed (npe) and buried (npb) surface points (3 doubles
+ * each, relative to the centre, on the sphere's spiral), the p^2 x p^2 column-major
+ * surface-integral matrix.  pbsam_moltype_create builds these itself when imats is NULL and
+ * imat_npts > 0. */
+int pbsam_imat_build(pbsam_ctx *ctx, int p, int npe, const double *spe, int npb, const double *spb,
+                     double *imat);
 /* CXFormA::reset + xformH (src/xforms.cpp:51-77,123-132) for n pairs.
  * P: 3n (target - source), a: 2n (source radius, target radius), order: n (<= p),
  * in/out: n*p*p blocks. */

@@ -570,6 +570,38 @@ struct pbsam_moltype {
   DBuf<int> d_selx, d_selp;
 };
 
+// CSolExpCenter::build_IMat (expcenter.cpp:667-902) on the device: from the smaller of the
+// exposed (spe) and buried (spb) point sets, into `out` (device, p^4 doubles, column-major)
+static void imat_build_dev(pbsam_ctx *c, int p, int npe, const double *spe, int npb, const double *spb,
+                           double *out) {
+  hipStream_t s = c->stream;
+  const int P2 = p * p;
+  const bool useE = (npe <= npb);
+  const int npts = useE ? npe : npb;
+  const double *pts = useE ? spe : spb;
+  DBuf<double> dp, Y, G, Gw;
+  dp.upload(std::vector<double>(pts, pts + 3 * (size_t)npts), s);
+  Y.alloc(std::max<size_t>((size_t)npts * P2, 1));
+  G.alloc((size_t)P2 * P2);
+  Gw.alloc((size_t)p * p);
+  if (npts > 0) {
+    hipLaunchKernelGGL(k_surfmat, dim3((unsigned)((npts + 3) / 4)), dim3(256), 0, s, c->tabs, dp.p, (long long)npts,
+                       Y.p, p);
+    HIPCHK(hipGetLastError());
+  }
+  const int nt = (P2 + 15) / 16;
+  hipLaunchKernelGGL(k_imat_gram, dim3(nt, nt), dim3(64), 0, s, Y.p, npts, p, G.p);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_imat_gramw, dim3((p * p + 63) / 64), dim3(64), 0, s, Y.p, npts, p, Gw.p);
+  HIPCHK(hipGetLastError());
+  const double dA = 4 * M_PI / (double)(npe + npb - 1);
+  const long long tot = (long long)P2 * P2;
+  hipLaunchKernelGGL(k_imat_assemble, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, G.p, Gw.p, p, dA,
+                     useE ? 1 : 0, out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+}
+
 static void moltype_setup(pbsam_moltype *m, const pbsam_moltype_desc *d) {
   pbsam_ctx *c = m->c;
   hipStream_t s = c->stream;
@@ -727,11 +759,45 @@ static void moltype_setup(pbsam_moltype *m, const pbsam_moltype_desc *d) {
     std::vector<double> im(d->imats, d->imats + (size_t)ns * P2 * P2);
     m->d_imats.upload(im, s);
     for (int k = 0; k < ns; k++) m->imat_ptr[k] = m->d_imats.p + (size_t)k * P2 * P2;
+  } else if (d->imat_npts > 0) {
+    // built on the device from a spiral of imat_npts points per sphere, split into exposed and
+    // buried by the neighbours (the exact identity where nothing is buried)
+    std::vector<std::vector<double>> spe(ns), spb(ns);
+    int nb = 0;
+    for (int ki = 0; ki < ns; ki++) {
+      const SphHost &S = m->sph[ki];
+      if (S.neigh.empty()) continue;
+      const double *ci = d->cen + 3 * ki;
+      std::vector<double> xyz;
+      spiral(d->imat_npts, S.rad, xyz);
+      for (int h = 0; h < d->imat_npts; h++) {
+        const double *q = &xyz[3 * h];
+        bool buried = false;
+        for (int j : S.neigh) {
+          const double *cj = d->cen + 3 * j;
+          double t[3] = {q[0] + ci[0], q[1] + ci[1], q[2] + ci[2]};
+          double u[3] = {t[0] - cj[0], t[1] - cj[1], t[2] - cj[2]};
+          if (dot3(u, u) < rad2[j]) { buried = true; break; }
+        }
+        for (int x = 0; x < 3; x++) (buried ? spb[ki] : spe[ki]).push_back(q[x]);
+      }
+      if (!spb[ki].empty()) nb++;
+    }
+    if (nb) {
+      m->d_imats.alloc((size_t)nb * P2 * P2);
+      int slot = 0;
+      for (int ki = 0; ki < ns; ki++) {
+        if (spb[ki].empty()) continue;
+        double *out = m->d_imats.p + (size_t)slot++ * P2 * P2;
+        imat_build_dev(c, p, (int)spe[ki].size() / 3, spe[ki].data(), (int)spb[ki].size() / 3, spb[ki].data(), out);
+        m->imat_ptr[ki] = out;
+      }
+    }
   } else {
     for (int k = 0; k < ns; k++)
       if (!m->sph[k].neigh.empty())
         fail(PBSAM_ERR_UNSUPPORTED, "sphere " + std::to_string(k) +
-                                        " overlaps a neighbour: its surface-integral matrix must be supplied");
+                                        " overlaps a neighbour: supply its surface-integral matrix or imat_npts");
   }
   m->d_imat.upload(m->imat_ptr, s);
   HIPCHK(hipStreamSynchronize(s));
@@ -2489,6 +2555,19 @@ static int xform_batch(pbsam_ctx *c, int n, int p, const double *P, const double
       std::vector<double> rF = dsF.download(s);
       memcpy(outF, rF.data(), sizeof(double) * n * P2);
     }
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_imat_build(pbsam_ctx *c, int p, int npe, const double *spe, int npb, const double *spb, double *imat) {
+  if (!c || p < 1 || p > PMAX || npe < 0 || npb < 0 || npe + npb < 2 || (npe && !spe) || (npb && !spb) || !imat)
+    return PBSAM_ERR_BADARG;
+  return guarded(&c->err, [&]() {
+    HIPCHK(hipSetDevice(c->device));
+    DBuf<double> out;
+    out.alloc((size_t)p * p * p * p);
+    imat_build_dev(c, p, npe, spe, npb, spb, out.p);
+    HIPCHK(hipMemcpy(imat, out.p, sizeof(double) * out.n, hipMemcpyDeviceToHost));
     return PBSAM_OK;
   });
 }

@@ -33,7 +33,7 @@ SYMBOLS = [
     "pbsam_xform_intra_batch", "pbsam_xform_num_batch", "pbsam_rotate_quat_batch",
     "pbsam_profile", "pbsam_profile_read", "pbsam_sync", "pbsam_gradient", "pbsam_forces",
     "pbsam_compute_forces", "pbsam_system_get_grad", "pbsam_xform_h_grad_batch",
-    "pbsam_batch_create", "pbsam_batch_status",
+    "pbsam_batch_create", "pbsam_batch_status", "pbsam_imat_build",
 ]
 
 KERNELS = {"xform": 0, "p2l": 1, "solve": 2, "surface": 3, "xgrad": 4, "surfgrad": 5}
@@ -88,6 +88,7 @@ def load():
         "pbsam_system_create": ([P, I, P, P, P, P], I),
         "pbsam_batch_create": ([P, I, P, P, P, P, P], I),
         "pbsam_batch_status": ([P, P, P, P], I),
+        "pbsam_imat_build": ([P, I, I, P, I, P, P], I),
         "pbsam_system_destroy": ([P], None),
         "pbsam_system_set_pose": ([P, I, P, P], I),
         "pbsam_system_set_cutoffs": ([P, D, D], I),
@@ -167,6 +168,13 @@ class Context:
         n = len(order)
         out = np.zeros((n, p * p))
         self.check(self.L.pbsam_xform_h_batch(self.h, n, p, _p(P3), _p(a), _p(order), _p(blocks), _p(out)))
+        return out
+
+    def imat_build(self, p, spe, spb):
+        """build_IMat from exposed and buried points (n x 3 each): p^2 x p^2 column-major"""
+        spe, spb = f64(spe).reshape(-1), f64(spb).reshape(-1)
+        out = np.zeros(p ** 4)
+        self.check(self.L.pbsam_imat_build(self.h, p, len(spe) // 3, _p(spe), len(spb) // 3, _p(spb), _p(out)))
         return out
 
     def xform_intra(self, P3, a, orders, H, F, p):
