@@ -1197,3 +1197,4 @@ extern "C" void orc_imat_build(orc_ctx *c, int p, int npe, const double *spe, in
 }
 
 #include "pbsam_oracle_solver.inc"
+#include "pbsam_oracle_bd.inc"

@@ -19,7 +19,9 @@
  *     of charges, central differences);
  *   orc_selfpol (reference order): Kirkwood's closed form for one sphere;
  *   orc_mutual / orc_system_pot: Debye-Hueckel interaction of two spheres;
- *   orc_gradient / orc_forces / orc_compute_forces: central differences of the energy.
+ *   orc_gradient / orc_forces / orc_compute_forces: central differences of the energy;
+ *   orc_bd_*: drand48 against the C library's, and whole trajectories against the
+ *     reference's own CBDnam::makeMove / compute_dt compiled from its sources.
  * Parity unpinned: orc_system_total_int_energy (not on the product's ABI), and the
  * Jacobi variants (orc_selfpol_round_jacobi, schedule 1 of orc_selfpol / orc_mutual /
  * orc_gradient), which have no reference counterpart and serve only to check the
@@ -137,6 +139,23 @@ void orc_system_get_grad(const orc_system *s, int i, int k, double *gH, double *
 double orc_system_pot(const orc_system *s, int i);
 double orc_system_total_int_energy(orc_system *s);
 int orc_system_counts(const orc_system *s, int *n_inter_pairs, int *n_polarized);
+
+/* ---- Brownian dynamics of one two-molecule trajectory (CBDnam) ----
+ * prm: dtr dr0 dr1 b_dist q_dist f_dist fact_tr ikbt coul_k dt_fixed(0: compute_dt)
+ * max_tries.  place != 0: CBDnam::run's random start from srand48(seed); else pos(6),
+ * quat(8).  Status: ESCAPED 0, DOCKED 1, RUNNING 2, STUCK 3 (-1: forces failed). */
+typedef struct orc_bd orc_bd;
+orc_bd *orc_bd_create(int ns0, const double *cen0, const double *rad0, int ns1,
+                      const double *cen1, const double *rad1, const double *prm,
+                      long long seed, int place, const double *pos, const double *quat);
+void orc_bd_destroy(orc_bd *b);
+void orc_bd_set_system(orc_bd *b, orc_system *s);
+void orc_bd_add_contact(orc_bd *b, int ncontact, int npair, const int *pairs,
+                        const double *dist);
+int orc_bd_step(orc_bd *b, const double *ft);
+void orc_bd_get(const orc_bd *b, double *pos, double *quat, double *rot, double *dt,
+                long long *steps);
+void orc_bd_rng(long long seed, int n, double *out);
 
 #ifdef __cplusplus
 }

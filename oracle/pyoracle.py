@@ -95,6 +95,17 @@ def lib():
         for name in ("orc_moltype_energy_self",):
             getattr(L, name).restype = D
             getattr(L, name).argtypes = [P]
+        L.orc_bd_create.restype = P
+        L.orc_bd_create.argtypes = [I, P, P, I, P, P, P, ctypes.c_longlong, I, P, P]
+        for name, args in [("orc_bd_destroy", [P]), ("orc_bd_set_system", [P, P]),
+                           ("orc_bd_add_contact", [P, I, I, P, P]),
+                           ("orc_bd_get", [P, P, P, P, P, P]),
+                           ("orc_bd_rng", [ctypes.c_longlong, I, P])]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = None
+        L.orc_bd_step.restype = I
+        L.orc_bd_step.argtypes = [P, P]
         L.orc_system_pot.restype = D
         L.orc_system_pot.argtypes = [P, I]
         L.orc_system_total_int_energy.restype = D
@@ -398,3 +409,61 @@ class System:
         a, b = ctypes.c_int(), ctypes.c_int()
         self.orc.L.orc_system_counts(self.h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+
+# CBDnam's constants: m_Dtr, m_Dr1, m_Dr2 (BDnam.cpp:143), b_DIST, q_DIST, f_DIST (BDnam.h:23-25),
+# FACT2, IKbT, COUL_K (BDnam.h:10,18-19), compute_dt schedule, 500 tries (BDnam.cpp:268)
+BD_DEFAULTS = [0.03, 4.5e-5, 4e-5, 100.0, 500.0, 100.0, 5.604586e2, 1.0 / (1.3806503e-23 * 298.15),
+               2.30708e-18, 0.0, 500]
+
+
+class BDTrajectory:
+    """One two-molecule Brownian-dynamics trajectory (CBDnam) on the oracle.
+
+    geom: [(cen (ns,3) relative to the molecule centre, rad (ns,))] * 2.  With place=True the
+    start is CBDnam::run's random placement from srand48(seed); else pos (2,3), quat (2,4).
+    """
+
+    def __init__(self, geom, seed, place=True, pos=None, quat=None, prm=None):
+        self.L = lib()
+        prm = f64(BD_DEFAULTS if prm is None else prm)
+        (c0, r0), (c1, r1) = [(f64(c), f64(r)) for c, r in geom]
+        self._keep = (c0, r0, c1, r1, prm)
+        pos = None if pos is None else f64(pos)
+        quat = None if quat is None else f64(quat)
+        self.h = self.L.orc_bd_create(len(r0), _p(c0), _p(r0), len(r1), _p(c1), _p(r1), _p(prm),
+                                      int(seed), 1 if place else 0, _p(pos), _p(quat))
+        self.system = None
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_bd_destroy(self.h)
+            self.h = None
+
+    def attach(self, system):
+        """Take forces from an oracle System of the same two molecule types."""
+        self.system = system
+        self.L.orc_bd_set_system(self.h, system.h)
+
+    def add_contact(self, ncontact, pairs, dist):
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).ravel()
+        dist = f64(dist)
+        self.L.orc_bd_add_contact(self.h, int(ncontact), len(dist), _p(pairs), _p(dist))
+
+    def step(self, ft=None):
+        ft = None if ft is None else f64(ft)
+        return self.L.orc_bd_step(self.h, _p(ft))
+
+    def state(self):
+        pos, quat, rot, dt = np.zeros(6), np.zeros(8), np.zeros(8), np.zeros(1)
+        steps = ctypes.c_longlong(0)
+        self.L.orc_bd_get(self.h, _p(pos), _p(quat), _p(rot), _p(dt), ctypes.byref(steps))
+        return dict(pos=pos.reshape(2, 3), quat=quat.reshape(2, 4), rot=rot.reshape(2, 4), dt=dt[0],
+                    steps=steps.value)
+
+
+def bd_rng(seed, n):
+    """(first n drand48 values after srand48(seed), first n normRand values)."""
+    out = np.zeros(2 * n)
+    lib().orc_bd_rng(int(seed), n, _p(out))
+    return out[:n], out[n:]

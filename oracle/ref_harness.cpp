@@ -14,6 +14,10 @@
 #include <string>
 #include <vector>
 
+#include <new>
+
+#include "BDnam.h"
+#include "contact.h"
 #include "expansion.h"
 #include "molecule.h"
 #include "rotcoeff.h"
@@ -43,6 +47,51 @@ static std::vector<double> padded(const CExpan &E, int p) {
   const std::vector<double> &M = E.getVector();
   for (size_t i = 0; i < M.size() && (int)i < p * p; i++) v[i] = M[i];
   return v;
+}
+
+// One molecule of a Brownian-dynamics pair, built with the constructor CBDnam uses
+// (molecule.cpp:422-487).  Reads: ns (x y z r)*ns nq (x y z q)*nq, centres relative to
+// the molecule centre.  Self expansions are zero: makeMove never reads them.
+static CMolecule *bd_molecule(int moltype) {
+  int ns = ri();
+  std::vector<CPnt> cens(ns);
+  std::vector<double> rad(ns);
+  for (int k = 0; k < ns; k++) {
+    double x = rd(), y = rd(), z = rd();
+    cens[k] = CPnt(x, y, z);
+    rad[k] = rd();
+  }
+  int nq = ri();
+  std::vector<CPnt> pos(nq);
+  std::vector<double> q(nq);
+  for (int i = 0; i < nq; i++) {
+    double x = rd(), y = rd(), z = rd();
+    pos[i] = CPnt(x, y, z);
+    q[i] = rd();
+  }
+  const double idiel = 4.0, intraRc = 30.0;
+  int L = N_POLES * N_POLES;
+  std::vector<double> zero(L, 0.0);
+  std::vector<CMulExpan *> Fs(ns), Hs(ns);
+  for (int k = 0; k < ns; k++) {
+    Fs[k] = new CMulExpan(&zero[0], CRange(0, N_POLES), rad[k]);
+    Hs[k] = new CMulExpan(&zero[0], CRange(0, N_POLES), rad[k]);
+  }
+  std::vector<std::vector<CPnt> > SPxes;
+  std::vector<int> nSPx;
+  std::vector<std::vector<int> > neighs, nearL, farL;
+  CMolecule::generateMolSPX(cens, rad, SPxes, nSPx, neighs);
+  CMolecule::generateMolTypeIntraPolLists(cens, rad, intraRc, nearL, farL);
+  std::vector<std::vector<double> > qF, qH;
+  std::vector<double> tF, tH;
+  CMolecule::generateMolExposedChargesFH(Fs, Hs, cens, rad, SPxes, nSPx, qF, qH, tF, tH);
+  std::vector<CLocalExpan *> LF, LH, LFf, LHf;
+  CMolecule::computeMolTypeValues(CPnt(), cens, rad, q, pos, idiel, intraRc, SPxes, nSPx, neighs, Fs, Hs,
+                                  qF, qH, tF, tH, nearL, farL, LF, LH, LFf, LHf);
+  std::vector<REAL *> iMats(ns, (REAL *)0);
+  std::vector<CMolCell> cells;
+  return new CMolecule(moltype, CPnt(), cens, rad, q, pos, idiel, iMats, intraRc, SPxes, nSPx, neighs, nearL,
+                       Fs, Hs, LF, LH, LFf, LHf, qF, qH, tF, tH, cells);
 }
 
 int main() {
@@ -338,6 +387,97 @@ int main() {
         res.insert(res.end(), b.begin(), b.end());
         res.insert(res.end(), c2.begin(), c2.end());
         res.insert(res.end(), d.begin(), d.end());
+      }
+    } else if (c == "bd") {
+      // seed place nsteps Dtr Dr1 Dr2 ft(9) | molecule 0 | molecule 1 | [pos0 q0 pos1 q1 if !place]
+      // | ndef (ncontact npair (k0 k1 dist)*npair)*ndef
+      //   -> per step: status dt pos0(3) q0(4) pos1(3) q1(4) rot1(4) rot2(4)
+      // Must be the only "bd" command of its process: normRand keeps a static spare value.
+      long seed = (long)rd();
+      int place = ri(), nsteps = ri();
+      double Dtr = rd(), Dr1 = rd(), Dr2 = rd();
+      std::vector<double> ft = rdv(9);
+      CMolecule::N_MOL = 0;
+      std::vector<CMolecule *> mols;
+      mols.push_back(bd_molecule(0));
+      mols.push_back(bd_molecule(1));
+      // a CBDnam around the two molecules, without the file-reading constructor
+      CBDnam *bd = static_cast<CBDnam *>(::operator new(sizeof(CBDnam)));
+      memset((void *)bd, 0, sizeof(CBDnam));
+      new (&bd->m_mols) std::vector<CMolecule *>(mols);
+      bd->m_Dtr = Dtr;
+      bd->m_Dr1 = Dr1;
+      bd->m_Dr2 = Dr2;
+      bd->m_rot1.identity();
+      bd->m_rot2.identity();
+      if (place) {  // CBDnam::run, BDnam.cpp:171-181
+        srand48(seed);
+        CQuat Q;
+        mols[0]->setPos(CPnt());
+        Q = CQuat::chooseRandom();
+        mols[0]->setOrient(Q);
+        mols[1]->setPos(b_DIST * randOrient());
+        Q = CQuat::chooseRandom();
+        mols[1]->setOrient(Q);
+      } else {
+        for (int i = 0; i < 2; i++) {
+          double x = rd(), y = rd(), z = rd();
+          double w = rd(), qx = rd(), qy = rd(), qz = rd();
+          mols[i]->setPos(CPnt(x, y, z));
+          mols[i]->setOrient(CQuat(w, CPnt(qx, qy, qz)));
+        }
+        srand48(seed);
+      }
+      CBDnam::initConstants(2);
+      CBDnam::MOLCONTACTLIST.assign(1, std::vector<CMolContact>());
+      int ndef = ri();
+      for (int d = 0; d < ndef; d++) {
+        int ncontact = ri(), npair = ri();
+        std::vector<CContact> cl;
+        for (int k = 0; k < npair; k++) {
+          int k0 = ri(), k1 = ri();
+          double dist = rd();
+          cl.push_back(CContact(k0, k1, dist));
+        }
+        CBDnam::MOLCONTACTLIST[0].push_back(CMolContact(1, ncontact, cl));
+      }
+      CPnt F1(ft[0], ft[1], ft[2]), T0(ft[3], ft[4], ft[5]), T1(ft[6], ft[7], ft[8]);
+      CBDnam::STATUS status = CBDnam::RUNNING;
+      REAL fact = FACT2;
+      for (int st = 0; st < nsteps && status == CBDnam::RUNNING; st++) {  // CBDnam::run's loop
+        REAL dt = bd->compute_dt();
+        REAL dist = CMolecule::getC2CDist(mols[0], mols[1]);
+        CPnt dR2, dO1, dO2;
+        if (dist < f_DIST) {  // the given force and torques stand in for computeForces
+          dR2 = (bd->m_Dtr * dt * fact) * F1;
+          dO1 = (bd->m_Dr1 * dt * IKbT * COUL_K) * T0;
+          dO2 = (bd->m_Dr2 * dt * IKbT * COUL_K) * T1;
+        } else {
+          dR2.zero();
+          dO1.zero();
+          dO2.zero();
+        }
+        status = bd->makeMove(dR2, dO1, dO2, dt);
+        res.push_back((double)status);
+        res.push_back(dt);
+        for (int i = 0; i < 2; i++) {
+          CPnt r = mols[i]->getRCen();
+          CQuat q = mols[i]->getOrient();
+          res.push_back(r.x());
+          res.push_back(r.y());
+          res.push_back(r.z());
+          res.push_back(q.w());
+          res.push_back(q.x());
+          res.push_back(q.y());
+          res.push_back(q.z());
+        }
+        const CQuat *rq[2] = {&bd->m_rot1, &bd->m_rot2};
+        for (int i = 0; i < 2; i++) {
+          res.push_back(rq[i]->w());
+          res.push_back(rq[i]->x());
+          res.push_back(rq[i]->y());
+          res.push_back(rq[i]->z());
+        }
       }
     } else {
       std::cout.rdbuf(old);
