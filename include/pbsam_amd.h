@@ -167,6 +167,56 @@ int pbsam_compute_forces(pbsam_system *sys, double tol, int max_rounds, double *
  * (3 p^2), as left by the last pbsam_gradient */
 int pbsam_system_get_grad(const pbsam_system *sys, int mol, int k, double *gH, double *gLHN);
 
+/* ---- Brownian dynamics of two-molecule trajectories (CBDnam, src/BDnam.cpp:166-417) ----
+ * A batch made by pbsam_batch_create with two molecules in every system runs one trajectory
+ * per system.  A step of a running trajectory is one pass of CBDnam::run's loop
+ * (src/BDnam.cpp:196-246): the time step of compute_dt (:409-417) or dt_fixed, forces and
+ * torques from the batched force evaluation when the centres are closer than f_dist (the
+ * system's cutoffs apply: set them as CBDnam does, 10 and 100 A, src/BDnam.cpp:138-140), and
+ * makeMove (:262-370) with its collision retries, escape, stored rotations and docking test
+ * (:377-401).  Each trajectory draws from its own drand48 stream seeded as srand48(seed), with
+ * normRand's polar method (src/util.h:514-537): trajectory g is the reference's run of one
+ * trajectory after srand48(seeds[g]).  Status values are CBDnam::STATUS (src/BDnam.h:39);
+ * PBSAM_BD_FAILED marks a trajectory whose force evaluation found overlapping spheres (the
+ * reference exits there, src/BDnam.cpp:211-213). */
+#define PBSAM_BD_ESCAPED 0
+#define PBSAM_BD_DOCKED 1
+#define PBSAM_BD_RUNNING 2
+#define PBSAM_BD_STUCK 3
+#define PBSAM_BD_FAILED 4
+typedef struct pbsam_bd_params {
+  double dtr;      /* translational diffusion of molecule 1 (m_Dtr, A^2/ps) */
+  double dr[2];    /* rotational diffusion of molecules 0 and 1 (m_Dr1, m_Dr2, rad^2/ps) */
+  double b_dist;   /* start distance of the random placement (b_DIST) */
+  double q_dist;   /* escape distance (q_DIST) */
+  double f_dist;   /* centre distance below which forces are computed (f_DIST) */
+  double fact_tr;  /* force -> displacement factor (FACT2) */
+  double ikbt;     /* 1/kT (IKbT) */
+  double coul_k;   /* torque unit factor (COUL_K) */
+  double dt_fixed; /* > 0: fixed time step instead of compute_dt */
+  int max_tries;   /* collision retries (500) */
+} pbsam_bd_params;
+/* CBDnam's values: src/BDnam.cpp:143, src/BDnam.h:10,18-25, compute_dt, 500 tries */
+void pbsam_bd_defaults(pbsam_bd_params *prm);
+/* Start one trajectory per system of the batch, seeds: n_sys values.  place != 0: CBDnam::run's
+ * random start (molecule 0 at the origin, molecule 1 at b_dist in a random direction, random
+ * orientations; src/BDnam.cpp:171-181); place == 0: the systems' current poses. */
+int pbsam_bd_start(pbsam_system *batch, const pbsam_bd_params *prm, const long long *seeds, int place);
+/* Docking criteria (CBDnam::addMolContact + IsDocked, src/BDnam.cpp:38-41,377-401): n_def
+ * definitions; definition d holds n_pair[d] sphere pairs (pairs: 2 ints each, sphere of
+ * molecule 0, sphere of molecule 1, all definitions in order) with separation thresholds
+ * dist, and docks when n_contact[d] of its pairs are within their thresholds. */
+int pbsam_bd_contacts(pbsam_system *batch, int n_def, const int *n_contact, const int *n_pair, const int *pairs,
+                      const double *dist);
+/* Advance every running trajectory by up to n_steps steps.  status and steps (n_sys each,
+ * either may be NULL): each trajectory's status and total steps taken.  The systems' poses
+ * follow the trajectories.  Returns PBSAM_ERR_NOCONV when a force evaluation hit its round
+ * limit (the reference logs and goes on). */
+int pbsam_bd_run(pbsam_system *batch, int n_steps, int *status, long long *steps);
+/* trajectory g: pos (6), quat (8, w x y z, as the reference keeps them: products of unit
+ * rotations, not renormalized), stored rotations m_rot1/m_rot2 (8), the last time step */
+int pbsam_bd_get(const pbsam_system *batch, int g, double *pos, double *quat, double *rot, double *dt);
+
 /* ---- kernel-level entry points (one batch of independent jobs; host arrays in/out) ---- */
 /* CSolExpCenter::build_IMat (src/expcenter.h:207, src/expcenter.cpp:667-902), the `pbsam mat`
  * stage for one sphere: from its exposed (npe) and buried (npb) surface points (3 doubles

@@ -1267,6 +1267,8 @@ struct GradState {
   PinBuf<double> h_dev;                    // per-step changes read back
 };
 
+struct BdState;  // Brownian-dynamics state of a batch (pbsam_host_bd.inc)
+
 struct pbsam_system {
   pbsam_ctx *c;
   int p, nmol, nsph = 0;
@@ -1315,6 +1317,8 @@ struct pbsam_system {
   std::vector<int> grp_rc;      // status of each group's last solve
   std::vector<int> grp_rounds;  // mutual rounds of each group's last solve
   std::vector<int> grp_passes;  // gradient passes of each group's last gradient phase
+  std::vector<char> grp_skip;   // groups left out of the next force evaluation (none when empty)
+  std::shared_ptr<BdState> bd;  // trajectories, when the batch runs Brownian dynamics
 };
 
 static void system_static(pbsam_system *S) {
@@ -1479,6 +1483,7 @@ static int system_lists(pbsam_system *S) {
   S->grp_passes.assign(S->ngrp, 0);
   for (int gi = 0; gi < S->ngrp; gi++) {
     const int m0 = S->grp_mol[gi], m1 = S->grp_mol[gi + 1];
+    if (!S->grp_skip.empty() && S->grp_skip[gi]) continue;  // no pairs: nothing to solve
     if (group_pairs(S, m0, m1) == PBSAM_OK) continue;
     // overlapping spheres: the group gets no pairs (computeForces returns false)
     if (S->ngrp == 1) return PBSAM_ERR_OVERLAP;
@@ -2663,3 +2668,5 @@ int pbsam_rotate_quat_batch(pbsam_ctx *c, int n, int p, const double *quat, int 
 }
 
 }  // extern "C"
+
+#include "pbsam_host_bd.inc"

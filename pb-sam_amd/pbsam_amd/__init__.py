@@ -34,7 +34,9 @@ SYMBOLS = [
     "pbsam_profile", "pbsam_profile_read", "pbsam_sync", "pbsam_gradient", "pbsam_forces",
     "pbsam_compute_forces", "pbsam_system_get_grad", "pbsam_xform_h_grad_batch",
     "pbsam_batch_create", "pbsam_batch_status", "pbsam_imat_build",
+    "pbsam_bd_defaults", "pbsam_bd_start", "pbsam_bd_contacts", "pbsam_bd_run", "pbsam_bd_get",
 ]
+BD_ESCAPED, BD_DOCKED, BD_RUNNING, BD_STUCK, BD_FAILED = range(5)
 
 KERNELS = {"xform": 0, "p2l": 1, "solve": 2, "surface": 3, "xgrad": 4, "surfgrad": 5}
 
@@ -49,6 +51,11 @@ class MolDesc(ctypes.Structure):
     _fields_ = [("n_sph", I), ("cen", P), ("rad", P), ("n_chg", I), ("chg_xyz", P),
                 ("chg_q", P), ("rcen", P), ("p", I), ("eps_in", D), ("npts", P),
                 ("npts_cap", I), ("intra_rcutoff", D), ("imats", P), ("imat_npts", I)]
+
+
+class BDParams(ctypes.Structure):
+    _fields_ = [("dtr", D), ("dr", D * 2), ("b_dist", D), ("q_dist", D), ("f_dist", D), ("fact_tr", D),
+                ("ikbt", D), ("coul_k", D), ("dt_fixed", D), ("max_tries", I)]
 
 
 _LIB = None
@@ -89,6 +96,11 @@ def load():
         "pbsam_batch_create": ([P, I, P, P, P, P, P], I),
         "pbsam_batch_status": ([P, P, P, P], I),
         "pbsam_imat_build": ([P, I, I, P, I, P, P], I),
+        "pbsam_bd_defaults": ([ctypes.POINTER(BDParams)], None),
+        "pbsam_bd_start": ([P, ctypes.POINTER(BDParams), P, I], I),
+        "pbsam_bd_contacts": ([P, I, P, P, P, P], I),
+        "pbsam_bd_run": ([P, I, P, P], I),
+        "pbsam_bd_get": ([P, I, P, P, P, P], I),
         "pbsam_system_destroy": ([P], None),
         "pbsam_system_set_pose": ([P, I, P, P], I),
         "pbsam_system_set_cutoffs": ([P, D, D], I),
@@ -390,3 +402,50 @@ class System:
         a, b = ctypes.c_int(), ctypes.c_int()
         self.ctx.check(self.ctx.L.pbsam_system_counts(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    # ---- Brownian dynamics: one two-molecule trajectory per group (CBDnam) ----
+    @staticmethod
+    def bd_params(**kw):
+        """CBDnam's parameters (pbsam_bd_defaults) with keyword overrides (dr as a pair)."""
+        prm = BDParams()
+        load().pbsam_bd_defaults(ctypes.byref(prm))
+        for k, v in kw.items():
+            if k == "dr":
+                prm.dr[0], prm.dr[1] = v
+            else:
+                setattr(prm, k, v)
+        return prm
+
+    def bd_start(self, seeds, place=True, params=None):
+        seeds = np.ascontiguousarray(seeds, dtype=np.int64)
+        if len(seeds) != len(self.groups):
+            raise ValueError("one seed per trajectory")
+        prm = params if params is not None else self.bd_params()
+        self._bd_prm = prm
+        self.ctx.check(self.ctx.L.pbsam_bd_start(self.h, ctypes.byref(prm), seeds.ctypes.data_as(P),
+                                                 1 if place else 0))
+
+    def bd_contacts(self, defs):
+        """defs: [(n_contact, [(k0, k1), ...], [dist, ...]), ...]"""
+        nc = np.array([d[0] for d in defs], dtype=np.int32)
+        npair = np.array([len(d[1]) for d in defs], dtype=np.int32)
+        pairs = np.ascontiguousarray(np.concatenate([np.asarray(d[1], np.int32).ravel() for d in defs])
+                                     if defs else np.zeros(0, np.int32), dtype=np.int32)
+        dist = f64(np.concatenate([np.asarray(d[2], float) for d in defs]) if defs else np.zeros(0))
+        self.ctx.check(self.ctx.L.pbsam_bd_contacts(self.h, len(defs), nc.ctypes.data_as(P),
+                                                    npair.ctypes.data_as(P), pairs.ctypes.data_as(P), _p(dist)))
+
+    def bd_run(self, n_steps):
+        """Advance every running trajectory by up to n_steps; returns (rc, status[g], steps[g])."""
+        n = len(self.groups)
+        st = np.zeros(n, dtype=np.int32)
+        steps = np.zeros(n, dtype=np.int64)
+        rc = self.ctx.L.pbsam_bd_run(self.h, int(n_steps), st.ctypes.data_as(P), steps.ctypes.data_as(P))
+        if rc not in (OK, ERR_NOCONV):
+            self.ctx.check(rc)
+        return rc, st, steps
+
+    def bd_get(self, g):
+        pos, quat, rot, dt = np.zeros(6), np.zeros(8), np.zeros(8), np.zeros(1)
+        self.ctx.check(self.ctx.L.pbsam_bd_get(self.h, int(g), _p(pos), _p(quat), _p(rot), _p(dt)))
+        return dict(pos=pos.reshape(2, 3), quat=quat.reshape(2, 4), rot=rot.reshape(2, 4), dt=dt[0])
