@@ -16,6 +16,9 @@ mutual solve alone (CMolecule::polarize_mutual, src/molecule.cpp:1989-2075), tim
 `batch` is the batched path: 256 Brownian-dynamics snapshots of the same two molecules
 (centre distance uniform in [50, 80] A) evaluated together through pbsam_batch_create,
 beside the plain one-system-per-call path timed on a sample of the same snapshots.
+`bd` is batched Brownian dynamics (CBDnam trajectories, one per system of a batch):
+1024 trajectories of two 100-sphere molecules at p=10, forces every step, reported as
+trajectory-steps/s beside the oracle's single trajectory on one host core.
 
 Prints one JSON line.  `roofline` is the kernel with the largest share of device time,
 timed with HIP events around each of its launches on the library's stream, with the
@@ -134,6 +137,89 @@ def batch_leg(pbsam_amd, synth, ctx, mols, mts, nsys, reps, nplain):
                                  gflop_per_batch=v["flops"] / reps / 1e9) for k, v in prof.items()})
 
 
+BD_SPH, BD_P, BD_NPTS, BD_SEEDS = 100, 10, 400, (301, 302)
+
+
+def bd_leg(pbsam_amd, synth, ctx, kappa, ntraj, nsteps, cpu_budget):
+    """Batched Brownian dynamics (pbsam_bd_*): `ntraj` trajectories of two 100-sphere molecules
+    (p=10, 400 points/sphere), D_t=0.01 A^2/ps, D_r=1e-4 rad^2/ps, fixed 2 ps steps, forces
+    and torques every step (f_dist beyond reach), CBDnam's cutoffs (10 A polarization, 100 A
+    interaction), starting near contact (centre distance 2R+6 A + U[0,10] A, random
+    orientations).  One warm-up step, then `nsteps` timed steps of every trajectory.  The
+    oracle runs trajectory 0 from the same start and seed on one host core for about
+    `cpu_budget` seconds; its poses after the first steps are compared with the device's."""
+    import numpy as np
+    rng = np.random.default_rng(BATCH_SEED + 7)
+    mols = [synth.molecule(BD_SPH, seed=s) for s in BD_SEEDS]
+    mts = []
+    for m in mols:
+        mt = pbsam_amd.MolType(ctx, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], BD_P, eps_in=synth.EPS_IN,
+                               rcen=m["rcen"], npts=[BD_NPTS] * BD_SPH)
+        mt.selfpol(tol=0, max_rounds=0)
+        mt.values()
+        mts.append(mt)
+    d0 = 2 * synth.ball_radius(BD_SPH) + 6.0
+    P, Q = [], []
+    for g in range(ntraj):
+        p, q = synth.place_pair(mols[0], mols[1], d0 + rng.uniform(0.0, 10.0), seed=BATCH_SEED + 100 + g)
+        P.append(p)
+        Q.append(q)
+    P, Q = np.vstack(P), np.vstack(Q)
+    batch = pbsam_amd.System(ctx, [mts[0], mts[1]] * ntraj, P, Q, groups=[2] * ntraj)
+    batch.set_cutoffs(10.0, 100.0)
+    prm = pbsam_amd.System.bd_params(dtr=0.01, dr=(1e-4, 1e-4), dt_fixed=2.0, f_dist=1e9)
+    seeds = np.arange(ntraj, dtype=np.int64) + 424242
+    batch.bd_start(seeds, place=False, params=prm)
+    batch.bd_run(1)
+    ctx.sync()
+    mark = roctx()
+    if mark:
+        mark.roctxRangePushA(b"bd_region")
+    t0 = time.perf_counter()
+    rc, st, steps = batch.bd_run(nsteps)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    if mark:
+        mark.roctxRangePop()
+    running = int((st == pbsam_amd.BD_RUNNING).sum())
+    out = dict(trajectories=ntraj, steps_timed=nsteps, traj_steps_per_s=ntraj * nsteps / el, s_per_step=el / nsteps,
+               running_after=running, status_counts=np.bincount(st, minlength=5).tolist(),
+               workload="%d BD trajectories x 2 molecules x %d spheres, p=%d, %d points/sphere, D_t=0.01, "
+                        "D_r=1e-4, dt=2 ps, forces every step, cutoffs 10/100 A, start 2R+6+U[0,10] A" %
+                        (ntraj, BD_SPH, BD_P, BD_NPTS))
+    if cpu_budget > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        orc = pyoracle.Oracle(kappa)
+        omts = []
+        for m, g in zip(mols, mts):
+            o = pyoracle.MolType(orc, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], BD_P, eps_in=synth.EPS_IN,
+                                 rcen=m["rcen"], npts=[BD_NPTS] * BD_SPH)
+            o.set_self(*g.FH())
+            o.values()
+            omts.append(o)
+        osys = pyoracle.System(orc, omts, P[0:2], Q[0:2])
+        osys.set_cutoffs(10.0, 100.0)
+        oprm = [prm.dtr, prm.dr[0], prm.dr[1], prm.b_dist, prm.q_dist, prm.f_dist, prm.fact_tr, prm.ikbt,
+                prm.coul_k, prm.dt_fixed, prm.max_tries]
+        b = pyoracle.BDTrajectory([(m["cen"] - m["rcen"], m["rad"]) for m in mols], int(seeds[0]), False, P[0:2],
+                                  Q[0:2], prm=oprm)
+        b.attach(osys)
+        n, t0, err = 0, time.perf_counter(), None
+        while True:
+            b.step()
+            n += 1
+            if n == 1 + nsteps:
+                err = float(np.abs(b.state()["pos"] - batch.bd_get(0)["pos"]).max())
+            tc = time.perf_counter() - t0
+            if n >= 1 + nsteps and tc >= cpu_budget:
+                break
+        out["cpu_baseline"] = dict(value=n / tc, unit="trajectory-steps/s", cores=1, kind="port",
+                                   sample="trajectory 0 of the same batch, %d steps in %.1f s on the oracle" % (n, tc))
+        out["max_abs_pos_diff_traj0_device_vs_oracle_A"] = err
+    return out
+
+
 def cpu_baseline(kappa, mols, mts, pos, quat, gpu_force, gpu_torque, budget_s):
     """The oracle (a scalar restatement of the reference, 1 thread) on the same molecules,
     self values and pose: complete force+torque evaluations (pair lists, mutual solve,
@@ -210,6 +296,8 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=2, help="timed batched calls")
     ap.add_argument("--plain-sample", type=int, default=8, help="systems timed one by one for the ratio")
     ap.add_argument("--solves", type=int, default=20, help="mutual solves timed for solves_per_s")
+    ap.add_argument("--bd", type=int, default=1024, help="trajectories in the Brownian-dynamics leg (0: skip it)")
+    ap.add_argument("--bd-steps", type=int, default=3, help="timed steps of the Brownian-dynamics leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -288,6 +376,12 @@ def main():
         print("batch of %d systems: %.1f evals/s (plain loop %.1f evals/s)" %
               (args.batch, batch["evals_per_s"], batch["plain_loop_evals_per_s"]), file=sys.stderr, flush=True)
 
+    bd = None
+    if args.bd > 0:
+        bd = bd_leg(pbsam_amd, synth, ctx, kappa, args.bd, args.bd_steps, 0.0 if args.no_cpu else 5.0)
+        print("Brownian dynamics, %d trajectories: %.1f trajectory-steps/s" % (args.bd, bd["traj_steps_per_s"]),
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(kappa, mols, mts, pos, quat, force, torque, args.cpu_budget)
@@ -314,6 +408,7 @@ def main():
         },
         "solves_per_s": solves_per_s,
         "batch": batch,
+        "bd": bd,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -13,6 +13,8 @@
 #include <string>
 #include <vector>
 
+#include <thread>
+
 #include "../../include/pbsam_amd.h"
 #include "pbsam_kernels.hip"
 
@@ -101,6 +103,23 @@ struct PinBuf {
     return p;
   }
 };
+
+// fn(i) for i in [0, n) on up to 16 host threads (serially below `grain` items per thread)
+template <class F>
+static void host_parallel(int n, int grain, F fn) {
+  int nt = std::min<int>(16, (int)std::max(1u, std::thread::hardware_concurrency()));
+  nt = std::min(nt, n / std::max(grain, 1));
+  if (nt <= 1) {
+    for (int i = 0; i < n; i++) fn(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int w = 0; w < nt; w++)
+    th.emplace_back([&, w]() {
+      for (int i = w; i < n; i += nt) fn(i);
+    });
+  for (std::thread &t : th) t.join();
+}
 
 inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 inline void cross3(const double *a, const double *b, double *o) {
@@ -1259,6 +1278,8 @@ struct GradState {
   DBuf<TJob> tj, dtj;
   DBuf<RAJob> ra, ra2;
   DBuf<int> ia, ib, ic, tqmap;             // scratch lists of the self-value batches
+  DBuf<int> dpairs;                        // (source, target) sphere pairs of a re-expansion batch
+  DBuf<double> lab, rad;                   // spheres' lab centres and radii
   DBuf<int> is_off, is_lst, ps_off, ps_lst, ident, esph, eitem, sbuf;
   DBuf<MJob> mj;
   DBuf<double> mpart;
@@ -1481,10 +1502,15 @@ static int system_lists(pbsam_system *S) {
   S->grp_rc.assign(S->ngrp, PBSAM_OK);
   S->grp_rounds.assign(S->ngrp, 0);
   S->grp_passes.assign(S->ngrp, 0);
+  // groups touch only their own molecules: their pair lists are built concurrently
+  std::vector<int> grc(S->ngrp, PBSAM_OK);
+  host_parallel(S->ngrp, 16, [&](int gi) {
+    if (!S->grp_skip.empty() && S->grp_skip[gi]) return;  // no pairs: nothing to solve
+    grc[gi] = group_pairs(S, S->grp_mol[gi], S->grp_mol[gi + 1]);
+  });
   for (int gi = 0; gi < S->ngrp; gi++) {
     const int m0 = S->grp_mol[gi], m1 = S->grp_mol[gi + 1];
-    if (!S->grp_skip.empty() && S->grp_skip[gi]) continue;  // no pairs: nothing to solve
-    if (group_pairs(S, m0, m1) == PBSAM_OK) continue;
+    if (grc[gi] == PBSAM_OK) continue;
     // overlapping spheres: the group gets no pairs (computeForces returns false)
     if (S->ngrp == 1) return PBSAM_ERR_OVERLAP;
     S->grp_rc[gi] = PBSAM_ERR_OVERLAP;

@@ -151,7 +151,42 @@ struct GArgs {
   unsigned long long *work;
 };
 
-__global__ void __launch_bounds__(GT) k_xgrad(Tabs tb, GArgs A) {
+// Job tables of a batch of gradient re-expansions, one (source, target) sphere pair each:
+// pair k gets translation matrices at T + 2*TS*k and output slot k.  P is the target's lab
+// centre minus the source's, as the host forms it.
+__global__ void __launch_bounds__(256) k_dta_jobs(int n, const int *pairs, const double *lab, const double *rad,
+                                                  double kappa, double *T, long long TS, TJob *tj, XJob *xj,
+                                                  int *lst) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const int src = pairs[2 * k], tgt = pairs[2 * k + 1];
+  XJob J{};
+  TJob M{};
+  for (int x = 0; x < 3; x++) {
+    const double d = lab[3 * tgt + x] - lab[3 * src + x];
+    J.P[x] = d;
+    M.P[x] = d;
+  }
+  J.src = src;
+  J.tgt = tgt;
+  J.slot = k;
+  J.kind = 0;
+  J.TH = T + (size_t)k * 2 * TS;
+  J.a_src = M.a_src = rad[src];
+  J.a_tgt = M.a_tgt = rad[tgt];
+  M.kappa_seed = kappa;
+  M.off = (long long)k * 2 * TS;
+  M.grad = 1;
+  M.src = src;
+  tj[k] = M;
+  xj[k] = J;
+  lst[k] = k;
+}
+
+// NT threads per pair; QMAX bounds the order (each thread owns up to two (n, s) outputs,
+// so q (q + 1) / 2 <= 2 NT)
+template <int NT, int QMAX>
+__global__ void __launch_bounds__(NT) k_xgrad(Tabs tb, GArgs A) {
   extern __shared__ double sh[];
   const XJob J = A.jobs[blockIdx.x];
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
@@ -176,8 +211,8 @@ __global__ void __launch_bounds__(GT) k_xgrad(Tabs tb, GArgs A) {
   cd *b3 = b2 + q;                             // T (dP M)
   cd *R = b3 + q;                              // ROT level m, row n at n^2 + n + s
   cd *dR = R + NR * NR;                        // dROT level m
-  for (int i = tid; i < q * q; i += GT) hs[i] = A.H[(size_t)J.src * P2 + i];
-  for (int i = tid; i < 3 * P2; i += GT) G[(i / P2) * P2 + i % P2] = 0.0;
+  for (int i = tid; i < q * q; i += NT) hs[i] = A.H[(size_t)J.src * P2 + i];
+  for (int i = tid; i < 3 * P2; i += NT) G[(i / P2) * P2 + i % P2] = 0.0;
   // CRotCoeff::initParams (rotcoeff.cpp:209-244) with xi = pi
   double sint = sin(th), cost, cott = 0.0;
   int sing = 0;
@@ -221,30 +256,30 @@ __global__ void __launch_bounds__(GT) k_xgrad(Tabs tb, GArgs A) {
     // six scratch blocks in place of the rotation buffers
     double *t1 = reinterpret_cast<double *>(R), *t2 = t1 + q * q, *Gr = t2 + q * q, *Gt = Gr + q * q,
            *Gp = Gt + q * q, *t3 = Gp + q * q;
-    sing_rot(hs, t1, q, sing, tid, GT);                      // tM1 = R M
+    sing_rot(hs, t1, q, sing, tid, NT);                      // tM1 = R M
     __syncthreads();
-    blk_translate(dT, tq, t1, t2, q, tid, GT);               // dT tM1
+    blk_translate(dT, tq, t1, t2, q, tid, NT);               // dT tM1
     __syncthreads();
-    sing_rot(t2, Gr, q, sing, tid, GT);                      // Gr = R^H dT R M
+    sing_rot(t2, Gr, q, sing, tid, NT);                      // Gr = R^H dT R M
     __syncthreads();
-    blk_translate(T, tq, t1, t2, q, tid, GT);                // tL = T R M
+    blk_translate(T, tq, t1, t2, q, tid, NT);                // tL = T R M
     __syncthreads();
-    sing_dT(tb, t2, Gt, q, sing, false, tid, GT);            // Gt = dR^H tL
-    sing_dP(tb, t2, Gp, q, sing, false, tid, GT);            // Gp = dP^H tL
-    sing_dT(tb, hs, t1, q, sing, true, tid, GT);             // dR M
+    sing_dT(tb, t2, Gt, q, sing, false, tid, NT);            // Gt = dR^H tL
+    sing_dP(tb, t2, Gp, q, sing, false, tid, NT);            // Gp = dP^H tL
+    sing_dT(tb, hs, t1, q, sing, true, tid, NT);             // dR M
     __syncthreads();
-    blk_translate(T, tq, t1, t2, q, tid, GT);
+    blk_translate(T, tq, t1, t2, q, tid, NT);
     __syncthreads();
-    sing_rot(t2, t3, q, sing, tid, GT);
+    sing_rot(t2, t3, q, sing, tid, NT);
     __syncthreads();
-    for (int i = tid; i < q * q; i += GT) Gt[i] += t3[i];
-    sing_dP(tb, hs, t1, q, sing, true, tid, GT);             // dP M
+    for (int i = tid; i < q * q; i += NT) Gt[i] += t3[i];
+    sing_dP(tb, hs, t1, q, sing, true, tid, NT);             // dP M
     __syncthreads();
-    blk_translate(T, tq, t1, t2, q, tid, GT);
+    blk_translate(T, tq, t1, t2, q, tid, NT);
     __syncthreads();
-    sing_rot(t2, t3, q, sing, tid, GT);
+    sing_rot(t2, t3, q, sing, tid, NT);
     __syncthreads();
-    for (int i = tid; i < q * q; i += GT) {
+    for (int i = tid; i < q * q; i += NT) {
       Gp[i] += t3[i];
       const double ar[3] = {Gr[i], Gt[i], Gp[i]};
       for (int d = 0; d < 3; d++) G[d * P2 + i] = JR[d][0] * ar[0] + JR[d][1] * ar[1] + JR[d][2] * ar[2];
@@ -253,7 +288,7 @@ __global__ void __launch_bounds__(GT) k_xgrad(Tabs tb, GArgs A) {
   }
 
   // level-0 seeds: ROT(n,s,0) = conj-ordered SH of order 2q-1 (rotcoeff.cpp:785-795)
-  for (int m = tid; m < NR; m += GT) {
+  for (int m = tid; m < NR; m += NT) {
     sh_column(tb, NR, m, cost, sint, ph, [&](int n, double re, double im) {
       if (m == 0) {
         R[n * n + n] = {re, 0.0};
@@ -265,7 +300,7 @@ __global__ void __launch_bounds__(GT) k_xgrad(Tabs tb, GArgs A) {
   }
   __syncthreads();
   // dROT level 0 (rotcoeff.cpp:818-836); sh_comp(n, s) = ROT(n, -s, 0) for s >= 0
-  for (int n = tid; n < NR; n += GT) {
+  for (int n = tid; n < NR; n += NT) {
     if (n == 0) {
       dR[0] = {0.0, 0.0};
       continue;
@@ -289,7 +324,7 @@ __global__ void __launch_bounds__(GT) k_xgrad(Tabs tb, GArgs A) {
   int on[2], os[2];
   cd aGr[2], aGt1[2], aGt2[2], aGp1[2], aGp2[2];
   for (int k = 0; k < 2; k++) {
-    int e = tid + k * GT;
+    int e = tid + k * NT;
     on[k] = -1;
     os[k] = 0;
     aGr[k] = aGt1[k] = aGt2[k] = aGp1[k] = aGp2[k] = {0.0, 0.0};
@@ -303,13 +338,13 @@ __global__ void __launch_bounds__(GT) k_xgrad(Tabs tb, GArgs A) {
   }
   const double r1 = -0.5 * (1 + cost), r2 = 0.5 * (1 - cost), r3 = -sint;
   const double dr1 = 0.5 * sint, dr2 = dr1, dr3 = -cost;
-  constexpr int NV = ((2 * PMAX - 2) * (2 * PMAX - 2) + GT - 1) / GT;  // staged values per thread
+  constexpr int NV = ((2 * QMAX - 2) * (2 * QMAX - 2) + NT - 1) / NT;  // staged values per thread
   __syncthreads();
   for (int m = 0; m < q; m++) {
     // (a) forward rotations of this level: R M, dR M, dP M for rows n in [m, q)
     {
       const int g = tid / XG, lane = tid % XG;
-      for (int n = m + g; n < q; n += GT / XG) {
+      for (int n = m + g; n < q; n += NT / XG) {
         const cd *row = R + n * n + n, *drow = dR + n * n + n;
         cd s0 = {0, 0}, s1 = {0, 0}, s2 = {0, 0};
         for (int s = lane; s <= n; s += XG) {
@@ -341,7 +376,7 @@ __global__ void __launch_bounds__(GT) k_xgrad(Tabs tb, GArgs A) {
     {
       const int g = tid / XG, lane = tid % XG;
       const int w = tq - m;
-      for (int n = m + g; n < q; n += GT / XG) {
+      for (int n = m + g; n < q; n += NT / XG) {
         const double *Tm = T + tmat_off(tq, m) + (size_t)(n - m) * w;
         const double *dTm = dT + tmat_off(tq, m) + (size_t)(n - m) * w;
         cd t0 = {0, 0}, t1 = {0, 0}, t2 = {0, 0}, t3 = {0, 0};
@@ -393,7 +428,7 @@ __global__ void __launch_bounds__(GT) k_xgrad(Tabs tb, GArgs A) {
       cd vR[NV], vD[NV];
       int vi[NV];
       int cnt = 0;
-      for (int e = tid; e < total; e += GT, cnt++) {
+      for (int e = tid; e < total; e += NT, cnt++) {
         int t = e + base;
         int n = (int)sqrt((double)t);
         while (n * n > t) n--;
