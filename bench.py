@@ -16,9 +16,9 @@ mutual solve alone (CMolecule::polarize_mutual, src/molecule.cpp:1989-2075), tim
 `batch` is the batched path: 256 Brownian-dynamics snapshots of the same two molecules
 (centre distance uniform in [50, 80] A) evaluated together through pbsam_batch_create,
 beside the plain one-system-per-call path timed on a sample of the same snapshots.
-`bd` is batched Brownian dynamics (CBDnam trajectories, one per system of a batch):
-1024 trajectories of two 100-sphere molecules at p=10, forces every step, reported as
-trajectory-steps/s beside the oracle's single trajectory on one host core.
+`bd` (with --bd 1024) is batched Brownian dynamics (CBDnam trajectories, one per system of a
+batch): 1024 trajectories of two 100-sphere molecules at p=10, forces every step, reported as
+trajectory-steps/s (with --bd-cpu, beside the oracle's single trajectory on one host core).
 
 Prints one JSON line.  `roofline` is the kernel with the largest share of device time,
 timed with HIP events around each of its launches on the library's stream, with the
@@ -27,6 +27,7 @@ reference's loop bounds at the truncation orders actually used).  `cpu_baseline`
 CPU oracle (a scalar restatement of the reference, 1 thread) on the same system.
 """
 import argparse
+import faulthandler
 import json
 import os
 import sys
@@ -46,7 +47,8 @@ PEAK_HBM = 8.0e12        # MI355X HBM3E, B/s
 # dispatches from the system's pair lists on), each scaled by the same counter for one
 # 256 MiB device-to-device copy (FETCH_SIZE reads half the bytes: x2.00; WRITE_SIZE x1.00),
 # averaged over the group's dispatches in one evaluation.
-TRAFFIC = {"solve": 185587, "xform": 2402283, "surfgrad": 13526132, "p2l": 36640290}
+TRAFFIC = {"solve": 226495, "xform": 2415558, "surfgrad": 13528711, "p2l": 36637552, "surface": 23238426,
+           "xgrad": 166077744}
 
 
 def roctx():
@@ -296,11 +298,15 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=2, help="timed batched calls")
     ap.add_argument("--plain-sample", type=int, default=8, help="systems timed one by one for the ratio")
     ap.add_argument("--solves", type=int, default=20, help="mutual solves timed for solves_per_s")
-    ap.add_argument("--bd", type=int, default=1024, help="trajectories in the Brownian-dynamics leg (0: skip it)")
+    ap.add_argument("--bd", type=int, default=0,
+                    help="trajectories in the Brownian-dynamics leg (0: skip it; 1024 is the BD workload)")
     ap.add_argument("--bd-steps", type=int, default=3, help="timed steps of the Brownian-dynamics leg")
+    ap.add_argument("--bd-cpu", action="store_true",
+                    help="also time the oracle's trajectory 0 in the Brownian-dynamics leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    faulthandler.enable()  # a crash prints the Python stack
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -378,7 +384,8 @@ def main():
 
     bd = None
     if args.bd > 0:
-        bd = bd_leg(pbsam_amd, synth, ctx, kappa, args.bd, args.bd_steps, 0.0 if args.no_cpu else 5.0)
+        # the oracle's trajectory runs only with --bd-cpu (see the Brownian-dynamics notes)
+        bd = bd_leg(pbsam_amd, synth, ctx, kappa, args.bd, args.bd_steps, 5.0 if args.bd_cpu else 0.0)
         print("Brownian dynamics, %d trajectories: %.1f trajectory-steps/s" % (args.bd, bd["traj_steps_per_s"]),
               file=sys.stderr, flush=True)
 

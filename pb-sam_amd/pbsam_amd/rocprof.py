@@ -3,7 +3,7 @@
     python -m pbsam_amd.rocprof <run_kernel_trace.csv> [run_marker_api_trace.csv [bench.json]]
 
 Prints a Markdown table of per-kernel dispatch statistics over the whole trace and, when
-the marker trace holds bench.py's "timed_region" (and "batch_region") range, over the
+the marker trace holds bench.py's "timed_region" (and "batch_region", "bd_region") range, over the
 dispatches that ran inside it; with bench.py's JSON line it also lists bench.py's own HIP-event averages of
 the instrumented kernels beside rocprofv3's.
 
@@ -22,8 +22,8 @@ import re
 import sys
 from collections import defaultdict
 
-INSTRUMENTED = {"xform": ("k_xform",), "p2l": ("k_p2l", "k_p2lgemv"), "solve": ("k_solve", "k_solve3"),
-                "surface": ("k_surface",), "xgrad": ("k_xgrad",), "surfgrad": ("k_surfgrad", "k_surfgemv")}
+INSTRUMENTED = {"xform": ("k_xform",), "p2l": ("k_p2l", "k_p2lgemv", "k_p2lgemm"), "solve": ("k_solve", "k_solve3"),
+                "surface": ("k_surface",), "xgrad": ("k_xgrad",), "surfgrad": ("k_surfgemm", "k_surfgemm_k4")}
 
 
 def short(name):
@@ -69,10 +69,11 @@ def main(argv):
     table(rows, "All dispatches of the run")
     if len(argv) < 3:
         return
-    brng = find_range(argv[2], "batch_region")
-    if brng is not None:
-        inb = [r for r in rows if r[1] >= brng[0] and r[2] <= brng[1]]
-        table(inb, "Dispatches inside bench.py's batched leg (%.3f ms of wall time)" % ((brng[1] - brng[0]) / 1e6))
+    for label, what in (("batch_region", "bench.py's batched leg"), ("bd_region", "bench.py's Brownian-dynamics leg")):
+        brng = find_range(argv[2], label)
+        if brng is not None:
+            inb = [r for r in rows if r[1] >= brng[0] and r[2] <= brng[1]]
+            table(inb, "Dispatches inside %s (%.3f ms of wall time)" % (what, (brng[1] - brng[0]) / 1e6))
     rng = find_range(argv[2], "timed_region")
     if rng is None:
         print("no timed_region range in the marker trace")
