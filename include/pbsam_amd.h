@@ -217,6 +217,37 @@ int pbsam_bd_run(pbsam_system *batch, int n_steps, int *status, long long *steps
  * rotations, not renormalized), stored rotations m_rot1/m_rot2 (8), the last time step */
 int pbsam_bd_get(const pbsam_system *batch, int g, double *pos, double *quat, double *rot, double *dt);
 
+/* ---- the reference's file formats (src/readutil.cpp; host only, no device needed) ----
+ * Errors are PBSAM_ERR_BADARG (unreadable file, a matrix of lower order than asked) where the
+ * reference exits. */
+/* readpqr (src/readutil.cpp:285-355): "ATOM" records, x y z charge radius from column 31;
+ * "CEN" at column 18 marks a dielectric sphere (centre, radius), other records are charges
+ * (position, charge).  Call with NULL arrays for the counts, then with arrays of at least
+ * that many (*n_sph, *n_chg give their sizes).  err (err_len bytes, may be NULL) gets a message. */
+int pbsam_read_pqr(const char *path, int *n_sph, double *cen, double *rad, int *n_chg, double *chg_xyz,
+                   double *chg_q, char *err, int err_len);
+/* writeMat_Unformatted / readMat_Unformatted (src/readutil.cpp:378-435): an int pole order,
+ * then p^4 doubles (the p^2 x p^2 matrix, column-major).  Reading order p from a file of
+ * order p_file >= p keeps the leading p^2 rows of the leading p^2 columns. */
+int pbsam_write_imat(const char *path, int p, const double *imat);
+int pbsam_read_imat(const char *path, int p, double *imat, int *p_file);
+/* writeExpansion / readExpansion (src/readutil.cpp:541-595): text, pole order, scale, cutoff,
+ * then the p^2 coefficients (precision 8, one line per order); a file of lower order is
+ * padded with zeros, one of higher order cut */
+int pbsam_write_expansion(const char *path, int p, double scale, double rcut, const double *coef);
+int pbsam_read_expansion(const char *path, int p, double *scale, double *rcut, double *coef, int *p_file);
+/* readConfigWithMolType (src/readutil.cpp:835-886) and CMolecule::saveConfig
+ * (src/molecule.cpp:254-270): lines "x y z w qx qy qz moltype"; read quaternions are
+ * normalized.  pbsam_read_config: NULL arrays for the count first. */
+int pbsam_read_config(const char *path, int *n, double *pos, double *quat, int *moltype);
+int pbsam_write_config(const char *path, int n, const double *pos, const double *quat, const int *moltype);
+/* readMats and readExpansionsP (src/readutil.cpp:493-506,603-630): a molecule type's files in
+ * the reference's layout, <dir>/imat.sp<k>.out.bin and <dir>/<run>.<k>.{F,H}.exp, k < ns.
+ * imats: ns * p^4; F, H: ns * p^2; intra_rcutoff: the cutoff the expansion files carry. */
+int pbsam_read_imats(const char *dir, int ns, int p, double *imats);
+int pbsam_read_expansions(const char *dir, const char *run, int ns, int p, double *F, double *H,
+                          double *intra_rcutoff);
+
 /* ---- kernel-level entry points (one batch of independent jobs; host arrays in/out) ---- */
 /* CSolExpCenter::build_IMat (src/expcenter.h:207, src/expcenter.cpp:667-902), the `pbsam mat`
  * stage for one sphere: from its exposed (npe) and buried (npb) surface points (3 doubles

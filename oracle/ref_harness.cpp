@@ -17,6 +17,7 @@
 #include <new>
 
 #include "BDnam.h"
+#include "readutil.h"
 #include "contact.h"
 #include "expansion.h"
 #include "molecule.h"
@@ -387,6 +388,77 @@ int main() {
         res.insert(res.end(), b.begin(), b.end());
         res.insert(res.end(), c2.begin(), c2.end());
         res.insert(res.end(), d.begin(), d.end());
+      }
+    } else if (c == "pqr") {  // path -> ncen (x y z)*ncen rad*ncen natom (x y z)*natom q*natom
+      char path[4096];
+      if (scanf("%4095s", path) != 1) return 3;
+      std::vector<CPnt> pnt, cen;
+      std::vector<REAL> ch, rad;
+      readpqr(path, pnt, ch, rad, cen);
+      res.push_back(cen.size());
+      for (size_t i = 0; i < cen.size(); i++) {
+        res.push_back(cen[i].x());
+        res.push_back(cen[i].y());
+        res.push_back(cen[i].z());
+      }
+      res.insert(res.end(), rad.begin(), rad.end());
+      res.push_back(pnt.size());
+      for (size_t i = 0; i < pnt.size(); i++) {
+        res.push_back(pnt[i].x());
+        res.push_back(pnt[i].y());
+        res.push_back(pnt[i].z());
+      }
+      res.insert(res.end(), ch.begin(), ch.end());
+    } else if (c == "wrmat") {  // path p values(p^4) -> 0
+      char path[4096];
+      if (scanf("%4095s", path) != 1) return 3;
+      int p = ri();
+      std::vector<double> m = rdv(p * p * p * p);
+      writeMat_Unformatted(&m[0], p, path);
+      res.push_back(0);
+    } else if (c == "rdmat") {  // path maxp -> maxp^4 values
+      char path[4096];
+      if (scanf("%4095s", path) != 1) return 3;
+      int p = ri();
+      std::vector<double> m(p * p * p * p, 0.0);
+      readMat_Unformatted(&m[0], p, path);
+      res = m;
+    } else if (c == "wrexp") {  // path p scale rcut values(p^2) -> 0
+      char path[4096];
+      if (scanf("%4095s", path) != 1) return 3;
+      int p = ri();
+      double scale = rd(), rcut = rd();
+      std::vector<double> v = rdv(p * p);
+      CMulExpan E(&v[0], CRange(0, p), scale);
+      writeExpansion(E, rcut, path);
+      res.push_back(0);
+    } else if (c == "rdexp") {  // path maxp -> scale rcut values(maxp^2)
+      char path[4096];
+      if (scanf("%4095s", path) != 1) return 3;
+      int p = ri();
+      std::vector<double> v;
+      REAL scale = 0, rcut = 0;
+      readExpansion(v, p, path, scale, rcut);
+      res.push_back(scale);
+      res.push_back(rcut);
+      res.insert(res.end(), v.begin(), v.end());
+    } else if (c == "rdcfg") {  // path -> n (x y z w qx qy qz type)*n
+      char path[4096];
+      if (scanf("%4095s", path) != 1) return 3;
+      std::vector<CPnt> rc;
+      std::vector<CQuat> rots;
+      std::vector<int> ids, types, nps;
+      readConfigWithMolType(path, rc, rots, ids, types, nps);
+      res.push_back(rc.size());
+      for (size_t i = 0; i < rc.size(); i++) {
+        res.push_back(rc[i].x());
+        res.push_back(rc[i].y());
+        res.push_back(rc[i].z());
+        res.push_back(rots[i].w());
+        res.push_back(rots[i].x());
+        res.push_back(rots[i].y());
+        res.push_back(rots[i].z());
+        res.push_back(ids[i]);
       }
     } else if (c == "bd") {
       // seed place nsteps Dtr Dr1 Dr2 ft(9) | molecule 0 | molecule 1 | [pos0 q0 pos1 q1 if !place]

@@ -2696,3 +2696,4 @@ int pbsam_rotate_quat_batch(pbsam_ctx *c, int n, int p, const double *quat, int 
 }  // extern "C"
 
 #include "pbsam_host_bd.inc"
+#include "pbsam_host_io.inc"

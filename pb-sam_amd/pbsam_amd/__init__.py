@@ -35,6 +35,8 @@ SYMBOLS = [
     "pbsam_compute_forces", "pbsam_system_get_grad", "pbsam_xform_h_grad_batch",
     "pbsam_batch_create", "pbsam_batch_status", "pbsam_imat_build",
     "pbsam_bd_defaults", "pbsam_bd_start", "pbsam_bd_contacts", "pbsam_bd_run", "pbsam_bd_get",
+    "pbsam_read_pqr", "pbsam_write_imat", "pbsam_read_imat", "pbsam_write_expansion", "pbsam_read_expansion",
+    "pbsam_read_config", "pbsam_write_config", "pbsam_read_imats", "pbsam_read_expansions",
 ]
 BD_ESCAPED, BD_DOCKED, BD_RUNNING, BD_STUCK, BD_FAILED = range(5)
 
@@ -101,6 +103,15 @@ def load():
         "pbsam_bd_contacts": ([P, I, P, P, P, P], I),
         "pbsam_bd_run": ([P, I, P, P], I),
         "pbsam_bd_get": ([P, I, P, P, P, P], I),
+        "pbsam_read_pqr": ([ctypes.c_char_p, P, P, P, P, P, P, ctypes.c_char_p, I], I),
+        "pbsam_write_imat": ([ctypes.c_char_p, I, P], I),
+        "pbsam_read_imat": ([ctypes.c_char_p, I, P, P], I),
+        "pbsam_write_expansion": ([ctypes.c_char_p, I, D, D, P], I),
+        "pbsam_read_expansion": ([ctypes.c_char_p, I, P, P, P, P], I),
+        "pbsam_read_config": ([ctypes.c_char_p, P, P, P, P], I),
+        "pbsam_write_config": ([ctypes.c_char_p, I, P, P, P], I),
+        "pbsam_read_imats": ([ctypes.c_char_p, I, I, P], I),
+        "pbsam_read_expansions": ([ctypes.c_char_p, ctypes.c_char_p, I, I, P, P, P], I),
         "pbsam_system_destroy": ([P], None),
         "pbsam_system_set_pose": ([P, I, P, P], I),
         "pbsam_system_set_cutoffs": ([P, D, D], I),
@@ -128,6 +139,85 @@ def load():
         f.restype = res
     _LIB = L
     return L
+
+
+# ---- the reference's file formats (host only) ----
+def _io_check(rc, what):
+    if rc != OK:
+        raise PbsamError(rc, what)
+
+
+def read_pqr(path):
+    """readpqr: (sphere centres (n,3), radii (n,), charge positions (m,3), charges (m,))"""
+    L = load()
+    ns, nq = ctypes.c_int(0), ctypes.c_int(0)
+    err = ctypes.create_string_buffer(256)
+    _io_check(L.pbsam_read_pqr(os.fsencode(path), ctypes.byref(ns), None, None, ctypes.byref(nq), None, None,
+                               err, 256), err.value.decode() or str(path))
+    cen, rad = np.zeros(3 * ns.value), np.zeros(ns.value)
+    xyz, q = np.zeros(3 * nq.value), np.zeros(nq.value)
+    _io_check(L.pbsam_read_pqr(os.fsencode(path), ctypes.byref(ns), _p(cen), _p(rad), ctypes.byref(nq), _p(xyz),
+                               _p(q), err, 256), err.value.decode() or str(path))
+    return cen.reshape(-1, 3), rad, xyz.reshape(-1, 3), q
+
+
+def write_imat(path, imat, p):
+    m = f64(imat).reshape(-1)
+    if m.size != p ** 4:
+        raise ValueError("an IMat of order p holds p^4 values")
+    _io_check(load().pbsam_write_imat(os.fsencode(path), p, _p(m)), str(path))
+
+
+def read_imat(path, p):
+    """(p^2 x p^2 column-major matrix as p^4 values, the file's pole order)"""
+    m, pf = np.zeros(p ** 4), ctypes.c_int(0)
+    _io_check(load().pbsam_read_imat(os.fsencode(path), p, _p(m), ctypes.byref(pf)), str(path))
+    return m, pf.value
+
+
+def write_expansion(path, coef, p, scale, rcut):
+    c = f64(coef).reshape(-1)
+    if c.size < p * p:
+        raise ValueError("an expansion of order p holds p^2 values")
+    _io_check(load().pbsam_write_expansion(os.fsencode(path), p, scale, rcut, _p(c)), str(path))
+
+
+def read_expansion(path, p):
+    """(coefficients (p^2,), scale, rcut, the file's pole order)"""
+    c, sc, rc, pf = np.zeros(p * p), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    _io_check(load().pbsam_read_expansion(os.fsencode(path), p, ctypes.byref(sc), ctypes.byref(rc), _p(c),
+                                          ctypes.byref(pf)), str(path))
+    return c, sc.value, rc.value, pf.value
+
+
+def read_config(path):
+    """(positions (n,3), quaternions (n,4) w x y z, molecule types (n,))"""
+    L = load()
+    n = ctypes.c_int(0)
+    _io_check(L.pbsam_read_config(os.fsencode(path), ctypes.byref(n), None, None, None), str(path))
+    pos, quat, t = np.zeros(3 * n.value), np.zeros(4 * n.value), np.zeros(n.value, dtype=np.int32)
+    _io_check(L.pbsam_read_config(os.fsencode(path), ctypes.byref(n), _p(pos), _p(quat), _p(t)), str(path))
+    return pos.reshape(-1, 3), quat.reshape(-1, 4), t
+
+
+def write_config(path, pos, quat, moltype):
+    pos, quat = f64(pos).reshape(-1), f64(quat).reshape(-1)
+    t = np.ascontiguousarray(moltype, dtype=np.int32)
+    _io_check(load().pbsam_write_config(os.fsencode(path), len(t), _p(pos), _p(quat), _p(t)), str(path))
+
+
+def read_imats(directory, ns, p):
+    m = np.zeros(ns * p ** 4)
+    _io_check(load().pbsam_read_imats(os.fsencode(directory), ns, p, _p(m)), str(directory))
+    return m.reshape(ns, p ** 4)
+
+
+def read_expansions(directory, run, ns, p):
+    """(F (ns,p^2), H (ns,p^2), intra cutoff) from <dir>/<run>.<k>.{F,H}.exp"""
+    F, H, rc = np.zeros(ns * p * p), np.zeros(ns * p * p), ctypes.c_double()
+    _io_check(load().pbsam_read_expansions(os.fsencode(directory), run.encode(), ns, p, _p(F), _p(H),
+                                           ctypes.byref(rc)), str(directory))
+    return F.reshape(ns, -1), H.reshape(ns, -1), rc.value
 
 
 def kappa_from_salt(salt_molar, eps_out=78.0, temp=298.15):
