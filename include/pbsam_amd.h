@@ -167,6 +167,23 @@ int pbsam_compute_forces(pbsam_system *sys, double tol, int max_rounds, double *
  * (3 p^2), as left by the last pbsam_gradient */
 int pbsam_system_get_grad(const pbsam_system *sys, int mol, int k, double *gH, double *gLHN);
 
+/* ---- energy and potential queries ----
+ * pbsam_potential: CMolecule::computePotInSpace (src/moldynamics.cpp:469-507) at n points
+ * (xyz: 3n, lab frame, pot: n): the sum over every molecule and sphere of the sphere's H
+ * contracted with the screened local expansion of a unit charge at the point (order
+ * computeOrder(rho, TQH, a)), from the current state (self values before any solve).
+ * pbsam_total_int_energy: CMolecule::computeTotalIntEnergy (src/moldynamics.cpp:358-386 with
+ * interactCenters_LowMemory, src/molecule.cpp:1851-1890), one value per system of the handle
+ * (e: n_sys): for every molecule pair, each sphere's H with the re-expansion of the other
+ * molecule's H onto it.  The reference forms both with the spheres' unrotated centres and
+ * molecule-frame blocks, which is right only at the identity orientation; by default the
+ * library uses the rotated centres (and lab-frame blocks), and PBSAM_POT_REFERENCE_FRAME asks
+ * for the reference's formula exactly.  The reference's aggregated molecular multipoles (an
+ * optional shortcut for far points, off by default there) are not used. */
+#define PBSAM_POT_REFERENCE_FRAME 1u
+int pbsam_potential(pbsam_system *sys, int n, const double *xyz, double *pot, unsigned flags);
+int pbsam_total_int_energy(pbsam_system *sys, unsigned flags, double *e);
+
 /* ---- Brownian dynamics of two-molecule trajectories (CBDnam, src/BDnam.cpp:166-417) ----
  * A batch made by pbsam_batch_create with two molecules in every system runs one trajectory
  * per system.  A step of a running trajectory is one pass of CBDnam::run's loop

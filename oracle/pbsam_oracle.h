@@ -22,8 +22,10 @@
  *   orc_gradient / orc_forces / orc_compute_forces: central differences of the energy;
  *   orc_bd_*: drand48 against the C library's, and whole trajectories against the
  *     reference's own CBDnam::makeMove / compute_dt compiled from its sources.
- * Parity unpinned: orc_system_total_int_energy (not on the product's ABI), and the
- * Jacobi variants (orc_selfpol_round_jacobi, schedule 1 of orc_selfpol / orc_mutual /
+ *   orc_system_total_int_energy, orc_pot_in_space: the reference's own computeTotalIntEnergy
+ *     and computePotInSpace compiled from its sources (tests/golden/pot_ref.npz), and the
+ *     rotation-correct potential by invariance under a rotation of the whole system.
+ * Parity unpinned: the Jacobi variants (orc_selfpol_round_jacobi, schedule 1 of orc_selfpol / orc_mutual /
  * orc_gradient), which have no reference counterpart and serve only to check the
  * device's Jacobi rounds against the same restatement.
  */
@@ -138,6 +140,8 @@ void orc_system_get(const orc_system *s, int i, int k, double *F, double *H, dou
 void orc_system_get_grad(const orc_system *s, int i, int k, double *gH, double *gLHN);
 double orc_system_pot(const orc_system *s, int i);
 double orc_system_total_int_energy(orc_system *s);
+/* computePotInSpace at n points (xyz: 3n); ref_frame != 0: the reference's unrotated formula */
+void orc_pot_in_space(orc_system *s, int n, const double *xyz, double *out, int ref_frame);
 int orc_system_counts(const orc_system *s, int *n_inter_pairs, int *n_polarized);
 
 /* ---- Brownian dynamics of one two-molecule trajectory (CBDnam) ----

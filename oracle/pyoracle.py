@@ -106,6 +106,8 @@ def lib():
             f.restype = None
         L.orc_bd_step.restype = I
         L.orc_bd_step.argtypes = [P, P]
+        L.orc_pot_in_space.restype = None
+        L.orc_pot_in_space.argtypes = [P, I, P, P, I]
         L.orc_system_pot.restype = D
         L.orc_system_pot.argtypes = [P, I]
         L.orc_system_total_int_energy.restype = D
@@ -386,6 +388,12 @@ class System:
         f, t = np.zeros(3 * self.n), np.zeros(3 * self.n)
         rc = self.orc.L.orc_compute_forces(self.h, schedule, tol, max_rounds, _p(f), _p(t))
         return rc, f.reshape(-1, 3), t.reshape(-1, 3)
+
+    def pot_in_space(self, xyz, ref_frame=False):
+        xyz = f64(xyz).reshape(-1)
+        out = np.zeros(len(xyz) // 3)
+        self.orc.L.orc_pot_in_space(self.h, len(out), _p(xyz), _p(out), 1 if ref_frame else 0)
+        return out
 
     def get(self, i, k):
         p = self.p

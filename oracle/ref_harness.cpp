@@ -50,10 +50,10 @@ static std::vector<double> padded(const CExpan &E, int p) {
   return v;
 }
 
-// One molecule of a Brownian-dynamics pair, built with the constructor CBDnam uses
-// (molecule.cpp:422-487).  Reads: ns (x y z r)*ns nq (x y z q)*nq, centres relative to
-// the molecule centre.  Self expansions are zero: makeMove never reads them.
-static CMolecule *bd_molecule(int moltype) {
+// One molecule built with the constructor CBDnam uses (molecule.cpp:422-487).  Reads:
+// ns (x y z r)*ns nq (x y z q)*nq [F(ns*N^2) H(ns*N^2) with withFH], centres relative to the
+// molecule centre.  Without withFH the self expansions are zero (makeMove never reads them).
+static CMolecule *bd_molecule(int moltype, bool withFH = false) {
   int ns = ri();
   std::vector<CPnt> cens(ns);
   std::vector<double> rad(ns);
@@ -73,10 +73,11 @@ static CMolecule *bd_molecule(int moltype) {
   const double idiel = 4.0, intraRc = 30.0;
   int L = N_POLES * N_POLES;
   std::vector<double> zero(L, 0.0);
+  std::vector<double> fh = withFH ? rdv(2 * ns * L) : std::vector<double>(2 * ns * L, 0.0);
   std::vector<CMulExpan *> Fs(ns), Hs(ns);
   for (int k = 0; k < ns; k++) {
-    Fs[k] = new CMulExpan(&zero[0], CRange(0, N_POLES), rad[k]);
-    Hs[k] = new CMulExpan(&zero[0], CRange(0, N_POLES), rad[k]);
+    Fs[k] = new CMulExpan(&fh[k * L], CRange(0, N_POLES), rad[k]);
+    Hs[k] = new CMulExpan(&fh[(ns + k) * L], CRange(0, N_POLES), rad[k]);
   }
   std::vector<std::vector<CPnt> > SPxes;
   std::vector<int> nSPx;
@@ -460,6 +461,24 @@ int main() {
         res.push_back(rots[i].z());
         res.push_back(ids[i]);
       }
+    } else if (c == "potspace") {
+      // npts (x y z)*npts | molecule 0 | molecule 1 (with F, H) | pos0 q0 pos1 q1
+      //   -> computePotInSpace at each point, then computeTotalIntEnergy
+      int npts = ri();
+      std::vector<double> P = rdv(3 * npts);
+      CMolecule::N_MOL = 0;
+      std::vector<CMolecule *> mols;
+      mols.push_back(bd_molecule(0, true));
+      mols.push_back(bd_molecule(1, true));
+      for (int i = 0; i < 2; i++) {
+        double x = rd(), y = rd(), z = rd();
+        double w = rd(), qx = rd(), qy = rd(), qz = rd();
+        mols[i]->setPos(CPnt(x, y, z));
+        mols[i]->setOrient(CQuat(w, CPnt(qx, qy, qz)));
+      }
+      for (int t = 0; t < npts; t++)
+        res.push_back(CMolecule::computePotInSpace(mols, CPnt(P[3 * t], P[3 * t + 1], P[3 * t + 2])));
+      res.push_back(CMolecule::computeTotalIntEnergy(mols));
     } else if (c == "bd") {
       // seed place nsteps Dtr Dr1 Dr2 ft(9) | molecule 0 | molecule 1 | [pos0 q0 pos1 q1 if !place]
       // | ndef (ncontact npair (k0 k1 dist)*npair)*ndef

@@ -126,6 +126,18 @@ inline void cross3(const double *a, const double *b, double *o) {
   double x = a[1] * b[2] - a[2] * b[1], y = a[2] * b[0] - a[0] * b[2], z = a[0] * b[1] - a[1] * b[0];
   o[0] = x; o[1] = y; o[2] = z;
 }
+// CXFormBase::computeOrder (xforms.cpp:402-414), clamped to the run's p (host twin of the
+// device compute_order)
+inline int host_compute_order(double rho, double tq, double scale, int p) {
+  if (tq < 1e-10) return 1;
+  double cc = rho / scale - 1;
+  if (cc <= 1) return p;
+  int q = int(log(tq / (1e-6 * scale * (cc - 1))) / log(cc) - 1 + 0.5);
+  if (q < 1) q = 1;
+  if (q > p) q = p;
+  return q;
+}
+
 // q * r for a unit quaternion q = (w,x,y,z)  (util.h:463-468)
 inline void quat_rot(const double *q, const double *r, double *o) {
   double qr[3], t[3], u[3];
@@ -1280,6 +1292,7 @@ struct GradState {
   DBuf<int> ia, ib, ic, tqmap;             // scratch lists of the self-value batches
   DBuf<int> dpairs;                        // (source, target) sphere pairs of a re-expansion batch
   DBuf<double> lab, rad;                   // spheres' lab centres and radii
+  DBuf<double> pts, mol7, cen3;            // potential queries: points, poses, sphere centres
   DBuf<int> is_off, is_lst, ps_off, ps_lst, ident, esph, eitem, sbuf;
   DBuf<MJob> mj;
   DBuf<double> mpart;
@@ -2696,4 +2709,5 @@ int pbsam_rotate_quat_batch(pbsam_ctx *c, int n, int p, const double *quat, int 
 }  // extern "C"
 
 #include "pbsam_host_bd.inc"
+#include "pbsam_host_energy.inc"
 #include "pbsam_host_io.inc"

@@ -1413,3 +1413,4 @@ __global__ void __launch_bounds__(256) k_surfgemm_k4(SGArgs A) {
 }  // namespace pbsam
 
 #include "pbsam_kernels_imat.hip"
+#include "pbsam_kernels_pot.hip"

@@ -37,7 +37,9 @@ SYMBOLS = [
     "pbsam_bd_defaults", "pbsam_bd_start", "pbsam_bd_contacts", "pbsam_bd_run", "pbsam_bd_get",
     "pbsam_read_pqr", "pbsam_write_imat", "pbsam_read_imat", "pbsam_write_expansion", "pbsam_read_expansion",
     "pbsam_read_config", "pbsam_write_config", "pbsam_read_imats", "pbsam_read_expansions",
+    "pbsam_potential", "pbsam_total_int_energy",
 ]
+POT_REFERENCE_FRAME = 1
 BD_ESCAPED, BD_DOCKED, BD_RUNNING, BD_STUCK, BD_FAILED = range(5)
 
 KERNELS = {"xform": 0, "p2l": 1, "solve": 2, "surface": 3, "xgrad": 4, "surfgrad": 5}
@@ -112,6 +114,8 @@ def load():
         "pbsam_write_config": ([ctypes.c_char_p, I, P, P, P], I),
         "pbsam_read_imats": ([ctypes.c_char_p, I, I, P], I),
         "pbsam_read_expansions": ([ctypes.c_char_p, ctypes.c_char_p, I, I, P, P, P], I),
+        "pbsam_potential": ([P, I, P, P, ctypes.c_uint], I),
+        "pbsam_total_int_energy": ([P, ctypes.c_uint, P], I),
         "pbsam_system_destroy": ([P], None),
         "pbsam_system_set_pose": ([P, I, P, P], I),
         "pbsam_system_set_cutoffs": ([P, D, D], I),
@@ -478,6 +482,21 @@ class System:
     def energy(self):
         e = np.zeros(self.n)
         self.ctx.check(self.ctx.L.pbsam_energy(self.h, _p(e)))
+        return e
+
+    def potential(self, xyz, reference_frame=False):
+        """computePotInSpace at points (n,3) from the current state"""
+        xyz = f64(xyz).reshape(-1)
+        out = np.zeros(len(xyz) // 3)
+        self.ctx.check(self.ctx.L.pbsam_potential(self.h, len(out), _p(xyz), _p(out),
+                                                  POT_REFERENCE_FRAME if reference_frame else 0))
+        return out
+
+    def total_int_energy(self, reference_frame=False):
+        """computeTotalIntEnergy per system of the handle"""
+        e = np.zeros(len(self.groups))
+        self.ctx.check(self.ctx.L.pbsam_total_int_energy(self.h, POT_REFERENCE_FRAME if reference_frame else 0,
+                                                         _p(e)))
         return e
 
     def status(self):
