@@ -16,9 +16,9 @@ mutual solve alone (CMolecule::polarize_mutual, src/molecule.cpp:1989-2075), tim
 `batch` is the batched path: 256 Brownian-dynamics snapshots of the same two molecules
 (centre distance uniform in [50, 80] A) evaluated together through pbsam_batch_create,
 beside the plain one-system-per-call path timed on a sample of the same snapshots.
-`bd` (with --bd 1024) is batched Brownian dynamics (CBDnam trajectories, one per system of a
-batch): 1024 trajectories of two 100-sphere molecules at p=10, forces every step, reported as
-trajectory-steps/s (with --bd-cpu, beside the oracle's single trajectory on one host core).
+`bd` is batched Brownian dynamics (CBDnam trajectories, one per system of a batch): 1024
+trajectories of two 100-sphere molecules at p=10, forces every step, reported as
+trajectory-steps/s beside the oracle's single trajectory on one host core.
 
 Prints one JSON line.  `roofline` is the kernel with the largest share of device time,
 timed with HIP events around each of its launches on the library's stream, with the
@@ -298,11 +298,10 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=2, help="timed batched calls")
     ap.add_argument("--plain-sample", type=int, default=8, help="systems timed one by one for the ratio")
     ap.add_argument("--solves", type=int, default=20, help="mutual solves timed for solves_per_s")
-    ap.add_argument("--bd", type=int, default=0,
-                    help="trajectories in the Brownian-dynamics leg (0: skip it; 1024 is the BD workload)")
+    ap.add_argument("--bd", type=int, default=1024, help="trajectories in the Brownian-dynamics leg (0: skip it)")
     ap.add_argument("--bd-steps", type=int, default=3, help="timed steps of the Brownian-dynamics leg")
-    ap.add_argument("--bd-cpu", action="store_true",
-                    help="also time the oracle's trajectory 0 in the Brownian-dynamics leg")
+    ap.add_argument("--pot-grid", type=int, default=0, metavar="N",
+                    help="also time pbsam_potential on an N^3 grid around the solved system (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -382,10 +381,24 @@ def main():
         print("batch of %d systems: %.1f evals/s (plain loop %.1f evals/s)" %
               (args.batch, batch["evals_per_s"], batch["plain_loop_evals_per_s"]), file=sys.stderr, flush=True)
 
+    potential = None
+    if args.pot_grid > 0:
+        # potential map of the solved two-molecule system (computePotInSpace at every grid point)
+        system.solve(tol=0, max_rounds=0)
+        g = np.linspace(-40.0, 90.0, args.pot_grid)
+        pts = np.stack(np.meshgrid(g, g - 25.0, g - 25.0, indexing="ij"), -1).reshape(-1, 3)
+        system.potential(pts[:1024])
+        ctx.sync()
+        t0 = time.perf_counter()
+        system.potential(pts)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        potential = dict(points=len(pts), seconds=el, points_per_s=len(pts) / el,
+                         grid="%d^3 points spanning 130 A around the two molecules" % args.pot_grid)
+
     bd = None
     if args.bd > 0:
-        # the oracle's trajectory runs only with --bd-cpu (see the Brownian-dynamics notes)
-        bd = bd_leg(pbsam_amd, synth, ctx, kappa, args.bd, args.bd_steps, 5.0 if args.bd_cpu else 0.0)
+        bd = bd_leg(pbsam_amd, synth, ctx, kappa, args.bd, args.bd_steps, 0.0 if args.no_cpu else 5.0)
         print("Brownian dynamics, %d trajectories: %.1f trajectory-steps/s" % (args.bd, bd["traj_steps_per_s"]),
               file=sys.stderr, flush=True)
 
@@ -416,6 +429,7 @@ def main():
         "solves_per_s": solves_per_s,
         "batch": batch,
         "bd": bd,
+        "potential": potential,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -12,7 +12,7 @@ import os
 import numpy as np
 import pytest
 
-from oracle import pyoracle
+import pyoracle
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIX = os.path.join(HERE, "golden", "bd_ref.npz")
