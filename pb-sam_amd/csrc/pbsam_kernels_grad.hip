@@ -905,7 +905,11 @@ struct MArgs {
 };
 constexpr int MT = 256;
 constexpr int MPTS = 32;  // points staged per tile
-constexpr int MKC = (PMAX * PMAX + MT - 1) / MT;  // coefficients per thread
+constexpr int MCW = 2;    // columns per thread, MT apart: a workgroup covers MCW * MT columns
+// Grid (job, point split, column chunk).  Each thread owns columns c0 + tid + k MT and
+// issues the loads of eight points of both its columns and both maps (32 loads) before it
+// adds them, so a launch keeps enough bytes in flight to stream the maps from HBM; the sums
+// are formed in the same order as one column at a time.
 template <int R>
 __global__ void __launch_bounds__(MT) k_p2lgemv(MArgs A) {
   __shared__ double qs[MPTS][2 * R];
@@ -914,15 +918,24 @@ __global__ void __launch_bounds__(MT) k_p2lgemv(MArgs A) {
   const int pF = compute_order(J.rho, A.TQF[J.src], J.a_src, p);
   const int pH = compute_order(J.rho, A.TQH[J.src], J.a_src, p);
   const int nF = pF * pF, nH = pH * pH;
+  const int c0 = blockIdx.z * (MCW * MT);
   const int hb = blockIdx.y * A.spl, he = min(J.npts, hb + A.spl);
   double *out = A.part + (((size_t)blockIdx.x * A.nsplit + blockIdx.y) * 2) * R * P2;
   const double *MF = J.M, *MH = MF + (size_t)J.npts * P2;
-  double aF[MKC][R], aH[MKC][R];
+  bool inF[MCW], inH[MCW];
 #pragma unroll
-  for (int k = 0; k < MKC; k++)
+  for (int k = 0; k < MCW; k++) {
+    const int c = c0 + tid + k * MT;
+    inF[k] = c < nF;
+    inH[k] = c < nH;
+  }
+  double aF[MCW][R], aH[MCW][R];
+#pragma unroll
+  for (int k = 0; k < MCW; k++)
 #pragma unroll
     for (int r = 0; r < R; r++) aF[k][r] = aH[k][r] = 0.0;
-  for (int h0 = hb; h0 < he; h0 += MPTS) {  // uniform across the workgroup
+  const bool any = c0 < nF || c0 < nH;  // uniform across the workgroup
+  for (int h0 = hb; any && h0 < he; h0 += MPTS) {
     const int h1 = min(he, h0 + MPTS);
     __syncthreads();
     // stage this tile's charges (zero where the point is skipped)
@@ -937,61 +950,92 @@ __global__ void __launch_bounds__(MT) k_p2lgemv(MArgs A) {
       qs[t][v] = q;
     }
     __syncthreads();
+    double tF[MCW][R], tH[MCW][R];
 #pragma unroll
-    for (int k = 0; k < MKC; k++) {
-      const int c = tid + k * MT;
-      double tF[R], tH[R];
+    for (int k = 0; k < MCW; k++)
 #pragma unroll
-      for (int r = 0; r < R; r++) tF[r] = tH[r] = 0.0;
-      if (c < nF) {
-        const double *col = MF + (size_t)h0 * P2 + c;
-        int h = h0;
-        for (; h + 4 <= h1; h += 4) {  // four independent loads in flight
-          const double m0 = col[0], m1 = col[P2], m2 = col[2 * P2], m3 = col[3 * P2];
-          const int t = h - h0;
+      for (int r = 0; r < R; r++) tF[k][r] = tH[k][r] = 0.0;
+    int h = h0;
+    // groups of 8 points (32 loads in flight), then of 4; each 4 points added as one term
+    for (; h + 8 <= h1; h += 8) {
+      double mF[MCW][8], mH[MCW][8];
 #pragma unroll
-          for (int r = 0; r < R; r++)
-            tF[r] += qs[t][r] * m0 + qs[t + 1][r] * m1 + qs[t + 2][r] * m2 + qs[t + 3][r] * m3;
-          col += 4 * (size_t)P2;
+      for (int k = 0; k < MCW; k++) {
+        const size_t o = (size_t)h * P2 + c0 + tid + k * MT;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          mF[k][u] = inF[k] ? MF[o + (size_t)u * P2] : 0.0;
+          mH[k][u] = inH[k] ? MH[o + (size_t)u * P2] : 0.0;
         }
-        for (; h < h1; h++, col += P2)
-#pragma unroll
-          for (int r = 0; r < R; r++) tF[r] += qs[h - h0][r] * col[0];
       }
-      if (c < nH) {
-        const double *col = MH + (size_t)h0 * P2 + c;
-        int h = h0;
-        for (; h + 4 <= h1; h += 4) {
-          const double m0 = col[0], m1 = col[P2], m2 = col[2 * P2], m3 = col[3 * P2];
-          const int t = h - h0;
 #pragma unroll
-          for (int r = 0; r < R; r++)
-            tH[r] += qs[t][R + r] * m0 + qs[t + 1][R + r] * m1 + qs[t + 2][R + r] * m2 + qs[t + 3][R + r] * m3;
-          col += 4 * (size_t)P2;
-        }
-        for (; h < h1; h++, col += P2)
+      for (int g4 = 0; g4 < 8; g4 += 4) {
+        const int t = h - h0 + g4;
 #pragma unroll
-          for (int r = 0; r < R; r++) tH[r] += qs[h - h0][R + r] * col[0];
-      }
-      // tiles are added in point order (a split of one tile keeps the tile's sum)
+        for (int k = 0; k < MCW; k++)
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        aF[k][r] = (h0 == hb) ? tF[r] : aF[k][r] + tF[r];
-        aH[k][r] = (h0 == hb) ? tH[r] : aH[k][r] + tH[r];
+          for (int r = 0; r < R; r++) {
+            tF[k][r] += qs[t][r] * mF[k][g4] + qs[t + 1][r] * mF[k][g4 + 1] + qs[t + 2][r] * mF[k][g4 + 2] +
+                        qs[t + 3][r] * mF[k][g4 + 3];
+            tH[k][r] += qs[t][R + r] * mH[k][g4] + qs[t + 1][R + r] * mH[k][g4 + 1] +
+                        qs[t + 2][R + r] * mH[k][g4 + 2] + qs[t + 3][R + r] * mH[k][g4 + 3];
+          }
       }
     }
+    for (; h + 4 <= h1; h += 4) {
+      double mF[MCW][4], mH[MCW][4];
+#pragma unroll
+      for (int k = 0; k < MCW; k++) {
+        const size_t o = (size_t)h * P2 + c0 + tid + k * MT;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          mF[k][u] = inF[k] ? MF[o + (size_t)u * P2] : 0.0;
+          mH[k][u] = inH[k] ? MH[o + (size_t)u * P2] : 0.0;
+        }
+      }
+      const int t = h - h0;
+#pragma unroll
+      for (int k = 0; k < MCW; k++)
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          tF[k][r] += qs[t][r] * mF[k][0] + qs[t + 1][r] * mF[k][1] + qs[t + 2][r] * mF[k][2] +
+                      qs[t + 3][r] * mF[k][3];
+          tH[k][r] += qs[t][R + r] * mH[k][0] + qs[t + 1][R + r] * mH[k][1] + qs[t + 2][R + r] * mH[k][2] +
+                      qs[t + 3][R + r] * mH[k][3];
+        }
+    }
+    for (; h < h1; h++) {
+#pragma unroll
+      for (int k = 0; k < MCW; k++) {
+        const size_t o = (size_t)h * P2 + c0 + tid + k * MT;
+        const double mf = inF[k] ? MF[o] : 0.0, mh = inH[k] ? MH[o] : 0.0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          tF[k][r] += qs[h - h0][r] * mf;
+          tH[k][r] += qs[h - h0][R + r] * mh;
+        }
+      }
+    }
+    // tiles are added in point order (a split of one tile keeps the tile's sum)
+#pragma unroll
+    for (int k = 0; k < MCW; k++)
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        aF[k][r] = (h0 == hb) ? tF[k][r] : aF[k][r] + tF[k][r];
+        aH[k][r] = (h0 == hb) ? tH[k][r] : aH[k][r] + tH[k][r];
+      }
   }
 #pragma unroll
-  for (int k = 0; k < MKC; k++) {
-    const int c = tid + k * MT;
+  for (int k = 0; k < MCW; k++) {
+    const int c = c0 + tid + k * MT;
     if (c < P2)
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        out[(size_t)r * P2 + c] = aF[k][r];
-        out[(size_t)(R + r) * P2 + c] = aH[k][r];
+        out[(size_t)r * P2 + c] = inF[k] ? aF[k][r] : 0.0;
+        out[(size_t)(R + r) * P2 + c] = inH[k] ? aH[k][r] : 0.0;
       }
   }
-  if (A.work && tid == 0) {
+  if (A.work && tid == 0 && blockIdx.z == 0) {
     const int np = he - hb > 0 ? he - hb : 0;
     atomicAdd(&A.work[0], (unsigned long long)(2.0 * R * (nF + nH) * np));
     atomicAdd(&A.work[1], (unsigned long long)(8.0 * (nF + nH) * np));
