@@ -1296,6 +1296,9 @@ struct GradState {
   DBuf<int> is_off, is_lst, ps_off, ps_lst, ident, esph, eitem, sbuf;
   DBuf<MJob> mj;
   DBuf<double> mpart;
+  DBuf<double> s3spec;                     // k_solve3 column workgroups: [row][F, H][3][p^2]
+  DBuf<unsigned long long> s3mask;         // [row][3][2]
+  DBuf<unsigned int> s3cnt;                // [row], zero between launches
   DBuf<const double *> yptr;               // [sphere] harmonic rows of its exposed points
   PinBuf<int> h_sb;                        // step lists staged for upload
   PinBuf<double> h_dev;                    // per-step changes read back

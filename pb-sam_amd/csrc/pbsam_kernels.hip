@@ -1040,17 +1040,18 @@ __device__ __forceinline__ double id_sweep(IdState<KW> &S) {
 // One sweep as id_sweep without the computeDev sum: true when one of this lane's pairs
 // alone makes the sweep's computeDev exceed 1e-8 -- its term s (s >= 0 for every pair)
 // exceeds c = 4 p^2 1e-8 with a relative margin of 1e-6, tested as top > c2 bot (c2 = c, or
-// c/2 where the term carries the reference's factor 2) without the division.  Pairs whose
-// old and new values are all below 1e-30 are skipped as in the reference; a pair with only
-// one side that small (the reference's term 1) certifies only through the same test, which
-// can only leave a sweep uncertified.  The sum of the nonnegative terms is then above the
-// threshold whatever the summation order.  m = 0 pairs keep zero imaginary parts by
-// themselves (zero right-hand side and start), so no selects are needed here.
+// c/2 where the term carries the reference's factor 2) without the division.  The pair
+// must also have bot > 4e-60, so that one of its old and new values is at least 1e-30: the
+// reference skips a pair only when all of them are below 1e-30, and gives the term 1 (above
+// c) when only one side is.  The sum of the nonnegative terms is then above the threshold
+// whatever the summation order.  The sweep has no branches, so a wavefront issues the next
+// sweep's updates while this one's test completes: pairs a lane does not hold (nn < 0) must
+// be all zeros, which stay zeros and never pass the test.  m = 0 pairs keep zero imaginary
+// parts by themselves (zero right-hand side and start), so no selects are needed here.
 template <int KW>
 __device__ __forceinline__ bool id_sweep_fast(IdState<KW> &S) {
   bool big = false;
   for (int k = 0; k < KW; k++) {
-    if (S.nn[k] < 0) continue;
     const double ar = S.Hr[k], ai = S.Hi[k];
     double xr = S.fbr[k] + S.kH1[k] * S.Hr[k] + S.kF1[k] * S.Fr[k];
     double xi = S.fbi[k] + S.kH1[k] * S.Hi[k] + S.kF1[k] * S.Fi[k];
@@ -1061,13 +1062,21 @@ __device__ __forceinline__ bool id_sweep_fast(IdState<KW> &S) {
     const double br = xr * S.kIH[k], bi = xi * S.kIH[k];
     S.Hr[k] = br;
     S.Hi[k] = bi;
-    const double mx = fmax(fmax(fabs(ar), fabs(ai)), fmax(fabs(br), fabs(bi)));
     const double dr = ar - br, di = ai - bi;
     const double top = dr * dr + di * di;
     const double bot = ar * ar + ai * ai + br * br + bi * bi;
-    big = big || (mx >= 1e-30 && top > S.c2[k] * bot);
+    big = big | ((bot > 4e-60) & (top > S.c2[k] * bot));
   }
   return big;
+}
+// every field of pair k zero (a pair the lane does not hold)
+template <int KW>
+__device__ __forceinline__ void id_clear(IdState<KW> &S, int k) {
+  S.Fr[k] = S.Fi[k] = S.Hr[k] = S.Hi[k] = 0.0;
+  S.fbr[k] = S.fbi[k] = S.hbr[k] = S.hbi[k] = S.uHr[k] = S.uHi[k] = 0.0;
+  S.kH1[k] = S.kF1[k] = S.kH2[k] = S.kF2[k] = S.kIH[k] = S.kInv[k] = S.c2[k] = 0.0;
+  S.nn[k] = -1;
+  S.i0[k] = 0;
 }
 
 // Sweeps until the summed computeDev of all NW wavefronts' pairs <= 1e-8 (or 100 sweeps),
@@ -1086,7 +1095,8 @@ __device__ int id_iterate(IdState<KW> &S, int p, double *red) {
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
   int *flags = reinterpret_cast<int *>(red + 2 * IC * NW + 2 * NW);
   const double cthr = 4.0 * p * p * 1e-8 * (1.0 + 1e-6);
-  for (int k = 0; k < KW; k++) S.c2[k] = (S.nn[k] >= 0 && S.i0[k] != S.nn[k] * S.nn[k]) ? 0.5 * cthr : cthr;
+  for (int k = 0; k < KW; k++)
+    if (S.nn[k] >= 0) S.c2[k] = (S.i0[k] != S.nn[k] * S.nn[k]) ? 0.5 * cthr : cthr;
   int ct = 0;
   for (int chunk = 0;; chunk++) {
     double sFr[KW], sFi[KW], sHr[KW], sHi[KW];
@@ -1181,7 +1191,7 @@ __device__ void solve_identity(const SArgs &A, int g, int lst0, int lst1) {
   IdState<KW> S;
   for (int k = 0; k < KW; k++) {
     const int ep = threadIdx.x + ST * k;
-    S.nn[k] = -1;
+    id_clear(S, k);
     if (ep >= npair) continue;
     int n, i;
     pair_nm(ep, n, i);

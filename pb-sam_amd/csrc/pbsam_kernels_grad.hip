@@ -500,13 +500,19 @@ struct S3Args {
   const int *lfi;          // right-hand-side block of each entry (null: the workgroup index)
   double *E;               // [block][p^2][6]: the scaled surface right-hand sides of the
                            // result (k_surfgemm's operand), or null
+  // identity-matrix entries as three column workgroups (grid y = 3; see solve3_split)
+  double *spec;                // [row][F, H][3][p^2] tentative states
+  unsigned long long *mask;    // [row][3][2] certified-sweep masks
+  unsigned int *cnt;           // [row] arrivals, zero between launches
 };
 
 // three right-hand sides: WPC wavefronts per column, column d = wave / WPC
 constexpr int WPC = 4;
 constexpr int S3T = 3 * WPC * 64;
+// fin (or null): the loop's final F and H ([F, H][3][p^2]) after its 100 sweeps, computed
+// by solve3_split; the epilogue then runs on them in place of the loop
 template <int KW>
-__device__ void solve3_identity(const S3Args &A, int g, int e, int lb, double *red) {
+__device__ void solve3_identity(const S3Args &A, int g, int e, int lb, double *red, const double *fin = nullptr) {
   constexpr int NW = 3 * WPC;
   const int p = A.p, P2 = p * p, lane = threadIdx.x % 64, w = threadIdx.x / 64, d = w / WPC, sub = w % WPC;
   const double *cs = A.consts + (size_t)g * 10 * p;
@@ -516,7 +522,7 @@ __device__ void solve3_identity(const S3Args &A, int g, int e, int lb, double *r
   IdState<KW> S;
   for (int k = 0; k < KW; k++) {
     const int ep = sub * 64 + lane + WPC * 64 * k;
-    S.nn[k] = -1;
+    id_clear(S, k);
     if (ep >= npair) continue;
     int n, i;
     pair_nm(ep, n, i);
@@ -534,8 +540,15 @@ __device__ void solve3_identity(const S3Args &A, int g, int e, int lb, double *r
     S.Hi[k] = two ? A.H[o + i + 1] : 0.0;
     S.uHr[k] = S.Hr[k];
     S.uHi[k] = S.Hi[k];
+    if (fin) {
+      const double *fF = fin + (size_t)d * P2, *fH = fin + (size_t)(3 + d) * P2;
+      S.Fr[k] = fF[i];
+      S.Hr[k] = fH[i];
+      S.Fi[k] = two ? fF[i + 1] : 0.0;
+      S.Hi[k] = two ? fH[i + 1] : 0.0;
+    }
   }
-  const int ct = id_iterate<KW, NW>(S, p, red);
+  const int ct = fin ? 100 : id_iterate<KW, NW>(S, p, red);
   double td, ts;
   id_dev2(S, td, ts);
   for (int k = 0; k < KW; k++) {
@@ -580,6 +593,97 @@ __device__ void solve3_identity(const S3Args &A, int g, int e, int lb, double *r
   }
 }
 
+// Column workgroup d = blockIdx.y of an identity-matrix entry: the column's (n, m) pairs,
+// one per thread, through all 100 sweeps of the joint loop without its stopping test; bit s
+// of the column's mask is set when one of its pairs alone shows that sweep s continues (the
+// test of id_sweep_fast).  States go to the row's tentative block.  The last of the three
+// column workgroups to arrive (a device-scope counter; nothing waits) finishes the entry:
+// when every sweep 0..98 is certified by some column, the joint loop runs all 100 sweeps and
+// the tentative states are its result, taken through solve3_identity's epilogue; otherwise
+// it runs the joint loop itself from the entry's inputs, which no column workgroup writes.
+// Either way the outputs are the joint loop's, bit for bit, from three CUs instead of one.
+template <int KW>
+__device__ void solve3_split(const S3Args &A, int g, int e, int lb, double *red) {
+  const int p = A.p, P2 = p * p, tid = threadIdx.x, lane = tid % 64, w = tid / 64, d = blockIdx.y;
+  const int npair = p * (p + 1) / 2;
+  const double *cs = A.consts + (size_t)g * 10 * p;
+  const double *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p;
+  const size_t o = ((size_t)e * 3 + d) * P2, ol = ((size_t)lb * 3 + d) * P2;
+  double *spec = A.spec + (size_t)blockIdx.x * 6 * P2;
+  IdState<1> S;
+  id_clear(S, 0);
+  int n = 0, i = 0;
+  if (tid < npair) {
+    pair_nm(tid, n, i);
+    S.nn[0] = n;
+    S.i0[0] = i;
+    id_consts(S, 0, n, cs, p);
+    const bool two = (i != n * n);
+    S.fbr[0] = cLH1[n] * A.LH[ol + i] + cLF1[n] * A.LF[ol + i];
+    S.hbr[0] = cLH2[n] * A.LH[ol + i] + cLF2[n] * A.LF[ol + i];
+    S.fbi[0] = two ? cLH1[n] * A.LH[ol + i + 1] + cLF1[n] * A.LF[ol + i + 1] : 0.0;
+    S.hbi[0] = two ? cLH2[n] * A.LH[ol + i + 1] + cLF2[n] * A.LF[ol + i + 1] : 0.0;
+    S.Fr[0] = A.F[o + i];
+    S.Hr[0] = A.H[o + i];
+    S.Fi[0] = two ? A.F[o + i + 1] : 0.0;
+    S.Hi[0] = two ? A.H[o + i + 1] : 0.0;
+    const double cthr = 4.0 * p * p * 1e-8 * (1.0 + 1e-6);  // as id_iterate
+    S.c2[0] = two ? 0.5 * cthr : cthr;
+  }
+  unsigned long long m0 = 0, m1 = 0;  // sweeps 0..63, 64..99
+  if (w * 64 < npair) {               // wavefronts with pairs (uniform per wavefront)
+    for (int t = 0; t < 64; t++)
+      if (__ballot(id_sweep_fast(S))) m0 |= 1ull << t;
+    for (int t = 64; t < 100; t++)
+      if (__ballot(id_sweep_fast(S))) m1 |= 1ull << (t - 64);
+  }
+  if (tid < npair) {
+    spec[(size_t)d * P2 + i] = S.Fr[0];
+    spec[(size_t)(3 + d) * P2 + i] = S.Hr[0];
+    if (i != n * n) {
+      spec[(size_t)d * P2 + i + 1] = S.Fi[0];
+      spec[(size_t)(3 + d) * P2 + i + 1] = S.Hi[0];
+    }
+  }
+  unsigned long long *wm = reinterpret_cast<unsigned long long *>(red);
+  int *flag = reinterpret_cast<int *>(wm + 2 * (S3T / 64));
+  if (lane == 0) {
+    wm[2 * w] = m0;
+    wm[2 * w + 1] = m1;
+  }
+  __syncthreads();
+  unsigned long long *mk = A.mask + (size_t)blockIdx.x * 6;
+  if (tid == 0) {
+    unsigned long long a0 = 0, a1 = 0;
+    for (int x = 0; x < S3T / 64; x++) {
+      a0 |= wm[2 * x];
+      a1 |= wm[2 * x + 1];
+    }
+    mk[2 * d] = a0;
+    mk[2 * d + 1] = a1;
+  }
+  __threadfence();  // this workgroup's states and mask, device-wide, before it arrives
+  __syncthreads();
+  if (tid == 0) flag[0] = (atomicAdd(&A.cnt[blockIdx.x], 1u) == 2u);
+  __syncthreads();
+  if (!flag[0]) return;  // uniform
+  __threadfence();       // the other columns' states and masks
+  if (tid == 0) {
+    (void)atomicExch(&A.cnt[blockIdx.x], 0u);  // for the next launch
+    unsigned long long a0 = 0, a1 = 0;
+    for (int x = 0; x < 3; x++) {
+      a0 |= mk[2 * x];
+      a1 |= mk[2 * x + 1];
+    }
+    const unsigned long long need1 = (1ull << 35) - 1;  // sweeps 64..98 (sweep 99 ends the loop)
+    flag[1] = (a0 == ~0ull) && ((a1 & need1) == need1);
+  }
+  __syncthreads();
+  const bool ok = flag[1] != 0;
+  __syncthreads();  // red is the joint loop's from here
+  solve3_identity<KW>(A, g, e, lb, red, ok ? spec : nullptr);
+}
+
 __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
   extern __shared__ double sh[];
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
@@ -588,11 +692,18 @@ __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
   const double *IM = A.imat[g];
   if (!IM) {  // uniform across the workgroup
     __shared__ double red[2 * IC * 3 * WPC + 2 * 3 * WPC + 3 * WPC + 1];
-    if (p <= 22) solve3_identity<1>(A, g, e, lb, red);
-    else if (p <= 31) solve3_identity<2>(A, g, e, lb, red);
-    else solve3_identity<3>(A, g, e, lb, red);
+    if (gridDim.y == 3) {
+      if (p <= 22) solve3_split<1>(A, g, e, lb, red);
+      else if (p <= 31) solve3_split<2>(A, g, e, lb, red);
+      else solve3_split<3>(A, g, e, lb, red);
+    } else {
+      if (p <= 22) solve3_identity<1>(A, g, e, lb, red);
+      else if (p <= 31) solve3_identity<2>(A, g, e, lb, red);
+      else solve3_identity<3>(A, g, e, lb, red);
+    }
     return;
   }
+  if (blockIdx.y != 0) return;  // general matrix: the first workgroup of the row only
   // general matrix: F, H, X of the three columns in LDS
   double *F = sh, *H = F + 3 * P2, *X = H + 3 * P2, *oldH = X + 3 * P2, *red = oldH + 3 * P2;
   const double *cs = A.consts + (size_t)g * 10 * p;
