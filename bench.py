@@ -23,7 +23,10 @@ trajectory-steps/s beside the oracle's single trajectory on one host core.
 Prints one JSON line.  `roofline` is the kernel with the largest share of device time,
 timed with HIP events around each of its launches on the library's stream, with the
 algorithmic flops/bytes its launches processed (counted on the device from the
-reference's loop bounds at the truncation orders actually used).  `cpu_baseline` is the
+reference's loop bounds at the truncation orders actually used).  The event pairs add
+host calls and queue packets between dependent kernels (about a fifth of an evaluation),
+so they time a repeat of the K evaluations right after the timed region, not the timed
+region itself; the batched leg's per-kernel figures likewise come from one extra call.  `cpu_baseline` is the
 CPU oracle (a scalar restatement of the reference, 1 thread) on the same system.
 """
 import argparse
@@ -99,7 +102,6 @@ def batch_leg(pbsam_amd, synth, ctx, mols, mts, nsys, reps, nplain):
     batch.set_cutoffs(100.0, 100.0)
     batch.compute_forces(tol=0, max_rounds=0)
     ctx.sync()
-    ctx.profile(True)
     mark = roctx()
     if mark:
         mark.roctxRangePushA(b"batch_region")
@@ -110,6 +112,10 @@ def batch_leg(pbsam_amd, synth, ctx, mols, mts, nsys, reps, nplain):
     el = time.perf_counter() - t0
     if mark:
         mark.roctxRangePop()
+    # per-kernel HIP-event timings from one more, instrumented, batched call
+    ctx.profile(True)
+    batch.compute_forces(tol=0, max_rounds=0)
+    ctx.sync()
     prof = {k: ctx.profile_read(k) for k in pbsam_amd.KERNELS}
     ctx.profile(False)
     st, rounds, passes = batch.status()
@@ -135,8 +141,8 @@ def batch_leg(pbsam_amd, synth, ctx, mols, mts, nsys, reps, nplain):
                 plain_loop_evals_per_s=plain, plain_loop_sample=nplain, speedup_vs_plain_loop=value / plain,
                 max_rel_err_force_batch_vs_plain=err, groups_ok=int((st == 0).sum()),
                 mutual_rounds_max=int(rounds.max()), grad_passes_mean=float(passes.mean()),
-                kernels={k: dict(launches=v["launches"], ms_per_batch=v["ms"] / reps,
-                                 gflop_per_batch=v["flops"] / reps / 1e9) for k, v in prof.items()})
+                kernels={k: dict(launches=v["launches"], ms_per_batch=v["ms"],
+                                 gflop_per_batch=v["flops"] / 1e9) for k, v in prof.items()})
 
 
 BD_SPH, BD_P, BD_NPTS, BD_SEEDS = 100, 10, 400, (301, 302)
@@ -329,7 +335,6 @@ def main():
     for _ in range(args.warmup):
         system.compute_forces(tol=0, max_rounds=0)
     ctx.sync()
-    ctx.profile(True)
     mark = roctx()
     if mark:
         mark.roctxRangePushA(b"timed_region")
@@ -340,12 +345,25 @@ def main():
     elapsed = time.perf_counter() - t0
     if mark:
         mark.roctxRangePop()
-    prof = {k: ctx.profile_read(k) for k in pbsam_amd.KERNELS}
-    ctx.profile(False)
     ms_step = 1e3 * elapsed / args.steps
     value = args.steps / elapsed
     print("timed %d force+torque evaluations in %.3f s (mutual rounds %d)" % (args.steps, elapsed, rounds),
           file=sys.stderr, flush=True)
+    # Per-kernel HIP-event timings: the same K evaluations again with an event pair around
+    # every instrumented launch, on the stream it is launched on.  The event records add host
+    # calls and queue packets between dependent kernels, so they stay out of the timed region.
+    ctx.profile(True)
+    if mark:
+        mark.roctxRangePushA(b"kernel_timing_region")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        system.compute_forces(tol=0, max_rounds=0)
+    ctx.sync()
+    elapsed_ev = time.perf_counter() - t0
+    if mark:
+        mark.roctxRangePop()
+    prof = {k: ctx.profile_read(k) for k in pbsam_amd.KERNELS}
+    ctx.profile(False)
     # the solve alone (outside the timed region above)
     nsol = args.solves
     solves_per_s = None
@@ -368,6 +386,8 @@ def main():
         achieved = d["bytes"] / max(d["ms"] * 1e-3, 1e-30)
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s", frac=achieved / PEAK_HBM)
     roof.update(kernel=dom, launches=d["launches"], avg_launch_us=avg_s * 1e6,
+                timing="HIP events on the launching stream over %d instrumented evaluations run after the timed "
+                       "region (%.1f ms each with the events)" % (args.steps, 1e3 * elapsed_ev / args.steps),
                 algorithmic_per_launch=(d["flops"] if compute_bound else d["bytes"]) / max(d["launches"], 1),
                 traffic=TRAFFIC.get(dom), traffic_unit="HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                                         "copy-calibrated)")

@@ -3,9 +3,10 @@
     python -m pbsam_amd.rocprof <run_kernel_trace.csv> [run_marker_api_trace.csv [bench.json]]
 
 Prints a Markdown table of per-kernel dispatch statistics over the whole trace and, when
-the marker trace holds bench.py's "timed_region" (and "batch_region", "bd_region") range, over the
-dispatches that ran inside it; with bench.py's JSON line it also lists bench.py's own HIP-event averages of
-the instrumented kernels beside rocprofv3's.
+the marker trace holds bench.py's "timed_region" (and "batch_region", "bd_region",
+"kernel_timing_region") range, over the dispatches that ran inside it; with bench.py's JSON line it also
+lists bench.py's own HIP-event averages of the instrumented kernels (timed in the instrumented repeat of
+the timed evaluations) beside rocprofv3's over the same dispatches.
 
     python -m pbsam_amd.rocprof pmc <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv> <copy_bytes>
         [--after KERNEL]
@@ -80,6 +81,12 @@ def main(argv):
         return
     inside = [r for r in rows if r[1] >= rng[0] and r[2] <= rng[1]]
     agg = table(inside, "Dispatches inside the timed region (%.3f ms of wall time)" % ((rng[1] - rng[0]) / 1e6))
+    # bench.py's HIP events time a repeat of the timed evaluations ("kernel_timing_region")
+    erng = find_range(argv[2], "kernel_timing_region")
+    if erng is not None:
+        ins = [r for r in rows if r[1] >= erng[0] and r[2] <= erng[1]]
+        agg = table(ins, "Dispatches inside the instrumented repeat (%.3f ms of wall time)" %
+                    ((erng[1] - erng[0]) / 1e6))
     if len(argv) < 4:
         return
     with open(argv[3]) as f:
