@@ -1296,8 +1296,8 @@ struct GradState {
   DBuf<int> is_off, is_lst, ps_off, ps_lst, ident, esph, eitem, sbuf;
   DBuf<MJob> mj;
   DBuf<double> mpart;
-  DBuf<double> s3spec;                     // k_solve3 column workgroups: [row][F, H][3][p^2]
-  DBuf<unsigned long long> s3mask;         // [row][3][2]
+  DBuf<double> s3spec;                     // k_solve3 part workgroups: [row][F, H][3][p^2]
+  DBuf<unsigned long long> s3mask;         // [row][parts][2]
   DBuf<unsigned int> s3cnt;                // [row], zero between launches
   DBuf<const double *> yptr;               // [sphere] harmonic rows of its exposed points
   PinBuf<int> h_sb;                        // step lists staged for upload

@@ -500,15 +500,17 @@ struct S3Args {
   const int *lfi;          // right-hand-side block of each entry (null: the workgroup index)
   double *E;               // [block][p^2][6]: the scaled surface right-hand sides of the
                            // result (k_surfgemm's operand), or null
-  // identity-matrix entries as three column workgroups (grid y = 3; see solve3_split)
+  // identity-matrix entries as S3_PARTS workgroups (grid y; see solve3_split)
   double *spec;                // [row][F, H][3][p^2] tentative states
-  unsigned long long *mask;    // [row][3][2] certified-sweep masks
+  unsigned long long *mask;    // [row][S3_PARTS][2] certified-sweep masks
   unsigned int *cnt;           // [row] arrivals, zero between launches
 };
 
 // three right-hand sides: WPC wavefronts per column, column d = wave / WPC
 constexpr int WPC = 4;
 constexpr int S3T = 3 * WPC * 64;
+constexpr int S3_HALF = 2;                // workgroups per column in solve3_split
+constexpr int S3_PARTS = 3 * S3_HALF;     // workgroups per entry in solve3_split
 // fin (or null): the loop's final F and H ([F, H][3][p^2]) after its 100 sweeps, computed
 // by solve3_split; the epilogue then runs on them in place of the loop
 template <int KW>
@@ -593,19 +595,23 @@ __device__ void solve3_identity(const S3Args &A, int g, int e, int lb, double *r
   }
 }
 
-// Column workgroup d = blockIdx.y of an identity-matrix entry: the column's (n, m) pairs,
-// one per thread, through all 100 sweeps of the joint loop without its stopping test; bit s
-// of the column's mask is set when one of its pairs alone shows that sweep s continues (the
-// test of id_sweep_fast).  States go to the row's tentative block.  The last of the three
-// column workgroups to arrive (a device-scope counter; nothing waits) finishes the entry:
+// Part workgroup blockIdx.y of an identity-matrix entry: half h = y % S3_HALF of column
+// d = y / S3_HALF's (n, m) pairs, one per thread, through all 100 sweeps of the joint loop
+// without its stopping test; bit s of the part's mask is set when one of its pairs alone
+// shows that sweep s continues (the test of id_sweep_fast).  States go to the row's
+// tentative block.  The last of the S3_PARTS workgroups to arrive (a device-scope counter;
+// nothing waits) finishes the entry:
 // when every sweep 0..98 is certified by some column, the joint loop runs all 100 sweeps and
 // the tentative states are its result, taken through solve3_identity's epilogue; otherwise
 // it runs the joint loop itself from the entry's inputs, which no column workgroup writes.
-// Either way the outputs are the joint loop's, bit for bit, from three CUs instead of one.
+// Either way the outputs are the joint loop's, bit for bit, from S3_PARTS CUs instead of one.
 template <int KW>
 __device__ void solve3_split(const S3Args &A, int g, int e, int lb, double *red) {
-  const int p = A.p, P2 = p * p, tid = threadIdx.x, lane = tid % 64, w = tid / 64, d = blockIdx.y;
-  const int npair = p * (p + 1) / 2;
+  const int p = A.p, P2 = p * p, tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+  const int d = blockIdx.y / S3_HALF, h = blockIdx.y % S3_HALF;
+  const int npair = p * (p + 1) / 2, per = (npair + S3_HALF - 1) / S3_HALF;
+  const int ep = h * per + tid;
+  const bool mine = tid < per && ep < npair;
   const double *cs = A.consts + (size_t)g * 10 * p;
   const double *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p;
   const size_t o = ((size_t)e * 3 + d) * P2, ol = ((size_t)lb * 3 + d) * P2;
@@ -613,8 +619,8 @@ __device__ void solve3_split(const S3Args &A, int g, int e, int lb, double *red)
   IdState<1> S;
   id_clear(S, 0);
   int n = 0, i = 0;
-  if (tid < npair) {
-    pair_nm(tid, n, i);
+  if (mine) {
+    pair_nm(ep, n, i);
     S.nn[0] = n;
     S.i0[0] = i;
     id_consts(S, 0, n, cs, p);
@@ -631,13 +637,13 @@ __device__ void solve3_split(const S3Args &A, int g, int e, int lb, double *red)
     S.c2[0] = two ? 0.5 * cthr : cthr;
   }
   unsigned long long m0 = 0, m1 = 0;  // sweeps 0..63, 64..99
-  if (w * 64 < npair) {               // wavefronts with pairs (uniform per wavefront)
+  if (w * 64 < per) {                 // wavefronts with pairs (uniform per wavefront)
     for (int t = 0; t < 64; t++)
       if (__ballot(id_sweep_fast(S))) m0 |= 1ull << t;
     for (int t = 64; t < 100; t++)
       if (__ballot(id_sweep_fast(S))) m1 |= 1ull << (t - 64);
   }
-  if (tid < npair) {
+  if (mine) {
     spec[(size_t)d * P2 + i] = S.Fr[0];
     spec[(size_t)(3 + d) * P2 + i] = S.Hr[0];
     if (i != n * n) {
@@ -652,26 +658,26 @@ __device__ void solve3_split(const S3Args &A, int g, int e, int lb, double *red)
     wm[2 * w + 1] = m1;
   }
   __syncthreads();
-  unsigned long long *mk = A.mask + (size_t)blockIdx.x * 6;
+  unsigned long long *mk = A.mask + (size_t)blockIdx.x * 2 * S3_PARTS;
   if (tid == 0) {
     unsigned long long a0 = 0, a1 = 0;
     for (int x = 0; x < S3T / 64; x++) {
       a0 |= wm[2 * x];
       a1 |= wm[2 * x + 1];
     }
-    mk[2 * d] = a0;
-    mk[2 * d + 1] = a1;
+    mk[2 * blockIdx.y] = a0;
+    mk[2 * blockIdx.y + 1] = a1;
   }
   __threadfence();  // this workgroup's states and mask, device-wide, before it arrives
   __syncthreads();
-  if (tid == 0) flag[0] = (atomicAdd(&A.cnt[blockIdx.x], 1u) == 2u);
+  if (tid == 0) flag[0] = (atomicAdd(&A.cnt[blockIdx.x], 1u) == (unsigned)(S3_PARTS - 1));
   __syncthreads();
   if (!flag[0]) return;  // uniform
   __threadfence();       // the other columns' states and masks
   if (tid == 0) {
     (void)atomicExch(&A.cnt[blockIdx.x], 0u);  // for the next launch
     unsigned long long a0 = 0, a1 = 0;
-    for (int x = 0; x < 3; x++) {
+    for (int x = 0; x < S3_PARTS; x++) {
       a0 |= mk[2 * x];
       a1 |= mk[2 * x + 1];
     }
@@ -692,7 +698,7 @@ __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
   const double *IM = A.imat[g];
   if (!IM) {  // uniform across the workgroup
     __shared__ double red[2 * IC * 3 * WPC + 2 * 3 * WPC + 3 * WPC + 1];
-    if (gridDim.y == 3) {
+    if (gridDim.y == S3_PARTS) {
       if (p <= 22) solve3_split<1>(A, g, e, lb, red);
       else if (p <= 31) solve3_split<2>(A, g, e, lb, red);
       else solve3_split<3>(A, g, e, lb, red);
@@ -845,16 +851,30 @@ __global__ void __launch_bounds__(256) k_sumfh(const int *sel, const int *obs, c
   for (int i = blockIdx.y * 256 + threadIdx.x; i < bs; i += 256 * gridDim.y) {
     double sF = 0.0, sH = 0.0;
     int e = l0;
-    for (; e + 8 <= l1; e += 8) {  // eight loads per array in flight, added in list order
-      double vF[8], vH[8];
+    for (; e + 16 <= l1; e += 16) {  // sixteen loads per array in flight, added in list order
+      double vF[16], vH[16];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
+      for (int u = 0; u < 16; u++) {
         const size_t o = (size_t)lst[e + u] * bs + i;
         vF[u] = slotF[o];
         vH[u] = slotH[o];
       }
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
+      for (int u = 0; u < 16; u++) {
+        sF += vF[u];
+        sH += vH[u];
+      }
+    }
+    for (; e + 4 <= l1; e += 4) {
+      double vF[4], vH[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const size_t o = (size_t)lst[e + u] * bs + i;
+        vF[u] = slotF[o];
+        vH[u] = slotH[o];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
         sF += vF[u];
         sH += vH[u];
       }
@@ -1284,12 +1304,19 @@ __global__ void __launch_bounds__(256) k_p2lsum(const MJob *jobs, const int *sel
     const size_t stride = (size_t)2 * R * P2;
     double s = 0.0;
     int sp = 0;
-    for (; sp + 8 <= nsplit; sp += 8) {  // eight loads in flight, added in split order
-      double v[8];
+    for (; sp + 16 <= nsplit; sp += 16) {  // sixteen loads in flight, added in split order
+      double v[16];
 #pragma unroll
-      for (int u = 0; u < 8; u++) v[u] = pp[(sp + u) * stride];
+      for (int u = 0; u < 16; u++) v[u] = pp[(sp + u) * stride];
 #pragma unroll
-      for (int u = 0; u < 8; u++) s += v[u];
+      for (int u = 0; u < 16; u++) s += v[u];
+    }
+    for (; sp + 4 <= nsplit; sp += 4) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = pp[(sp + u) * stride];
+#pragma unroll
+      for (int u = 0; u < 4; u++) s += v[u];
     }
     for (; sp < nsplit; sp++) s += pp[sp * stride];
     const int half = e / (R * P2), rc = e % (R * P2);
@@ -1506,23 +1533,20 @@ __global__ void __launch_bounds__(256) k_surfgemm_k4(SGArgs A) {
   const int ksteps = (P2 + 3) / 4, kper = (ksteps + 3) / 4;
   const int k0 = 4 * w * kper, k1 = min(P2, 4 * (w + 1) * kper);
   v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
-  constexpr int UNR = 8;
-  int k = k0;
-  for (; k + 4 * UNR <= k1 && k + 4 * UNR <= (P2 & ~3); k += 4 * UNR) {
+  // UNR steps of 4 per round trip, all loads issued before the products; steps past the
+  // wavefront's range and k + l/16 past p^2 read as zero (their products add exact zeros)
+  constexpr int UNR = 16;
+  for (int k = k0; k < k1; k += 4 * UNR) {
     double av[UNR], bv[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; u++) {
-      av[u] = rok ? ya[k + 4 * u] : 0.0;
-      bv[u] = cok ? eb[(size_t)(k + 4 * u) * 6] : 0.0;
+      const int kk = k + 4 * u;
+      const bool kok = kk < k1 && kk + kq < P2;
+      av[u] = (rok && kok) ? ya[kk] : 0.0;
+      bv[u] = (cok && kok) ? eb[(size_t)kk * 6] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < UNR; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
-  }
-  for (; k < k1; k += 4) {  // remaining steps; k + l/16 past p^2 reads as zero
-    const bool kok = k + kq < P2;
-    const double av = (rok && kok) ? ya[k] : 0.0;
-    const double bv = (cok && kok) ? eb[(size_t)k * 6] : 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
   }
   // lane l, register r holds row l/16 + 4 r, column l % 16
 #pragma unroll
