@@ -50,8 +50,8 @@ PEAK_HBM = 8.0e12        # MI355X HBM3E, B/s
 # dispatches from the system's pair lists on), each scaled by the same counter for one
 # 256 MiB device-to-device copy (FETCH_SIZE reads half the bytes: x2.00; WRITE_SIZE x1.00),
 # averaged over the group's dispatches in one evaluation.
-TRAFFIC = {"solve": 226495, "xform": 2415558, "surfgrad": 13528711, "p2l": 36637552, "surface": 23238426,
-           "xgrad": 166077744}
+TRAFFIC = {"solve": 1335319, "xform": 2392602, "surfgrad": 13542455, "p2l": 36798860, "surface": 23234652,
+           "xgrad": 165854577}
 
 
 def roctx():
