@@ -74,7 +74,7 @@ def f64(a):
 
 
 def load():
-    """Load libpbsam_amd.so (built by __graft_entry__.build / `make -C pb-sam_amd`)."""
+    """Load libpbsam_amd.so (built in-tree by `make -C pb-sam_amd`)."""
     global _LIB
     if _LIB is not None:
         return _LIB

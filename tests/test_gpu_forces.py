@@ -48,6 +48,6 @@ def test_forces_reference_order(gpu, orc, n_sph, p, npts):
                 s = slice(d * p * p, (d + 1) * p * p)
                 assert rot_close(gH[s], oH[s], p, 1e-8), (i, k, d, "gH")
                 assert rot_close(gL[s], oL[s], p, 1e-8), (i, k, d, "gLHN")
-    # forces and torques to the north-star bar (1e-6 relative), expected far tighter
+    # forces and torques: 1e-6 relative is required, far tighter is expected
     assert relerr(fg, fo) < 1e-8, (fg, fo)
     assert relerr(tg, to) < 1e-8, (tg, to)
