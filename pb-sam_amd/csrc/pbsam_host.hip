@@ -1316,6 +1316,7 @@ struct GradState {
   DBuf<unsigned int> s3cnt;                // [row], zero between launches
   DBuf<const double *> yptr;               // [sphere] harmonic rows of its exposed points
   PinBuf<int> h_sb;                        // step lists staged for upload
+  PinBuf<int> h_pairs, h_lists;            // gradient re-expansion batches: pair and chunk lists
   PinBuf<double> h_dev;                    // per-step changes read back
 };
 
