@@ -120,6 +120,8 @@ int pbsam_batch_create(pbsam_ctx *ctx, int n_sys, const int *n_mol, pbsam_moltyp
 /* per-group results of the last solve / gradient (arrays of n_sys, any may be NULL):
  * status (PBSAM_OK, PBSAM_ERR_OVERLAP, PBSAM_ERR_NOCONV), mutual rounds, gradient passes */
 int pbsam_batch_status(const pbsam_system *sys, int *status, int *mutual_rounds, int *grad_passes);
+/* Changing a pose or the cutoffs discards the last solution: pbsam_gradient, pbsam_forces and
+ * pbsam_system_get_grad return PBSAM_ERR_STATE until the next pbsam_solve_mutual. */
 int pbsam_system_set_pose(pbsam_system *sys, int i, const double *pos, const double *quat);
 int pbsam_system_set_cutoffs(pbsam_system *sys, double inter_rcutoff, double interact_rcutoff);
 

@@ -335,7 +335,11 @@ extern "C" void orc_mul_expan(orc_ctx *c, int n, const double *q, const double *
   mul_expan(c, n, q, xyz, p, bkappa != 0, scale, out);
 }
 
-// CGradExpan(g,pos,p,bKappa,scale,false) triexpan.cpp:127-153 (local variant)
+// CGradExpan(g,pos,p,bKappa,scale,false) triexpan.cpp:127-153 (local variant); the
+// reference skips charges with |g|^2 < 1e-15 (triexpan.cpp:136,146).  The threshold is a
+// setting of the oracle so that tests can measure what the skip leaves out of the forces.
+static double g_grad_skip = 1e-15;
+extern "C" void orc_set_grad_skip(double thr) { g_grad_skip = thr; }
 static void grad_local(const orc_ctx *c, int n, const double *g, const double *xyz, int p,
                        bool bKappa, double scale, double *out3) {
   int len = p * p;
@@ -343,7 +347,7 @@ static void grad_local(const orc_ctx *c, int n, const double *g, const double *x
   vec Y(len), bes(p + 1), L(len);
   for (int i = 0; i < n; i++) {
     const double *gi = g + 3 * i;
-    if (dot3(gi, gi) < 1e-15) continue;
+    if (dot3(gi, gi) < g_grad_skip || dot3(gi, gi) == 0.0) continue;
     for (int k = 0; k < len; k++) L[k] = 0.0;
     add_local_1(c, 1.0, xyz + 3 * i, p, bKappa, scale, L.data(), Y.data(), bes.data());
     for (int d = 0; d < 3; d++)

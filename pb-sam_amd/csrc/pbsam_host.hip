@@ -2326,6 +2326,8 @@ int pbsam_system_set_pose(pbsam_system *S, int i, const double *pos, const doubl
   if (!S || i < 0 || i >= S->nmol || !pos || !quat) return PBSAM_ERR_BADARG;
   mol_pose(S->mols[i], pos, quat);
   S->lists_ok = false;
+  S->solved = false;   // the solution and gradients belong to the old pose
+  S->grad_ok = false;
   return PBSAM_OK;
 }
 
@@ -2334,6 +2336,8 @@ int pbsam_system_set_cutoffs(pbsam_system *S, double inter_rc, double interact_r
   S->interRc = inter_rc;
   S->interactRc = interact_rc;
   S->lists_ok = false;
+  S->solved = false;   // the pair lists behind the solution change with the cutoffs
+  S->grad_ok = false;
   return PBSAM_OK;
 }
 

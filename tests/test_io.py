@@ -1,7 +1,10 @@
 """The reference's file formats (pbsam_read_* / pbsam_write_*) against the reference's own
 readers and writers (readpqr, writeMat_Unformatted / readMat_Unformatted, writeExpansion /
 readExpansion, readConfigWithMolType compiled from its sources; tests/golden/io_ref.npz,
-written by tests/golden/make_io_fixtures.py).  Host code only: no device is touched."""
+written by tests/golden/make_io_fixtures.py).  Host code only: no device is touched.
+
+Every test runs twice, once in the CPU set and once marked `gpu`, so that the run on the
+MI355X also checks the readers and writers of the library it loads there."""
 import os
 
 import numpy as np
@@ -16,8 +19,8 @@ def fx():
     return np.load(FIX)
 
 
-@pytest.fixture(scope="module")
-def lib():
+@pytest.fixture(scope="module", params=[pytest.param("host"), pytest.param("gpu-set", marks=pytest.mark.gpu)])
+def lib(request):
     import pbsam_amd
     pbsam_amd.load()
     return pbsam_amd
@@ -134,3 +137,13 @@ def test_directory_layouts(lib, tmp_path):
     # text at precision 8
     np.testing.assert_allclose(F2, F, rtol=1e-7, atol=1e-15)
     np.testing.assert_allclose(H2, H, rtol=1e-7, atol=1e-15)
+
+
+@pytest.mark.parametrize("head", ["-6 0.25 30.0\n", "0 0.25 30.0\n", "100000 0.25 30.0\n", "x 1 2\n"])
+def test_expansion_bad_order_is_an_error(lib, tmp_path, head):
+    # a malformed order (negative, zero, absurd or not a number) is rejected rather than read
+    # into a buffer sized for the caller's order
+    path = tmp_path / "bad.exp"
+    path.write_text(head + " ".join(["1.0"] * 50) + "\n")
+    with pytest.raises(lib.PbsamError):
+        lib.read_expansion(path, 4)
