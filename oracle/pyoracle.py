@@ -88,6 +88,12 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = I
+        L.orc_debug_full_order.argtypes = [I]
+        L.orc_debug_full_order.restype = None
+        L.orc_set_grad_skip.argtypes = [D]
+        L.orc_set_grad_skip.restype = None
+        L.orc_set_ext_sign.argtypes = [D]
+        L.orc_set_ext_sign.restype = None
         L.orc_moltype_create.restype = P
         L.orc_moltype_create.argtypes = [P, ctypes.POINTER(MolDesc)]
         L.orc_system_create.restype = P
@@ -255,6 +261,21 @@ class Oracle:
     @staticmethod
     def npts_rule(r, nin=10000):
         return lib().orc_npts_rule(r, nin)
+
+
+def exact_derivative(on):
+    """Switch the oracle between the reference's gradient phase (on=False, the default) and
+    the exact derivative of its energy (on=True): full truncation orders everywhere
+    (computeOrder, xforms.cpp:402-414, returns p), no skipping of small gradient charges
+    (triexpan.cpp:136,146), and the external spheres' geometric derivative with the sign
+    of d/d(position of molecule j) (prepareDTA_LowMemory, molecule.cpp:1495-1521, adds
+    d/dP with P = c_i - c_m for a sphere m of molecule j, whose motion changes P by minus
+    its displacement).  The mutual solve must refresh Hrot for an exact fixed point
+    (System.mutual(hrot_refresh=True)).  Process-wide: restore with on=False."""
+    L = lib()
+    L.orc_debug_full_order(1 if on else 0)
+    L.orc_set_grad_skip(0.0 if on else 1e-15)
+    L.orc_set_ext_sign(-1.0 if on else 1.0)
 
 
 class MolType:
