@@ -14,6 +14,8 @@
 #ifndef PBSAM_AMD_H
 #define PBSAM_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -114,7 +116,10 @@ void pbsam_system_destroy(pbsam_system *sys);
  * of that group alone up to the rounding of different launch shapes), while the groups'
  * Gauss-Seidel chains share kernel launches.
  * The batched counterpart of calling CMolecule::computeForces (src/moldynamics.cpp:554-587)
- * once per independent system; there is no single reference entry point for it. */
+ * once per independent system; there is no single reference entry point for it.
+ * Systems may use different pole orders (the molecules of one system share one): the
+ * handle then holds one batch per order and every call runs them in turn, with results
+ * and indices in the caller's order. */
 int pbsam_batch_create(pbsam_ctx *ctx, int n_sys, const int *n_mol, pbsam_moltype *const *mts,
                        const double *pos, const double *quat, pbsam_system **out);
 /* per-group results of the last solve / gradient (arrays of n_sys, any may be NULL):
@@ -138,6 +143,28 @@ int pbsam_system_set_cutoffs(pbsam_system *sys, double inter_rcutoff, double int
  * other groups are solved, an overlapping group gets no pairs; rounds_out is the largest
  * group's count). */
 #define PBSAM_HROT_REFRESH 1
+/* System settings (pbsam_system_set_flags) beyond the reference's behaviour.  Together,
+ * PBSAM_EXACT_DERIVATIVE, they make pbsam_forces the exact derivative of the energy of the
+ * converged solution (-d/dr of sum_k <H_k, LS_k> per molecule, orientation applied as the
+ * reference does):
+ *   PBSAM_HROT_REFRESH  the mutual solve refreshes the rotated H after each update (the
+ *                       reference keeps round-1 values, so its fixed point is not
+ *                       self-consistent)
+ *   PBSAM_FULL_ORDERS   no adaptive truncation (computeOrder, src/xforms.cpp:402-414,
+ *                       returns p everywhere)
+ *   PBSAM_NO_GRAD_SKIP  gradient surface charges with |g|^2 < 1e-15 are kept (the reference
+ *                       skips them, src/triexpan.cpp:136,146)
+ *   PBSAM_EXT_SIGN      the geometric derivative of a re-expansion from a sphere of molecule
+ *                       j onto a sphere of another molecule enters the gradient of j as -d/dP
+ *                       (P = c_i - c_m); the reference adds +d/dP
+ *                       (prepareDTA_LowMemory, src/molecule.cpp:1495-1521), which leaves its
+ *                       forces 0.5-2 % off the energy gradient
+ * Setting flags discards the last solution (as pbsam_system_set_pose does). */
+#define PBSAM_FULL_ORDERS 2
+#define PBSAM_NO_GRAD_SKIP 4
+#define PBSAM_EXT_SIGN 8
+#define PBSAM_EXACT_DERIVATIVE 15
+int pbsam_system_set_flags(pbsam_system *sys, unsigned flags);
 int pbsam_solve_mutual(pbsam_system *sys, double tol, int max_rounds, unsigned flags, int *rounds_out);
 /* round 1 from self values, then a fixed number of simultaneous (Jacobi) rounds */
 int pbsam_mutual_rounds(pbsam_system *sys, int rounds, unsigned flags, double *maxdev_out);

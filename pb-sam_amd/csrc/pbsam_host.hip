@@ -10,7 +10,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1306,6 +1308,7 @@ struct GradState {
   DBuf<RAJob> ra, ra2;
   DBuf<int> ia, ib, ic, tqmap;             // scratch lists of the self-value batches
   DBuf<int> dpairs;                        // (source, target) sphere pairs of a re-expansion batch
+  DBuf<int> ext;                           // entries whose geometric derivative changes sign
   DBuf<double> lab, rad;                   // spheres' lab centres and radii
   DBuf<double> pts, mol7, cen3;            // potential queries: points, poses, sphere centres
   DBuf<int> is_off, is_lst, ps_off, ps_lst, ident, esph, eitem, sbuf;
@@ -1329,6 +1332,10 @@ struct pbsam_system {
   int max_nexp = 0;
   std::vector<MolInst> mols;
   double interRc = 1e300, interactRc = 1e300;
+  unsigned flags = 0;  // PBSAM_HROT_REFRESH, PBSAM_FULL_ORDERS, PBSAM_NO_GRAD_SKIP, PBSAM_EXT_SIGN
+  // totals for the adaptive truncation order, or null (every kernel then uses full order)
+  const double *tq(const double *p) const { return (flags & PBSAM_FULL_ORDERS) ? nullptr : p; }
+  bool full() const { return (flags & PBSAM_FULL_ORDERS) != 0; }
   // static per-sphere data (copied from the molecule types)
   DBuf<double> d_pts, d_consts, d_Dfix, d_Vfix;
   DBuf<const double *> d_imat;
@@ -1372,6 +1379,9 @@ struct pbsam_system {
   std::vector<int> grp_passes;  // gradient passes of each group's last gradient phase
   std::vector<char> grp_skip;   // groups left out of the next force evaluation (none when empty)
   std::shared_ptr<BdState> bd;  // trajectories, when the batch runs Brownian dynamics
+  // a batch of systems with different pole orders: one part per order (pbsam_host_mixed.inc)
+  std::vector<std::unique_ptr<pbsam_system>> parts;
+  std::shared_ptr<struct MixedMap> mixed;
 };
 
 static void system_static(pbsam_system *S) {
@@ -1718,7 +1728,8 @@ static double system_round(pbsam_system *S, bool round1, bool refresh) {
   XArgs xa{};
   xa.jobs = S->d_xinter.p;
   xa.H = round1 ? S->d_HselfRot.p : S->d_Hrot.p;
-  xa.TQH = round1 ? S->d_TQHself.p : S->d_TQH.p;
+  xa.TQH = S->tq(round1 ? S->d_TQHself.p : S->d_TQH.p);
+  xa.full = S->full();
   xa.slotH = S->d_slotI.p;
   xa.p = p;
   launch_trans(c, S->n_inter, S->d_tinter.p, p, S->d_Tinter.p, nullptr, xa.TQH);
@@ -1751,8 +1762,9 @@ static double system_round(pbsam_system *S, bool round1, bool refresh) {
     xb.jobs = S->d_xintra.p;
     xb.H = S->d_H.p;
     xb.F = S->d_F.p;
-    xb.TQH = S->d_TQH.p;
-    xb.TQF = S->d_TQF.p;
+    xb.TQH = S->tq(S->d_TQH.p);
+    xb.TQF = S->tq(S->d_TQF.p);
+    xb.full = S->full();
     xb.slotH = S->d_slotH.p;
     xb.slotF = S->d_slotF.p;
     xb.p = p;
@@ -1762,8 +1774,8 @@ static double system_round(pbsam_system *S, bool round1, bool refresh) {
     pa.xyz = S->d_pts.p;
     pa.q1 = S->d_qF.p;
     pa.q2 = S->d_qH.p;
-    pa.TQ1 = S->d_TQF.p;
-    pa.TQ2 = S->d_TQH.p;
+    pa.TQ1 = S->tq(S->d_TQF.p);
+    pa.TQ2 = S->tq(S->d_TQH.p);
     pa.out1 = S->d_slotF.p;
     pa.out2 = S->d_slotH.p;
     pa.kap1 = 0;
@@ -1861,7 +1873,8 @@ static void system_gs_round(pbsam_system *S, const std::vector<double> &dev, con
   XArgs xa{};
   xa.jobs = S->d_xinter.p;
   xa.H = S->d_Hrot.p;
-  xa.TQH = S->d_TQH.p;
+  xa.TQH = S->tq(S->d_TQH.p);
+  xa.full = S->full();
   xa.slotH = S->d_slotI.p;
   xa.p = p;
   LArgs la{S->d_pol.p, S->d_inter_off.p, S->d_inter_lst.p, S->d_slotI.p, nullptr, S->d_LSrot.p, nullptr, p};
@@ -1869,8 +1882,9 @@ static void system_gs_round(pbsam_system *S, const std::vector<double> &dev, con
   xb.jobs = S->d_xintra.p;
   xb.H = S->d_H.p;
   xb.F = S->d_F.p;
-  xb.TQH = S->d_TQH.p;
-  xb.TQF = S->d_TQF.p;
+  xb.TQH = S->tq(S->d_TQH.p);
+  xb.TQF = S->tq(S->d_TQF.p);
+  xb.full = S->full();
   xb.slotH = S->d_slotH.p;
   xb.slotF = S->d_slotF.p;
   xb.p = p;
@@ -1879,8 +1893,8 @@ static void system_gs_round(pbsam_system *S, const std::vector<double> &dev, con
   pa.xyz = S->d_pts.p;
   pa.q1 = S->d_qF.p;
   pa.q2 = S->d_qH.p;
-  pa.TQ1 = S->d_TQF.p;
-  pa.TQ2 = S->d_TQH.p;
+  pa.TQ1 = S->tq(S->d_TQF.p);
+  pa.TQ2 = S->tq(S->d_TQH.p);
   pa.out1 = S->d_slotF.p;
   pa.out2 = S->d_slotH.p;
   pa.kap1 = 0;
@@ -1910,7 +1924,7 @@ static void system_gs_round(pbsam_system *S, const std::vector<double> &dev, con
   for (size_t s0 = 0; s0 < nsteps; s0++) {
     const std::array<int, 12> &r = rng[s0];
     if (r[1]) {
-      launch_trans(c, r[1], S->d_tinter.p, p, S->d_Tinter.p, sb + r[0], S->d_TQH.p);
+      launch_trans(c, r[1], S->d_tinter.p, p, S->d_Tinter.p, sb + r[0], S->tq(S->d_TQH.p));
       xa.sel = sb + r[0];
       launch_xform(c, r[1], xa);
     }
@@ -1991,6 +2005,7 @@ static void system_phase0(pbsam_system *S) {
 }
 
 #include "pbsam_host_grad.inc"
+#include "pbsam_host_mixed.inc"
 
 // ----------------------------------------------------------------------------
 // C ABI
@@ -2307,11 +2322,40 @@ int pbsam_system_create(pbsam_ctx *c, int n_mol, pbsam_moltype *const *mts, cons
 
 int pbsam_batch_create(pbsam_ctx *c, int n_sys, const int *n_mol, pbsam_moltype *const *mts, const double *pos,
                        const double *quat, pbsam_system **out) {
+  if (c && n_sys > 0 && n_mol && mts && pos && quat && out) {
+    long long tot = 0;
+    bool ok = true;
+    for (int g = 0; g < n_sys; g++) {
+      ok = ok && n_mol[g] > 0;
+      tot += n_mol[g];
+    }
+    for (long long i = 0; ok && i < tot && i < (1 << 24); i++) ok = mts[i] && mts[i]->c == c;
+    if (ok && tot <= (1 << 24))
+      for (long long i = 1; i < tot; i++)
+        if (mts[i]->p != mts[0]->p) {
+          *out = nullptr;
+          return guarded(&c->err, [&]() { return mixed_make(c, n_sys, n_mol, mts, pos, quat, out); });
+        }
+  }
   return system_make(c, n_sys, n_mol, mts, pos, quat, out);
 }
 
 int pbsam_batch_status(const pbsam_system *S, int *status, int *mutual_rounds, int *grad_passes) {
   if (!S) return PBSAM_ERR_BADARG;
+  if (S->mixed) {
+    for (size_t pi = 0; pi < S->parts.size(); pi++) {
+      const pbsam_system *P = S->parts[pi].get();
+      std::vector<int> a(P->ngrp), b(P->ngrp), d(P->ngrp);
+      pbsam_batch_status(P, a.data(), b.data(), d.data());
+      for (int l = 0; l < P->ngrp; l++) {
+        const int g = S->mixed->grps[pi][l];
+        if (status) status[g] = a[l];
+        if (mutual_rounds) mutual_rounds[g] = b[l];
+        if (grad_passes) grad_passes[g] = d[l];
+      }
+    }
+    return PBSAM_OK;
+  }
   for (int g = 0; g < S->ngrp; g++) {
     if (status) status[g] = S->grp_rc.empty() ? PBSAM_OK : S->grp_rc[g];
     if (mutual_rounds) mutual_rounds[g] = S->grp_rounds.empty() ? 0 : S->grp_rounds[g];
@@ -2324,6 +2368,8 @@ void pbsam_system_destroy(pbsam_system *S) { delete S; }
 
 int pbsam_system_set_pose(pbsam_system *S, int i, const double *pos, const double *quat) {
   if (!S || i < 0 || i >= S->nmol || !pos || !quat) return PBSAM_ERR_BADARG;
+  if (S->mixed)
+    return pbsam_system_set_pose(S->parts[S->mixed->part_of_mol[i]].get(), S->mixed->local_mol[i], pos, quat);
   mol_pose(S->mols[i], pos, quat);
   S->lists_ok = false;
   S->solved = false;   // the solution and gradients belong to the old pose
@@ -2333,6 +2379,7 @@ int pbsam_system_set_pose(pbsam_system *S, int i, const double *pos, const doubl
 
 int pbsam_system_set_cutoffs(pbsam_system *S, double inter_rc, double interact_rc) {
   if (!S) return PBSAM_ERR_BADARG;
+  for (auto &P : S->parts) pbsam_system_set_cutoffs(P.get(), inter_rc, interact_rc);
   S->interRc = inter_rc;
   S->interactRc = interact_rc;
   S->lists_ok = false;
@@ -2341,15 +2388,37 @@ int pbsam_system_set_cutoffs(pbsam_system *S, double inter_rc, double interact_r
   return PBSAM_OK;
 }
 
+int pbsam_system_set_flags(pbsam_system *S, unsigned flags) {
+  if (!S || (flags & ~(unsigned)PBSAM_EXACT_DERIVATIVE)) return PBSAM_ERR_BADARG;
+  for (auto &P : S->parts) pbsam_system_set_flags(P.get(), flags);
+  S->flags = flags;
+  S->lists_ok = false;
+  S->solved = false;   // the solution was made under the old settings
+  S->grad_ok = false;
+  return PBSAM_OK;
+}
+
 int pbsam_mutual_rounds(pbsam_system *S, int rounds, unsigned flags, double *maxdev_out) {
   if (!S || rounds < 0) return PBSAM_ERR_BADARG;
+  if (S->mixed) {
+    double md = 0.0;
+    for (auto &P : S->parts) {
+      double m = 0.0;
+      const int rc = pbsam_mutual_rounds(P.get(), rounds, flags, &m);
+      if (rc != PBSAM_OK) return rc;
+      md = std::max(md, m);
+    }
+    if (maxdev_out) *maxdev_out = md;
+    return PBSAM_OK;
+  }
   return guarded(&S->c->err, [&]() {
     int rc = system_lists(S);
     if (rc != PBSAM_OK) return rc;
     if (S->nmol == 1) return PBSAM_OK;
     system_phase0(S);
-    double md = system_round(S, true, (flags & PBSAM_HROT_REFRESH) != 0);
-    for (int r = 0; r < rounds; r++) md = system_round(S, false, (flags & PBSAM_HROT_REFRESH) != 0);
+    const bool refresh = ((flags | S->flags) & PBSAM_HROT_REFRESH) != 0;
+    double md = system_round(S, true, refresh);
+    for (int r = 0; r < rounds; r++) md = system_round(S, false, refresh);
     S->solved = true;
     if (maxdev_out) *maxdev_out = md;
     return PBSAM_OK;
@@ -2365,6 +2434,17 @@ static int groups_rc(const pbsam_system *S) {
 
 int pbsam_solve_mutual(pbsam_system *S, double tol, int max_rounds, unsigned flags, int *rounds_out) {
   if (!S) return PBSAM_ERR_BADARG;
+  if (S->mixed) {
+    int rmax = 0;
+    for (auto &P : S->parts) {
+      int r = 0;
+      const int rc = pbsam_solve_mutual(P.get(), tol, max_rounds, flags, &r);
+      if (rc != PBSAM_OK && rc != PBSAM_ERR_NOCONV && rc != PBSAM_ERR_OVERLAP) return rc;
+      rmax = std::max(rmax, r);
+    }
+    if (rounds_out) *rounds_out = rmax;
+    return mixed_groups_rc(S);
+  }
   return guarded(&S->c->err, [&]() {
     HIPCHK(hipSetDevice(S->c->device));
     int rc = system_lists(S);
@@ -2374,7 +2454,7 @@ int pbsam_solve_mutual(pbsam_system *S, double tol, int max_rounds, unsigned fla
       if (rounds_out) *rounds_out = 0;
       return groups_rc(S);
     }
-    bool refresh = (flags & PBSAM_HROT_REFRESH) != 0;
+    bool refresh = ((flags | S->flags) & PBSAM_HROT_REFRESH) != 0;
     double crit = 0.0;
     int r = system_mutual_ref(S, tol, max_rounds, refresh, &crit);
     S->solved = true;
@@ -2384,6 +2464,8 @@ int pbsam_solve_mutual(pbsam_system *S, double tol, int max_rounds, unsigned fla
 }
 
 int pbsam_system_get(const pbsam_system *S, int i, int k, double *F, double *H, double *LS, double *Hrot) {
+  if (S && S->mixed && i >= 0 && i < S->nmol)
+    return pbsam_system_get(S->parts[S->mixed->part_of_mol[i]].get(), S->mixed->local_mol[i], k, F, H, LS, Hrot);
   if (!S || i < 0 || i >= S->nmol || k < 0 || k >= S->mols[i].mt->ns) return PBSAM_ERR_BADARG;
   return guarded(&S->c->err, [&]() {
     hipStream_t s = S->c->stream;
@@ -2400,6 +2482,7 @@ int pbsam_system_get(const pbsam_system *S, int i, int k, double *F, double *H, 
 
 int pbsam_energy(pbsam_system *S, double *e) {
   if (!S || !e) return PBSAM_ERR_BADARG;
+  if (S->mixed) return mixed_per_mol(S, 1, e, [](pbsam_system *P, double *o) { return pbsam_energy(P, o); });
   return guarded(&S->c->err, [&]() {
     std::vector<double> H = S->d_H.download(S->c->stream), LS = S->d_LS.download(S->c->stream);
     const int p = S->p, P2 = p * p;
@@ -2426,6 +2509,16 @@ int pbsam_energy(pbsam_system *S, double *e) {
 
 int pbsam_system_counts(const pbsam_system *S, int *n_inter_pairs, int *n_polarized) {
   if (!S) return PBSAM_ERR_BADARG;
+  if (S->mixed) {
+    int a = 0, b = 0;
+    for (auto &P : S->parts) {
+      a += P->n_inter;
+      b += (int)P->pol.size();
+    }
+    if (n_inter_pairs) *n_inter_pairs = a;
+    if (n_polarized) *n_polarized = b;
+    return PBSAM_OK;
+  }
   if (n_inter_pairs) *n_inter_pairs = S->n_inter;
   if (n_polarized) *n_polarized = (int)S->pol.size();
   return PBSAM_OK;
@@ -2433,6 +2526,17 @@ int pbsam_system_counts(const pbsam_system *S, int *n_inter_pairs, int *n_polari
 
 int pbsam_gradient(pbsam_system *S, double tol, int max_rounds, int *passes_out) {
   if (!S) return PBSAM_ERR_BADARG;
+  if (S->mixed) {
+    int tot = 0;
+    for (auto &P : S->parts) {
+      int n = 0;
+      const int rc = pbsam_gradient(P.get(), tol, max_rounds, &n);
+      if (rc != PBSAM_OK) return rc;
+      tot += n;
+    }
+    if (passes_out) *passes_out = tot;
+    return PBSAM_OK;
+  }
   if (!S->solved) {
     S->c->err = "pbsam_gradient needs a mutual solve of the current poses";
     return PBSAM_ERR_STATE;
@@ -2448,6 +2552,20 @@ int pbsam_gradient(pbsam_system *S, double tol, int max_rounds, int *passes_out)
 
 int pbsam_forces(pbsam_system *S, double *force, double *torque) {
   if (!S || !force || !torque) return PBSAM_ERR_BADARG;
+  if (S->mixed) {
+    for (size_t pi = 0; pi < S->parts.size(); pi++) {
+      pbsam_system *P = S->parts[pi].get();
+      std::vector<double> f(3 * (size_t)P->nmol), tq(3 * (size_t)P->nmol);
+      const int r = pbsam_forces(P, f.data(), tq.data());
+      if (r != PBSAM_OK) return r;
+      for (int l = 0; l < P->nmol; l++)
+        for (int x = 0; x < 3; x++) {
+          force[3 * S->mixed->mols[pi][l] + x] = f[3 * l + x];
+          torque[3 * S->mixed->mols[pi][l] + x] = tq[3 * l + x];
+        }
+    }
+    return PBSAM_OK;
+  }
   if (!S->grad_ok) {
     S->c->err = "pbsam_forces needs pbsam_gradient first";
     return PBSAM_ERR_STATE;
@@ -2463,6 +2581,24 @@ int pbsam_compute_forces(pbsam_system *S, double tol, int max_rounds, double *fo
                          int *rounds_out) {
   if (!S || !force || !torque) return PBSAM_ERR_BADARG;
   for (int i = 0; i < 3 * S->nmol; i++) force[i] = torque[i] = 0.0;
+  if (S->mixed) {
+    int rmax = 0;
+    for (size_t pi = 0; pi < S->parts.size(); pi++) {
+      pbsam_system *P = S->parts[pi].get();
+      std::vector<double> f(3 * (size_t)P->nmol), tq(3 * (size_t)P->nmol);
+      int r = 0;
+      const int rc = pbsam_compute_forces(P, tol, max_rounds, f.data(), tq.data(), &r);
+      if (rc != PBSAM_OK && rc != PBSAM_ERR_NOCONV && rc != PBSAM_ERR_OVERLAP) return rc;
+      rmax = std::max(rmax, r);
+      for (int l = 0; l < P->nmol; l++)
+        for (int x = 0; x < 3; x++) {
+          force[3 * S->mixed->mols[pi][l] + x] = f[3 * l + x];
+          torque[3 * S->mixed->mols[pi][l] + x] = tq[3 * l + x];
+        }
+    }
+    if (rounds_out) *rounds_out = rmax;
+    return mixed_groups_rc(S);
+  }
   if (S->nmol == 1) {
     if (rounds_out) *rounds_out = 0;
     return PBSAM_OK;
@@ -2481,6 +2617,8 @@ int pbsam_compute_forces(pbsam_system *S, double tol, int max_rounds, double *fo
 }
 
 int pbsam_system_get_grad(const pbsam_system *S, int i, int k, double *gH, double *gLHN) {
+  if (S && S->mixed && i >= 0 && i < S->nmol)
+    return pbsam_system_get_grad(S->parts[S->mixed->part_of_mol[i]].get(), S->mixed->local_mol[i], k, gH, gLHN);
   if (!S || i < 0 || i >= S->nmol || k < 0 || k >= S->mols[i].mt->ns) return PBSAM_ERR_BADARG;
   if (!S->grad_ok) return PBSAM_ERR_STATE;
   return guarded(&S->c->err, [&]() {

@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(NT) k_xgrad(Tabs tb, GArgs A) {
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
   double rho, th, ph;
   cart_to_sph(J.P[0], J.P[1], J.P[2], rho, th, ph);
-  const int q = A.ord ? A.ord[blockIdx.x] : compute_order(rho, A.TQ[J.src], J.a_src, p);
+  const int q = A.ord ? A.ord[blockIdx.x] : (A.TQ ? compute_order(rho, A.TQ[J.src], J.a_src, p) : p);
   const int tq = J.tord ? J.tord : q;
   const double *T = J.TH, *dT = J.TH + tmat_size(tq);
   double *G = A.out + (size_t)J.slot * 3 * P2;
@@ -1046,8 +1046,9 @@ __global__ void __launch_bounds__(MT) k_p2lgemv(MArgs A) {
   __shared__ double qs[MPTS][2 * R];
   const MJob J = A.jobs[A.sel ? A.sel[blockIdx.x] : blockIdx.x];
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
-  const int pF = compute_order(J.rho, A.TQF[J.src], J.a_src, p);
-  const int pH = compute_order(J.rho, A.TQH[J.src], J.a_src, p);
+  // null totals: full order (no adaptive truncation)
+  const int pF = A.TQF ? compute_order(J.rho, A.TQF[J.src], J.a_src, p) : p;
+  const int pH = A.TQH ? compute_order(J.rho, A.TQH[J.src], J.a_src, p) : p;
   const int nF = pF * pF, nH = pH * pH;
   const int c0 = blockIdx.z * (MCW * MT);
   const int hb = blockIdx.y * A.spl, he = min(J.npts, hb + A.spl);
@@ -1209,7 +1210,8 @@ __global__ void __launch_bounds__(256) k_p2lgemm(PGArgs A) {
     if (tid < nj) {
       jb = A.bent[b0 + j0 + tid];
       const MJob J = A.jobs[jb];
-      const int o = compute_order(J.rho, (hf ? A.TQH : A.TQF)[J.src], J.a_src, p);
+      const double *TQ = hf ? A.TQH : A.TQF;  // null: full order
+      const int o = TQ ? compute_order(J.rho, TQ[J.src], J.a_src, p) : p;
       q = o * o;
       po = J.pt_off;
     }
@@ -1244,7 +1246,7 @@ __global__ void __launch_bounds__(256) k_p2lgemm(PGArgs A) {
         double v = 0.0;
         if (jj < nj && h < npts) {
           const size_t pt = (size_t)pto_s[jj] + h;
-          v = (nrm[pt] < 1e-15) ? 0.0 : q[(size_t)r * A.plane + pt];
+          v = (nrm && nrm[pt] < 1e-15) ? 0.0 : q[(size_t)r * A.plane + pt];  // null: no skip
         }
         Bs[kk * 3 * PGJ + col] = v;
       }
@@ -1593,3 +1595,12 @@ __global__ void __launch_bounds__(256) k_surfgemm_k4(SGArgs A) {
 
 #include "pbsam_kernels_imat.hip"
 #include "pbsam_kernels_pot.hip"
+
+// blocks listed in blk (B doubles each) multiplied by -1: one workgroup per (block, chunk)
+__global__ void __launch_bounds__(256) k_negblk(const int *blk, int B, double *out) {
+  const int i = blockIdx.y * 256 + threadIdx.x;
+  if (i < B) {
+    double *o = out + (size_t)blk[blockIdx.x] * B + i;
+    *o = -*o;
+  }
+}

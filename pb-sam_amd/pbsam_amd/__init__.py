@@ -17,6 +17,8 @@ LIBPATH = os.path.join(os.path.dirname(HERE), "libpbsam_amd.so")
 
 OK, ERR_BADARG, ERR_OVERLAP, ERR_NOCONV, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED, ERR_CHARGE, ERR_STATE = range(9)
 HROT_REFRESH = 1
+FULL_ORDERS, NO_GRAD_SKIP, EXT_SIGN = 2, 4, 8
+EXACT_DERIVATIVE = HROT_REFRESH | FULL_ORDERS | NO_GRAD_SKIP | EXT_SIGN
 PMAX = 32
 
 D = ctypes.c_double
@@ -37,7 +39,7 @@ SYMBOLS = [
     "pbsam_bd_defaults", "pbsam_bd_start", "pbsam_bd_contacts", "pbsam_bd_run", "pbsam_bd_get",
     "pbsam_read_pqr", "pbsam_write_imat", "pbsam_read_imat", "pbsam_write_expansion", "pbsam_read_expansion",
     "pbsam_read_config", "pbsam_write_config", "pbsam_read_imats", "pbsam_read_expansions",
-    "pbsam_potential", "pbsam_total_int_energy",
+    "pbsam_potential", "pbsam_total_int_energy", "pbsam_system_set_flags",
 ]
 POT_REFERENCE_FRAME = 1
 BD_ESCAPED, BD_DOCKED, BD_RUNNING, BD_STUCK, BD_FAILED = range(5)
@@ -119,6 +121,7 @@ def load():
         "pbsam_system_destroy": ([P], None),
         "pbsam_system_set_pose": ([P, I, P, P], I),
         "pbsam_system_set_cutoffs": ([P, D, D], I),
+        "pbsam_system_set_flags": ([P, ctypes.c_uint], I),
         "pbsam_solve_mutual": ([P, D, I, ctypes.c_uint, P], I),
         "pbsam_mutual_rounds": ([P, I, ctypes.c_uint, P], I),
         "pbsam_system_get": ([P, I, I, P, P, P, P], I),
@@ -430,6 +433,11 @@ class System:
     def set_cutoffs(self, inter, interact):
         self.ctx.check(self.ctx.L.pbsam_system_set_cutoffs(self.h, inter, interact))
 
+    def set_flags(self, flags):
+        """pbsam_system_set_flags: EXACT_DERIVATIVE (or its parts) for forces that are the
+        exact derivative of the energy; 0 for the reference's behaviour."""
+        self.ctx.check(self.ctx.L.pbsam_system_set_flags(self.h, int(flags)))
+
     def set_pose(self, i, pos, quat):
         pos, quat = f64(pos), f64(quat)
         self.ctx.check(self.ctx.L.pbsam_system_set_pose(self.h, i, _p(pos), _p(quat)))
@@ -451,7 +459,8 @@ class System:
         return md.value
 
     def get(self, i, k):
-        a = [np.zeros(self.p * self.p) for _ in range(4)]
+        p = self.mts[i].p
+        a = [np.zeros(p * p) for _ in range(4)]
         self.ctx.check(self.ctx.L.pbsam_system_get(self.h, i, k, *[_p(x) for x in a]))
         return dict(F=a[0], H=a[1], LS=a[2], Hrot=a[3])
 
@@ -475,7 +484,8 @@ class System:
         return r.value, rc, f.reshape(-1, 3), t.reshape(-1, 3)
 
     def get_grad(self, i, k):
-        gH, gL = np.zeros(3 * self.p * self.p), np.zeros(3 * self.p * self.p)
+        p = self.mts[i].p
+        gH, gL = np.zeros(3 * p * p), np.zeros(3 * p * p)
         self.ctx.check(self.ctx.L.pbsam_system_get_grad(self.h, i, k, _p(gH), _p(gL)))
         return gH, gL
 

@@ -145,3 +145,62 @@ def test_batch_of_snapshots(gpu, orc, n_sph, p, npts):
     assert rco == 0
     assert relerr(fb[6:8], fo) < 1e-8
     assert relerr(tb[6:8], to) < 1e-8
+
+
+def test_batch_of_mixed_pole_orders(gpu, orc):
+    """Systems at p = 10 and p = 30 with uneven sphere counts in one batch handle (the
+    mixed-order batch workload, reduced): each group equals its molecules solved alone,
+    indices and statuses come back in the caller's order, and one group of each order
+    matches the oracle."""
+    import pbsam_amd
+    import pyoracle
+    from pbsam_amd import synth
+    spec = [(10, 5, 400), (30, 3, 1800), (10, 8, 400), (30, 4, 1800), (10, 3, 400)]
+    types = {}
+    for t, (p, n, npts) in enumerate(spec):
+        m = synth.molecule(n, seed=700 + t)
+        args = (m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], p)
+        kw = dict(eps_in=4.0, rcen=m["rcen"], npts=[npts] * n)
+        o = pyoracle.MolType(orc, *args, **kw)
+        o.selfpol(0, 0.0, 0)
+        g = pbsam_amd.MolType(gpu, *args, **kw)
+        g.set_self(*o.FH())
+        o.values()
+        types[t] = (m, g, o)
+    groups = [(0, 2), (1, 3), (2, 4), (3, 1), (4, 0), (1, 1)]
+    rng = np.random.default_rng(31)
+    poses = []
+    for a, b in groups:
+        d = synth.ball_radius(len(types[a][0]["rad"])) + synth.ball_radius(len(types[b][0]["rad"])) + \
+            rng.uniform(4.0, 12.0)
+        poses.append(synth.place_pair(types[a][0], types[b][0], d, seed=800 + len(poses)))
+    batch = pbsam_amd.System(gpu, [types[t][1] for g in groups for t in g], np.vstack([p for p, _ in poses]),
+                             np.vstack([q for _, q in poses]), groups=[2] * len(groups))
+    batch.set_cutoffs(100.0, 100.0)
+    _, rc, fb, tb = batch.compute_forces()
+    assert rc == 0
+    eb = batch.energy()
+    st, br, bp = batch.status()
+    assert list(st) == [0] * len(groups)
+    checked = set()
+    for gi, (a, b) in enumerate(groups):
+        single = pbsam_amd.System(gpu, [types[a][1], types[b][1]], *poses[gi])
+        single.set_cutoffs(100.0, 100.0)
+        _, rc1, f1, t1 = single.compute_forces()
+        _, sr, sp = single.status()
+        assert rc1 == 0 and (br[gi], bp[gi]) == (sr[0], sp[0]), gi
+        sl = slice(2 * gi, 2 * gi + 2)
+        assert relerr(fb[sl], f1) < 1e-10, (gi, fb[sl], f1)
+        assert relerr(tb[sl], t1) < 1e-10, gi
+        assert relerr(eb[sl], single.energy()) < 1e-12, gi
+        p = spec[a][0]
+        if p not in checked:
+            checked.add(p)
+            so = pyoracle.System(orc, [types[a][2], types[b][2]], *poses[gi])
+            so.set_cutoffs(100.0, 100.0)
+            rco, fo, to = so.compute_forces(0, 0.0, 0)
+            assert rco == 0
+            assert relerr(fb[sl], fo) < 1e-8 and relerr(tb[sl], to) < 1e-8, (gi, p)
+            # per-sphere blocks come back at the molecule's own order
+            assert len(batch.get(2 * gi, 0)["H"]) == p * p
+    assert checked == {10, 30}

@@ -1,0 +1,158 @@
+"""Device forces under PBSAM_EXACT_DERIVATIVE against the oracle's exact-derivative mode and
+against central differences of the device's own energy, and the device's forces and torques
+against the Debye-Hueckel closed forms.
+
+With PBSAM_EXACT_DERIVATIVE (Hrot refreshed in the mutual solve, full truncation orders,
+no skip of small gradient charges, the external spheres' geometric derivative with the
+sign of d/d(position of molecule j)) the reference's forward-mode gradient phase, run in
+its own order, is the derivative of the converged energy: orientation * returned force =
+-d<H, LS>/dr of each molecule.  The oracle pins this to 1e-9 (tests/test_oracle_forces.py);
+here the device reproduces the oracle in that mode and its own energy's central
+differences (step 1e-4 A) to 1e-6."""
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+EPS_OUT = 78.0
+
+
+def quat_rot(q, v):
+    w, u = q[0], q[1:]
+    t = np.cross(u, v)
+    return v + 2 * (w * t + np.cross(u, t))
+
+
+def conj(q):
+    return q * np.array([1.0, -1.0, -1.0, -1.0])
+
+
+def pair(gpu, orc, ns, p, npts, seeds=(4, 5)):
+    import pbsam_amd
+    import pyoracle
+    from pbsam_amd import synth
+    mols = [synth.molecule(ns, seed=s) for s in seeds]
+    gs, os_ = [], []
+    for m in mols:
+        args = (m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], p)
+        kw = dict(eps_in=4.0, rcen=m["rcen"], npts=[npts] * ns)
+        o = pyoracle.MolType(orc, *args, **kw)
+        o.selfpol(0, 0.0, 0)
+        g = pbsam_amd.MolType(gpu, *args, **kw)
+        g.set_self(*o.FH())
+        o.set_self(*o.FH())
+        gs.append(g)
+        os_.append(o)
+    pos, quat = synth.place_pair(mols[0], mols[1], 2 * synth.ball_radius(ns) + 6.0, seed=9)
+    return gs, os_, pos, quat
+
+
+def device_forces(gpu, gs, pos, quat, flags, energy_only=False, tol=1e-20, gtol=1e-16):
+    import pbsam_amd
+    S = pbsam_amd.System(gpu, gs, pos, quat)
+    S.set_cutoffs(100.0, 100.0)
+    S.set_flags(flags)
+    rounds, rc = S.solve(tol=tol, max_rounds=5000)
+    assert rc == 0, rounds
+    if energy_only:
+        return S, S.energy()
+    S.gradient(tol=gtol, max_rounds=5000)
+    return S, S.forces()
+
+
+def device_fd_gap(gpu, gs, pos, quat, h=1e-4):
+    import pbsam_amd
+    _, (f, _) = device_forces(gpu, gs, pos, quat, pbsam_amd.EXACT_DERIVATIVE)
+    gap = 0.0
+    for mol in range(2):
+        F = quat_rot(quat[mol], f[mol])
+        fd = np.zeros(3)
+        for d in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[mol, d] += h
+            pm[mol, d] -= h
+            ep = device_forces(gpu, gs, pp, quat, pbsam_amd.EXACT_DERIVATIVE, True)[1][mol]
+            em = device_forces(gpu, gs, pm, quat, pbsam_amd.EXACT_DERIVATIVE, True)[1][mol]
+            fd[d] = -(ep - em) / (2 * h)
+        gap = max(gap, np.abs(F - fd).max() / np.linalg.norm(fd))
+    return gap
+
+
+@pytest.mark.parametrize("ns,p,npts", [(4, 10, 400), (3, 30, 600)])
+def test_exact_mode_matches_oracle(gpu, orc, ns, p, npts):
+    import pbsam_amd
+    import pyoracle
+    gs, os_, pos, quat = pair(gpu, orc, ns, p, npts)
+    _, (fg, tg) = device_forces(gpu, gs, pos, quat, pbsam_amd.EXACT_DERIVATIVE)
+    pyoracle.exact_derivative(True)
+    try:
+        So = pyoracle.System(orc, os_, pos, quat)
+        So.set_cutoffs(100.0, 100.0)
+        So.lists()
+        So.mutual(0, 1e-20, 5000, hrot_refresh=True)
+        So.gradient(0, 1e-16, 5000)
+        fo, to = So.forces()
+    finally:
+        pyoracle.exact_derivative(False)
+    assert relerr(fg, fo) < 1e-8, (fg, fo)
+    assert relerr(tg, to) < 1e-8, (tg, to)
+    # and the reference's settings give different forces (the modes are not the same code)
+    _, (fr, _) = device_forces(gpu, gs, pos, quat, 0)
+    assert relerr(fr, fg) > 1e-4
+
+
+@pytest.mark.parametrize("ns,p,npts", [(6, 10, 400), (4, 30, 1800)])
+def test_exact_forces_are_the_device_energy_gradient(gpu, orc, ns, p, npts):
+    gs, _, pos, quat = pair(gpu, orc, ns, p, npts)
+    assert device_fd_gap(gpu, gs, pos, quat) < 1e-6
+
+
+def dh_pair(gpu, d, flags, q0=(1.0, 0.0, 0.0, 0.0), q1=(1.0, 0.0, 0.0, 0.0), off=(0.0, 0.0, 0.0), p=10):
+    """As tests/test_oracle_forces.py: one-sphere molecules (radii 3, 2 A; charges +1, -1 at
+    the sphere centres), the first sphere `off` from its molecule centre."""
+    import pbsam_amd
+    a = (3.0, 2.0)
+    offs = (np.array(off, float), np.zeros(3))
+    mts = []
+    for ai, qi, oi in zip(a, (1.0, -1.0), offs):
+        m = pbsam_amd.MolType(gpu, oi[None, :], np.array([ai]), oi[None, :], np.array([qi]), p, eps_in=4.0,
+                              npts=np.array([400], dtype=np.int32))
+        m.selfpol(tol=0, max_rounds=0)
+        m.values()
+        mts.append(m)
+    quat = np.array([q0, q1], float)
+    quat /= np.linalg.norm(quat, axis=1)[:, None]
+    c0, c1 = np.zeros(3), np.array([d, 0.0, 0.0])
+    pos = np.array([c0 - quat_rot(quat[0], offs[0]), c1])
+    S = pbsam_amd.System(gpu, mts, pos, quat)
+    S.set_cutoffs(1e4, 1e4)
+    S.set_flags(flags)
+    S.solve(tol=1e-24, max_rounds=400)
+    S.gradient(tol=1e-24, max_rounds=400)
+    f, t = S.forces()
+    k = gpu.kappa
+    E = -np.exp(k * a[0]) / (1 + k * a[0]) * np.exp(k * a[1]) / (1 + k * a[1]) * np.exp(-k * d) / (EPS_OUT * d)
+    E /= EPS_OUT
+    f1 = E * (k + 1.0 / d) * (c1 - c0) / d
+    return f, t, f1, offs, quat, mts
+
+
+@pytest.mark.parametrize("d,tol", [(40.0, 1e-3), (80.0, 1e-4)])
+def test_device_force_and_torque_debye_hueckel(gpu, d, tol):
+    import pbsam_amd
+    f, t, f1, _, _, _ = dh_pair(gpu, d, pbsam_amd.EXACT_DERIVATIVE)
+    assert np.abs(f[1] - f1).max() < tol * np.linalg.norm(f1), (f[1], f1)
+    assert np.abs(f[0] + f1).max() < tol * np.linalg.norm(f1), (f[0], -f1)
+    # the reference's settings: within 5e-3 / 1e-3 (its external sign breaks the symmetry)
+    f, t, f1, _, _, _ = dh_pair(gpu, d, 0)
+    assert np.abs(f[1] - f1).max() < 5 * tol * np.linalg.norm(f1) + 1e-3 * (d < 50) * np.linalg.norm(f1)
+    rng = np.random.default_rng(17)
+    q0, q1 = rng.normal(size=4), rng.normal(size=4)
+    q0, q1 = q0 / np.linalg.norm(q0), q1 / np.linalg.norm(q1)
+    off = np.array([1.0, -2.0, 2.0]) * 4.0 / 3.0
+    f, t, f1, offs, _, _ = dh_pair(gpu, d, pbsam_amd.EXACT_DERIVATIVE, q0, q1, off)
+    assert np.abs(f[0] - quat_rot(conj(q0), -f1)).max() < tol * np.linalg.norm(f1)
+    tau = quat_rot(conj(q0), np.cross(offs[0], -f1))
+    assert np.abs(t[0] - tau).max() < tol * np.linalg.norm(tau), (t[0], tau)
