@@ -112,12 +112,6 @@ def batch_leg(pbsam_amd, synth, ctx, mols, mts, nsys, reps, nplain):
     el = time.perf_counter() - t0
     if mark:
         mark.roctxRangePop()
-    # per-kernel HIP-event timings from one more, instrumented, batched call
-    ctx.profile(True)
-    batch.compute_forces(tol=0, max_rounds=0)
-    ctx.sync()
-    prof = {k: ctx.profile_read(k) for k in pbsam_amd.KERNELS}
-    ctx.profile(False)
     st, rounds, passes = batch.status()
     # the plain path on a sample of the same systems
     singles = [pbsam_amd.System(ctx, [mts[0], mts[1]], P[2 * b:2 * b + 2], Q[2 * b:2 * b + 2])
@@ -141,11 +135,79 @@ def batch_leg(pbsam_amd, synth, ctx, mols, mts, nsys, reps, nplain):
                 plain_loop_evals_per_s=plain, plain_loop_sample=nplain, speedup_vs_plain_loop=value / plain,
                 max_rel_err_force_batch_vs_plain=err, groups_ok=int((st == 0).sum()),
                 mutual_rounds_max=int(rounds.max()), grad_passes_mean=float(passes.mean()),
-                kernels={k: dict(launches=v["launches"], ms_per_batch=v["ms"],
-                                 gflop_per_batch=v["flops"] / 1e9) for k, v in prof.items()})
+                snapshots=(P, Q))
 
 
 BD_SPH, BD_P, BD_NPTS, BD_SEEDS = 100, 10, 400, (301, 302)
+MIX_POOL, MIX_SPH, MIX_SEED = 4, (20, 60), 6000
+
+
+def mixed_leg(pbsam_amd, synth, ctx, nsys, reps, nplain):
+    """Independent two-molecule systems of mixed pole orders in one batch handle (the
+    mixed-order batch workload, reduced in size): a pool of MIX_POOL molecule types at p = 10
+    (400 points per sphere) and as many at p = 30 (1800 points), sphere counts uniform in
+    MIX_SPH; each system draws its order, then two types of that order, a centre distance
+    uniform in [30, 80] A (at least the two packing radii plus 4 A) and random orientations.
+    Complete force+torque evaluations of every system, timed over `reps` batched calls after
+    one warm-up, beside the plain one-system-per-call loop on the first `nplain` systems."""
+    import numpy as np
+    rng = np.random.default_rng(MIX_SEED)
+    pool = {10: [], 30: []}
+    for p, npts in ((10, 400), (30, 1800)):
+        for t in range(MIX_POOL):
+            n = int(rng.integers(MIX_SPH[0], MIX_SPH[1] + 1))
+            m = synth.molecule(n, seed=MIX_SEED + 100 * p + t)
+            g = pbsam_amd.MolType(ctx, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], p, eps_in=synth.EPS_IN,
+                                  rcen=m["rcen"], npts=[npts] * n)
+            g.selfpol(tol=0, max_rounds=0)
+            g.values()
+            pool[p].append((m, g))
+    types, P, Q, orders = [], [], [], []
+    for b in range(nsys):
+        p = 10 if rng.random() < 0.5 else 30
+        a, c = pool[p][rng.integers(MIX_POOL)], pool[p][rng.integers(MIX_POOL)]
+        dmin = synth.ball_radius(len(a[0]["rad"])) + synth.ball_radius(len(c[0]["rad"])) + 4.0
+        pp, qq = synth.place_pair(a[0], c[0], max(rng.uniform(30.0, 80.0), dmin), seed=MIX_SEED + 1000 + b)
+        types.append((a[1], c[1]))
+        P.append(pp)
+        Q.append(qq)
+        orders.append(p)
+    batch = pbsam_amd.System(ctx, [g for t in types for g in t], np.vstack(P), np.vstack(Q), groups=[2] * nsys)
+    batch.set_cutoffs(100.0, 100.0)
+    batch.compute_forces(tol=0, max_rounds=0)
+    ctx.sync()
+    mark = roctx()
+    if mark:
+        mark.roctxRangePushA(b"mixed_region")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, rc, fb, tb = batch.compute_forces(tol=0, max_rounds=0)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    if mark:
+        mark.roctxRangePop()
+    st, _, _ = batch.status()
+    singles = [pbsam_amd.System(ctx, list(types[b]), P[b], Q[b]) for b in range(nplain)]
+    for s1 in singles:
+        s1.set_cutoffs(100.0, 100.0)
+    singles[0].compute_forces(tol=0, max_rounds=0)
+    ctx.sync()
+    err = 0.0
+    t0 = time.perf_counter()
+    for b, s1 in enumerate(singles):
+        _, _, f1, _ = s1.compute_forces(tol=0, max_rounds=0)
+        err = max(err, float(np.abs(fb[2 * b:2 * b + 2] - f1).max() / max(np.abs(f1).max(), 1e-300)))
+    ctx.sync()
+    plain = nplain / (time.perf_counter() - t0)
+    value = reps * nsys / el
+    return dict(systems=nsys, systems_p10=int(sum(o == 10 for o in orders)), systems_p30=int(sum(o == 30 for o in orders)),
+                workload="%d two-molecule systems, p in {10, 30} per system (400 / 1800 points per sphere), %d-%d "
+                         "spheres per molecule from a pool of %d types per order, centre distance U[30,80] A, random "
+                         "orientations (reduced from 40-300 spheres per molecule)" % (nsys, MIX_SPH[0], MIX_SPH[1],
+                                                                                       MIX_POOL),
+                evals_per_s=value, ms_per_batch=1e3 * el / reps, reps=reps, plain_loop_evals_per_s=plain,
+                plain_loop_sample=nplain, speedup_vs_plain_loop=value / plain, max_rel_err_force_batch_vs_plain=err,
+                groups_ok=int((st == 0).sum()))
 
 
 def bd_leg(pbsam_amd, synth, ctx, kappa, ntraj, nsteps, cpu_budget):
@@ -213,19 +275,142 @@ def bd_leg(pbsam_amd, synth, ctx, kappa, ntraj, nsteps, cpu_budget):
         b = pyoracle.BDTrajectory([(m["cen"] - m["rcen"], m["rad"]) for m in mols], int(seeds[0]), False, P[0:2],
                                   Q[0:2], prm=oprm)
         b.attach(osys)
-        n, t0, err = 0, time.perf_counter(), None
-        while True:
+        err = None
+        for _ in range(1 + nsteps):
             b.step()
-            n += 1
-            if n == 1 + nsteps:
-                err = float(np.abs(b.state()["pos"] - batch.bd_get(0)["pos"]).max())
-            tc = time.perf_counter() - t0
-            if n >= 1 + nsteps and tc >= cpu_budget:
-                break
-        out["cpu_baseline"] = dict(value=n / tc, unit="trajectory-steps/s", cores=1, kind="port",
-                                   sample="trajectory 0 of the same batch, %d steps in %.1f s on the oracle" % (n, tc))
+        err = float(np.abs(b.state()["pos"] - batch.bd_get(0)["pos"]).max())
+        # all host cores: one trajectory of the batch per worker thread (the trajectories are
+        # independent), stepped for about cpu_budget seconds
+        import threading
+        workers = host_workers()
+        counts = [0] * workers
+        FH = [g.FH() for g in mts]
+        barrier = threading.Barrier(workers)
+        t_start = []
+
+        def work(w):
+            _one_omp_thread()
+            o2 = pyoracle.Oracle(kappa)
+            om = []
+            for m, (F, H) in zip(mols, FH):
+                o = pyoracle.MolType(o2, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], BD_P, eps_in=synth.EPS_IN,
+                                     rcen=m["rcen"], npts=[BD_NPTS] * BD_SPH)
+                o.set_self(F, H)
+                o.values()
+                om.append(o)
+            g = w % ntraj
+            s2 = pyoracle.System(o2, om, P[2 * g:2 * g + 2], Q[2 * g:2 * g + 2])
+            s2.set_cutoffs(10.0, 100.0)
+            tr = pyoracle.BDTrajectory([(m["cen"] - m["rcen"], m["rad"]) for m in mols], int(seeds[g]), False,
+                                       P[2 * g:2 * g + 2], Q[2 * g:2 * g + 2], prm=oprm)
+            tr.attach(s2)
+            barrier.wait()
+            if w == 0:
+                t_start.append(time.perf_counter())
+            while not t_start:
+                time.sleep(0.001)
+            while time.perf_counter() - t_start[0] < cpu_budget:
+                if tr.step() != 2:
+                    break
+                counts[w] += 1
+
+        threads = [threading.Thread(target=work, args=(w,)) for w in range(workers)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        tc = time.perf_counter() - t_start[0]
+        n = sum(counts)
+        out["cpu_baseline"] = dict(value=n / tc, unit="trajectory-steps/s", cores=workers, kind="port",
+                                   sample="trajectories 0..%d of the same batch, one per host thread, %d steps in %.1f s "
+                                          "on the oracle" % (workers - 1, n, tc))
         out["max_abs_pos_diff_traj0_device_vs_oracle_A"] = err
     return out
+
+
+def cpu_info():
+    """host cores this process may use and the CPU model"""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(nproc=os.cpu_count(), usable=usable, model=model)
+
+
+def host_workers():
+    """worker threads for the parallel CPU legs: the usable cores, at most 16 (the GPU box's
+    CPU share for one GPU)"""
+    return max(1, min(16, cpu_info()["usable"]))
+
+
+def _one_omp_thread():
+    """the oracle's OpenMP regions run one thread inside each worker thread"""
+    import ctypes
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(1)
+    except OSError:
+        pass
+
+
+def cpu_batch_baseline(kappa, mols, mts, P, Q, budget_s, workers):
+    """The oracle on independent systems of the batched workload, one system per worker
+    thread at a time (the oracle releases the GIL inside its calls; one OpenMP thread per
+    worker): complete force+torque evaluations finished within about `budget_s` seconds
+    (at least one round of `workers` systems)."""
+    import threading
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from pbsam_amd import synth
+    FH = [g.FH() for g in mts]
+    nsys = len(P) // 2
+    done, lock = [0], threading.Lock()
+    t_start = [None]
+    barrier = threading.Barrier(workers)
+
+    def work(w):
+        _one_omp_thread()
+        orc = pyoracle.Oracle(kappa)
+        omts = []
+        for m, (F, H) in zip(mols, FH):
+            o = pyoracle.MolType(orc, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], mts[0].p, eps_in=synth.EPS_IN,
+                                 rcen=m["rcen"], npts=[NPTS] * len(m["rad"]))
+            o.set_self(F, H)
+            o.values()
+            omts.append(o)
+        barrier.wait()
+        with lock:
+            if t_start[0] is None:
+                t_start[0] = time.perf_counter()
+        b = w
+        while True:
+            S = pyoracle.System(orc, omts, P[2 * (b % nsys):2 * (b % nsys) + 2], Q[2 * (b % nsys):2 * (b % nsys) + 2])
+            S.set_cutoffs(100.0, 100.0)
+            S.compute_forces(0, 0.0, 0)
+            with lock:
+                done[0] += 1
+            b += workers
+            if time.perf_counter() - t_start[0] >= budget_s:
+                break
+
+    threads = [threading.Thread(target=work, args=(w,)) for w in range(workers)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    el = time.perf_counter() - t_start[0]
+    return dict(value=done[0] / el, unit="force+torque evals/s", cores=workers, kind="port",
+                sample="%d complete force+torque evaluations of the batch's snapshots on %d host threads in %.1f s "
+                       "(one system per thread at a time)" % (done[0], workers, el))
 
 
 def cpu_baseline(kappa, mols, mts, pos, quat, gpu_force, gpu_torque, budget_s):
@@ -304,6 +489,8 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=2, help="timed batched calls")
     ap.add_argument("--plain-sample", type=int, default=8, help="systems timed one by one for the ratio")
     ap.add_argument("--solves", type=int, default=20, help="mutual solves timed for solves_per_s")
+    ap.add_argument("--mixed", type=int, default=256, help="systems in the mixed-order batch leg (0: skip it)")
+    ap.add_argument("--tight-solves", type=int, default=1, help="mutual solves timed at the tight tolerance")
     ap.add_argument("--bd", type=int, default=1024, help="trajectories in the Brownian-dynamics leg (0: skip it)")
     ap.add_argument("--bd-steps", type=int, default=3, help="timed steps of the Brownian-dynamics leg")
     ap.add_argument("--pot-grid", type=int, default=0, metavar="N",
@@ -373,6 +560,17 @@ def main():
             system.solve(tol=0, max_rounds=0)
         ctx.sync()
         solves_per_s = nsol / (time.perf_counter() - t0)
+    # the solve converged to a tighter fixed point: mean squared relative change of the
+    # coefficients <= 1e-8 (relative change ~1e-4 per round against the stock 1e-2), the
+    # reference's order; the Gauss-Seidel schedule converges slowly, so a few hundred rounds
+    tight = None
+    if args.tight_solves > 0:
+        t0 = time.perf_counter()
+        for _ in range(args.tight_solves):
+            trounds, trc = system.solve(tol=1e-8, max_rounds=1000)
+        ctx.sync()
+        tight = dict(solves_per_s=args.tight_solves / (time.perf_counter() - t0), tol=1e-8, mutual_rounds=trounds,
+                     status=trc)
 
     dom = max(prof, key=lambda k: prof[k]["ms"])
     d = prof[dom]
@@ -391,6 +589,15 @@ def main():
                 algorithmic_per_launch=(d["flops"] if compute_bound else d["bytes"]) / max(d["launches"], 1),
                 traffic=TRAFFIC.get(dom), traffic_unit="HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                                         "copy-calibrated)")
+    # the kernel group that streams HBM: the stored-map products of the gradient phase
+    h = prof["p2l"]
+    roof_hbm = None
+    if h["launches"] and h["ms"] > 0:
+        bw = h["bytes"] / (h["ms"] * 1e-3)
+        roof_hbm = dict(bound="hbm", kernel="p2l (stored-map GEMV k_p2lgemv of the gradient phase, with the solve's "
+                                          "few k_p2l launches)", achieved=bw / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
+                        frac=bw / PEAK_HBM, launches=h["launches"], avg_launch_us=1e3 * h["ms"] / h["launches"],
+                        algorithmic_per_launch=h["bytes"] / h["launches"], traffic=TRAFFIC.get("p2l"))
     kernels = {k: dict(launches=v["launches"], ms_per_step=v["ms"] / args.steps,
                        gflop_per_step=v["flops"] / args.steps / 1e9) for k, v in prof.items()}
 
@@ -400,6 +607,12 @@ def main():
                           min(args.plain_sample, args.batch))
         print("batch of %d systems: %.1f evals/s (plain loop %.1f evals/s)" %
               (args.batch, batch["evals_per_s"], batch["plain_loop_evals_per_s"]), file=sys.stderr, flush=True)
+
+    mixed = None
+    if args.mixed > 0:
+        mixed = mixed_leg(pbsam_amd, synth, ctx, args.mixed, args.batch_reps, min(args.plain_sample, args.mixed))
+        print("mixed-order batch of %d systems: %.1f evals/s (plain loop %.1f evals/s)" %
+              (args.mixed, mixed["evals_per_s"], mixed["plain_loop_evals_per_s"]), file=sys.stderr, flush=True)
 
     potential = None
     if args.pot_grid > 0:
@@ -425,6 +638,14 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(kappa, mols, mts, pos, quat, force, torque, args.cpu_budget)
+        cpu["host"] = cpu_info()
+        if batch is not None:
+            P_, Q_ = batch["snapshots"]
+            batch["cpu_baseline"] = cpu_batch_baseline(kappa, mols, mts, P_, Q_, args.cpu_budget, host_workers())
+            print("CPU oracle on %d threads: %.2f evals/s" % (batch["cpu_baseline"]["cores"],
+                                                              batch["cpu_baseline"]["value"]), file=sys.stderr, flush=True)
+    if batch is not None:
+        batch.pop("snapshots", None)
 
     line = {
         "metric": METRIC,
@@ -447,10 +668,13 @@ def main():
             "separation_A": DIST, "inter_rcutoff_A": 100.0, "mutual_rounds": rounds,
         },
         "solves_per_s": solves_per_s,
+        "tight_solve": tight,
         "batch": batch,
+        "mixed_batch": mixed,
         "bd": bd,
         "potential": potential,
         "roofline": roof,
+        "roofline_hbm": roof_hbm,
         "cpu_baseline": cpu,
         "kernels": kernels,
     }

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -183,6 +184,19 @@ struct KProf {
   double ms = 0.0;
 };
 
+// Launch-shape settings that do not change what is computed (only how it is split over
+// workgroups and launches, or summation order at the rounding level); pbsam_autotune picks
+// them per system shape by timing, the defaults are those measured on the benchmark's
+// two-molecule p = 30 evaluation.
+struct Tune {
+  int xsplit_max = 256;   // multi-plane re-expansion launches below this many jobs run one
+                          // plane per workgroup (more parallelism, no shared rotation)
+  int xbin_min = 512;     // re-expansion launches from this many jobs are binned by order class
+  int mgemv_wg = 4096;    // target workgroups of a stored-map GEMV launch (sets its point split)
+  int pgemm_min = 4;      // jobs sharing a stored map from which it runs as a matrix-core product
+  int grad_window = 64;   // gradient sweep steps enqueued between read-backs at most
+};
+
 struct pbsam_ctx {
   double kappa, sdiel;
   int device;
@@ -200,6 +214,8 @@ struct pbsam_ctx {
   // a second stream (default priority) for independent launch chains, joined by events
   hipStream_t stream0 = nullptr, stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  Tune tune;                                          // settings of the system being run
+  std::map<std::array<long long, 3>, Tune> tuned;     // pbsam_autotune results by (p, spheres, systems)
 };
 
 // Launch helpers use c->stream; StreamSwap points it at another stream for a scope.
@@ -386,10 +402,9 @@ static void launch_xform_qp(pbsam_ctx *c, int n, const XArgs &a, int qmax) {
 // Multi-plane jobs share one rotation generation per workgroup; small launches (fewer
 // workgroups than the chip has CUs) instead run one plane per workgroup, trading that
 // sharing for parallelism.
-static constexpr int XSPLIT_MAX = 256;
 static void launch_xform_q(pbsam_ctx *c, int n, XArgs a, int qmax) {
   a.work = prof_begin(c, PBSAM_K_XFORM);
-  if (a.planes == 3 && n < XSPLIT_MAX && !a.cls) {
+  if (a.planes == 3 && n < c->tune.xsplit_max && !a.cls) {
     a.split = 1;
     launch_xform_qp<1>(c, n, a, qmax);
   } else if (a.planes == 3) {
@@ -409,10 +424,9 @@ static void launch_xform_q(pbsam_ctx *c, int n, XArgs a, int qmax) {
 // little LDS; each class launch covers the whole batch and its workgroups past the class's
 // device-side count leave at once, so nothing is read back.
 // Results do not depend on the split.
-static constexpr int XBIN_MIN = 512;
 static void launch_xform(pbsam_ctx *c, int njobs, XArgs a) {
   if (!njobs) return;
-  if (a.ordH || a.full || njobs < XBIN_MIN) {
+  if (a.ordH || a.full || njobs < c->tune.xbin_min) {
     launch_xform_q(c, njobs, a, a.p);
     return;
   }
@@ -1379,6 +1393,8 @@ struct pbsam_system {
   std::vector<int> grp_passes;  // gradient passes of each group's last gradient phase
   std::vector<char> grp_skip;   // groups left out of the next force evaluation (none when empty)
   std::shared_ptr<BdState> bd;  // trajectories, when the batch runs Brownian dynamics
+  bool has_tune = false;  // launch-shape settings chosen for this system (pbsam_autotune)
+  Tune tune;
   // a batch of systems with different pole orders: one part per order (pbsam_host_mixed.inc)
   std::vector<std::unique_ptr<pbsam_system>> parts;
   std::shared_ptr<struct MixedMap> mixed;
@@ -1528,9 +1544,22 @@ static int group_pairs(pbsam_system *S, int m0, int m1) {
 }
 
 // generateInterXFormsForPolarize_LowMemory (molecule.cpp:1036-1126) and the job lists
+// launch-shape settings of a system: its own, else those tuned for its shape, else defaults
+static std::array<long long, 3> tune_key(const pbsam_system *S) { return {S->p, S->nsph, S->ngrp}; }
+static void tune_select(pbsam_system *S) {
+  pbsam_ctx *c = S->c;
+  if (S->has_tune) {
+    c->tune = S->tune;
+    return;
+  }
+  auto it = c->tuned.find(tune_key(S));
+  c->tune = (it != c->tuned.end()) ? it->second : Tune{};
+}
+
 static int system_lists(pbsam_system *S) {
   pbsam_ctx *c = S->c;
   hipStream_t s = c->stream;
+  tune_select(S);
   const int nmol = S->nmol, p = S->p, P2 = p * p;
   system_reset_state(S);
   S->solved = false;
@@ -2005,7 +2034,6 @@ static void system_phase0(pbsam_system *S) {
 }
 
 #include "pbsam_host_grad.inc"
-#include "pbsam_host_mixed.inc"
 
 // ----------------------------------------------------------------------------
 // C ABI
@@ -2026,6 +2054,8 @@ static int guarded(std::string *err, F f) {
   g_err = old;
   return rc;
 }
+
+#include "pbsam_host_mixed.inc"
 
 extern "C" {
 
@@ -2450,6 +2480,10 @@ int pbsam_solve_mutual(pbsam_system *S, double tol, int max_rounds, unsigned fla
     int rc = system_lists(S);
     if (rc != PBSAM_OK) return rc;
     if (S->pol.empty()) {
+      // no polarized pair: nothing to iterate, but spheres with interaction partners still
+      // carry their rotated self values into the gradient phase (phase 0 of polarize_mutual,
+      // molecule.cpp:2002-2015, runs whatever the lists hold)
+      system_phase0(S);
       S->solved = true;
       if (rounds_out) *rounds_out = 0;
       return groups_rc(S);

@@ -39,7 +39,7 @@ SYMBOLS = [
     "pbsam_bd_defaults", "pbsam_bd_start", "pbsam_bd_contacts", "pbsam_bd_run", "pbsam_bd_get",
     "pbsam_read_pqr", "pbsam_write_imat", "pbsam_read_imat", "pbsam_write_expansion", "pbsam_read_expansion",
     "pbsam_read_config", "pbsam_write_config", "pbsam_read_imats", "pbsam_read_expansions",
-    "pbsam_potential", "pbsam_total_int_energy", "pbsam_system_set_flags",
+    "pbsam_potential", "pbsam_total_int_energy", "pbsam_system_set_flags", "pbsam_autotune",
 ]
 POT_REFERENCE_FRAME = 1
 BD_ESCAPED, BD_DOCKED, BD_RUNNING, BD_STUCK, BD_FAILED = range(5)
@@ -122,6 +122,7 @@ def load():
         "pbsam_system_set_pose": ([P, I, P, P], I),
         "pbsam_system_set_cutoffs": ([P, D, D], I),
         "pbsam_system_set_flags": ([P, ctypes.c_uint], I),
+        "pbsam_autotune": ([P, I, P, P], I),
         "pbsam_solve_mutual": ([P, D, I, ctypes.c_uint, P], I),
         "pbsam_mutual_rounds": ([P, I, ctypes.c_uint, P], I),
         "pbsam_system_get": ([P, I, I, P, P, P, P], I),
@@ -437,6 +438,13 @@ class System:
         """pbsam_system_set_flags: EXACT_DERIVATIVE (or its parts) for forces that are the
         exact derivative of the energy; 0 for the reference's behaviour."""
         self.ctx.check(self.ctx.L.pbsam_system_set_flags(self.h, int(flags)))
+
+    def autotune(self, reps=1):
+        """pbsam_autotune: (best ms per evaluation, dict of the chosen settings)"""
+        ms, st = ctypes.c_double(), np.zeros(5, dtype=np.int32)
+        self.ctx.check(self.ctx.L.pbsam_autotune(self.h, reps, ctypes.byref(ms), st.ctypes.data_as(P)))
+        keys = ("grad_window", "xsplit_max", "xbin_min", "mgemv_wg", "pgemm_min")
+        return ms.value, dict(zip(keys, (int(v) for v in st)))
 
     def set_pose(self, i, pos, quat):
         pos, quat = f64(pos), f64(quat)

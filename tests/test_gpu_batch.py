@@ -204,3 +204,31 @@ def test_batch_of_mixed_pole_orders(gpu, orc):
             # per-sphere blocks come back at the molecule's own order
             assert len(batch.get(2 * gi, 0)["H"]) == p * p
     assert checked == {10, 30}
+
+
+def test_autotune_keeps_results(gpu, orc):
+    """pbsam_autotune times complete evaluations over launch-shape settings and keeps the
+    fastest; the forces stay those of the default settings to rounding, and a second batch
+    of the same shape picks the tuned settings up."""
+    import pbsam_amd
+    mols, gs, _ = moltypes(gpu, orc, 5, 10, 400, seeds=(4, 5))
+    from pbsam_amd import synth
+    poses = [synth.place_pair(mols[0], mols[1], 2 * synth.ball_radius(5) + 4.0 + b, seed=90 + b) for b in range(6)]
+    pos = np.vstack([pp for pp, _ in poses])
+    quat = np.vstack([qq for _, qq in poses])
+
+    def batch():
+        b = pbsam_amd.System(gpu, [gs[0], gs[1]] * 6, pos, quat, groups=[2] * 6)
+        b.set_cutoffs(100.0, 100.0)
+        return b
+
+    b0 = batch()
+    _, rc, f0, t0 = b0.compute_forces()
+    assert rc == 0
+    b1 = batch()
+    ms, settings = b1.autotune(reps=1)
+    assert ms > 0 and set(settings) == {"grad_window", "xsplit_max", "xbin_min", "mgemv_wg", "pgemm_min"}
+    _, rc, f1, t1 = b1.compute_forces()
+    assert rc == 0 and relerr(f1, f0) < 1e-12 and relerr(t1, t0) < 1e-12
+    _, rc, f2, _ = batch().compute_forces()   # same shape: the tuned settings
+    assert rc == 0 and relerr(f2, f1) < 1e-15

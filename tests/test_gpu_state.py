@@ -85,3 +85,40 @@ def test_batch_groups_with_uneven_lists(gpu, orc):
         assert rc1 == 0 and (br[b], bp[b]) == (sr[0], sp[0]), b
         assert relerr(fb[2 * b:2 * b + 2], f1) < 1e-10, (b, fb[2 * b:2 * b + 2], f1)
         assert relerr(tb[2 * b:2 * b + 2], t1) < 1e-10, b
+
+
+def test_forces_with_interaction_partners_only(gpu, orc):
+    """Molecules closer than the interaction cutoff but farther than the polarization
+    cutoff (the Brownian-dynamics cutoffs, 10 / 100 A): no pair is polarized, the spheres
+    with interaction partners use their rotated self values, and the forces are not zero.
+    Single systems and a batch of them against the oracle."""
+    import pbsam_amd
+    import pyoracle
+    from conftest import relerr
+    from pbsam_amd import synth
+    mols, gs = moltypes(gpu, orc, [6, 5], 8, 300, seed0=40)
+    os_ = []
+    for m, g in zip(mols, gs):
+        o = pyoracle.MolType(orc, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], 8, eps_in=4.0, rcen=m["rcen"],
+                             npts=[300] * len(m["rad"]))
+        o.set_self(*g.FH())
+        o.values()
+        os_.append(o)
+    d = synth.ball_radius(6) + synth.ball_radius(5) + 14.0
+    poses = [synth.place_pair(mols[0], mols[1], d + b, seed=70 + b) for b in range(3)]
+    batch = pbsam_amd.System(gpu, gs * 3, np.vstack([p for p, _ in poses]), np.vstack([q for _, q in poses]),
+                             groups=[2] * 3)
+    batch.set_cutoffs(10.0, 100.0)
+    _, rcb, fb, tb = batch.compute_forces()
+    assert rcb == 0
+    for b, (pos, quat) in enumerate(poses):
+        S = pbsam_amd.System(gpu, gs, pos, quat)
+        S.set_cutoffs(10.0, 100.0)
+        _, rc, f, t = S.compute_forces()
+        assert rc == 0 and S.counts()[1] == 0
+        So = pyoracle.System(orc, os_, pos, quat)
+        So.set_cutoffs(10.0, 100.0)
+        rco, fo, to = So.compute_forces(0, 0.0, 0)
+        assert rco == 0 and np.abs(fo).max() > 0
+        assert relerr(f, fo) < 1e-8 and relerr(t, to) < 1e-8, (b, f, fo)
+        assert relerr(fb[2 * b:2 * b + 2], fo) < 1e-8, b
