@@ -320,6 +320,23 @@ int pbsam_imat_build(pbsam_ctx *ctx, int p, int npe, const double *spe, int npb,
  * in/out: n*p*p blocks. */
 int pbsam_xform_h_batch(pbsam_ctx *ctx, int n, int p, const double *P, const double *a, const int *order,
                         const double *in, double *out);
+/* Allocation-free variant of pbsam_xform_h_batch; it has no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  P, a, order, in and out are DEVICE pointers
+ * with the meanings of pbsam_xform_h_batch (the caller owns them and orders its writes to
+ * them before the call); the results equal pbsam_xform_h_batch's.  The job tables and
+ * translation matrices live in a fixed per-device scratch array of
+ * PBSAM_NOALLOC_SCRATCH_BYTES, held until the process ends: a batch needing more
+ * (pbsam_xform_h_batch_noalloc_bytes; 64 MiB hold about 880 pairs at p = 30) returns
+ * PBSAM_ERR_BADARG before any launch, as does an order outside [1, p] (checked on the
+ * device; the call then waits for the stream).  Calls never allocate and return after the
+ * context's stream has finished them.  Calls on one device share the scratch: calls that
+ * overlap (other threads, other contexts on the same device) race. */
+#define PBSAM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int pbsam_xform_h_batch_noalloc(pbsam_ctx *ctx, int n, int p, const double *P, const double *a, const int *order,
+                                const double *in, double *out);
+/* scratch bytes a call of n pairs at order p needs (0 for invalid sizes) */
+size_t pbsam_xform_h_batch_noalloc_bytes(int n, int p);
 /* CXFormAIntra::reset + xformF/xformH (src/xforms.cpp:214-270) */
 int pbsam_xform_intra_batch(pbsam_ctx *ctx, int n, int p, const double *P, const double *a,
                             const int *orders /* 2n: pH, pF */, const double *H, const double *F,

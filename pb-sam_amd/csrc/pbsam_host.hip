@@ -2906,3 +2906,4 @@ int pbsam_rotate_quat_batch(pbsam_ctx *c, int n, int p, const double *quat, int 
 #include "pbsam_host_bd.inc"
 #include "pbsam_host_energy.inc"
 #include "pbsam_host_io.inc"
+#include "pbsam_host_noalloc.inc"

@@ -40,6 +40,7 @@ SYMBOLS = [
     "pbsam_read_pqr", "pbsam_write_imat", "pbsam_read_imat", "pbsam_write_expansion", "pbsam_read_expansion",
     "pbsam_read_config", "pbsam_write_config", "pbsam_read_imats", "pbsam_read_expansions",
     "pbsam_potential", "pbsam_total_int_energy", "pbsam_system_set_flags", "pbsam_autotune",
+    "pbsam_xform_h_batch_noalloc", "pbsam_xform_h_batch_noalloc_bytes",
 ]
 POT_REFERENCE_FRAME = 1
 BD_ESCAPED, BD_DOCKED, BD_RUNNING, BD_STUCK, BD_FAILED = range(5)
