@@ -192,6 +192,17 @@ int pbsam_forces(pbsam_system *sys, double *force, double *torque);
  * computed, as the reference does). */
 int pbsam_compute_forces(pbsam_system *sys, double tol, int max_rounds, double *force, double *torque,
                          int *rounds_out);
+/* Torques that are the exact derivative of the energy (no counterpart in the reference, whose
+ * torque is sum c x f with the surface term disabled, src/moldynamics.cpp:540-541): after a
+ * mutual solve with PBSAM_EXACT_DERIVATIVE set, the gradient sweeps of polarize_mutual run
+ * once more for a rotation of each molecule about its centre (sources: the sphere
+ * displacements omega x (c - r) and the turn of the rotated expansions), until the mean
+ * change <= tol or max_rounds passes, and torque (3 per molecule, lab frame) receives
+ * -dE_i/domega of each molecule's energy sum_k <H_k, LS_k>.  The gradient blocks then hold
+ * the rotational derivative (pbsam_forces returns PBSAM_ERR_STATE until pbsam_gradient runs
+ * again).  PBSAM_ERR_STATE without the flags or the solve; PBSAM_ERR_UNSUPPORTED when a
+ * sphere has interaction partners but no polarization partner. */
+int pbsam_torques_exact(pbsam_system *sys, double tol, int max_rounds, double *torque);
 /* gradient blocks of sphere k of molecule mol: gH (3 p^2, polarized spheres) and gLHN
  * (3 p^2), as left by the last pbsam_gradient */
 int pbsam_system_get_grad(const pbsam_system *sys, int mol, int k, double *gH, double *gLHN);

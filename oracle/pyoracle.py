@@ -94,6 +94,12 @@ def lib():
         L.orc_set_grad_skip.restype = None
         L.orc_set_ext_sign.argtypes = [D]
         L.orc_set_ext_sign.restype = None
+        L.orc_rot_generator.argtypes = [I, P, P, I]
+        L.orc_rot_generator.restype = None
+        L.orc_gradient_rot.argtypes = [P, D, I]
+        L.orc_gradient_rot.restype = I
+        L.orc_grad_energy.argtypes = [P, I, P]
+        L.orc_grad_energy.restype = None
         L.orc_moltype_create.restype = P
         L.orc_moltype_create.argtypes = [P, ctypes.POINTER(MolDesc)]
         L.orc_system_create.restype = P
@@ -399,6 +405,16 @@ class System:
 
     def gradient(self, schedule=1, tol=1e-24, max_rounds=400):
         return self.orc.L.orc_gradient(self.h, schedule, tol, max_rounds)
+
+    def gradient_rot(self, tol=1e-24, max_rounds=400):
+        """the gradient sweeps for a rotation of each molecule about its centre"""
+        return self.orc.L.orc_gradient_rot(self.h, tol, max_rounds)
+
+    def grad_energy(self, i):
+        """-sum <H, gLHN> + <gH, LS> of molecule i per plane of the last gradient sweeps"""
+        out = np.zeros(3)
+        self.orc.L.orc_grad_energy(self.h, i, _p(out))
+        return out
 
     def forces(self):
         f, t = np.zeros(3 * self.n), np.zeros(3 * self.n)

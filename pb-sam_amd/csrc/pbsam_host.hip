@@ -1323,6 +1323,9 @@ struct GradState {
   DBuf<int> ia, ib, ic, tqmap;             // scratch lists of the self-value batches
   DBuf<int> dpairs;                        // (source, target) sphere pairs of a re-expansion batch
   DBuf<int> ext;                           // entries whose geometric derivative changes sign
+  DBuf<double> rotu, slot2, rgenH, rgenL;  // rotational sources: arms, generator re-expansions,
+  DBuf<int> rotg, rotsel;                  // generator planes of Hrot and LSrot [sphere][3][p^2]
+  DBuf<XJob> rotxj;                        // re-expansion jobs of the generator planes
   DBuf<double> lab, rad;                   // spheres' lab centres and radii
   DBuf<double> pts, mol7, cen3;            // potential queries: points, poses, sphere centres
   DBuf<int> is_off, is_lst, ps_off, ps_lst, ident, esph, eitem, sbuf;
@@ -1383,6 +1386,7 @@ struct pbsam_system {
   double itot = 0.0;
   GradState gr;
   bool solved = false, grad_ok = false;
+  bool grad_rot = false;  // the gradient blocks hold the rotational derivative (pbsam_torques_exact)
   // independent systems solved together: groups of consecutive molecules that interact
   // only among themselves (one group unless made by pbsam_batch_create)
   int ngrp = 1;
@@ -1564,6 +1568,7 @@ static int system_lists(pbsam_system *S) {
   system_reset_state(S);
   S->solved = false;
   S->grad_ok = false;
+  S->grad_rot = false;
   for (MolInst &M : S->mols) {
     M.bInterXForm = false;
     M.interPol.assign(M.mt->ns, {});
@@ -2404,6 +2409,7 @@ int pbsam_system_set_pose(pbsam_system *S, int i, const double *pos, const doubl
   S->lists_ok = false;
   S->solved = false;   // the solution and gradients belong to the old pose
   S->grad_ok = false;
+  S->grad_rot = false;
   return PBSAM_OK;
 }
 
@@ -2415,6 +2421,7 @@ int pbsam_system_set_cutoffs(pbsam_system *S, double inter_rc, double interact_r
   S->lists_ok = false;
   S->solved = false;   // the pair lists behind the solution change with the cutoffs
   S->grad_ok = false;
+  S->grad_rot = false;
   return PBSAM_OK;
 }
 
@@ -2425,6 +2432,7 @@ int pbsam_system_set_flags(pbsam_system *S, unsigned flags) {
   S->lists_ok = false;
   S->solved = false;   // the solution was made under the old settings
   S->grad_ok = false;
+  S->grad_rot = false;
   return PBSAM_OK;
 }
 
@@ -2579,6 +2587,7 @@ int pbsam_gradient(pbsam_system *S, double tol, int max_rounds, int *passes_out)
     HIPCHK(hipSetDevice(S->c->device));
     int ct = system_gradient(S, tol, max_rounds);
     S->grad_ok = true;
+    S->grad_rot = false;
     if (passes_out) *passes_out = ct;
     return PBSAM_OK;
   });
@@ -2607,6 +2616,63 @@ int pbsam_forces(pbsam_system *S, double *force, double *torque) {
   return guarded(&S->c->err, [&]() {
     HIPCHK(hipSetDevice(S->c->device));
     system_forces(S, force, torque);
+    return PBSAM_OK;
+  });
+}
+
+int pbsam_torques_exact(pbsam_system *S, double tol, int max_rounds, double *torque) {
+  if (!S || !torque) return PBSAM_ERR_BADARG;
+  if (S->mixed) {
+    for (size_t pi = 0; pi < S->parts.size(); pi++) {
+      pbsam_system *P = S->parts[pi].get();
+      std::vector<double> t(3 * (size_t)P->nmol);
+      const int rc = pbsam_torques_exact(P, tol, max_rounds, t.data());
+      if (rc != PBSAM_OK) return rc;
+      for (int l = 0; l < P->nmol; l++)
+        for (int x = 0; x < 3; x++) torque[3 * S->mixed->mols[pi][l] + x] = t[3 * l + x];
+    }
+    return PBSAM_OK;
+  }
+  if (!S->solved || (S->flags & PBSAM_EXACT_DERIVATIVE) != PBSAM_EXACT_DERIVATIVE) {
+    S->c->err = "pbsam_torques_exact needs PBSAM_EXACT_DERIVATIVE and a mutual solve of the current poses";
+    return PBSAM_ERR_STATE;
+  }
+  for (const MolInst &M : S->mols)
+    for (int k = 0; k < M.mt->ns; k++)
+      if (M.interPol[k].empty() && !M.interact[k].empty()) {
+        S->c->err = "pbsam_torques_exact: spheres with interaction partners but no polarization partner (set the "
+                    "interaction cutoff to the polarization cutoff)";
+        return PBSAM_ERR_UNSUPPORTED;
+      }
+  return guarded(&S->c->err, [&]() {
+    HIPCHK(hipSetDevice(S->c->device));
+    system_gradient(S, tol, max_rounds, true);
+    S->grad_ok = false;  // the gradient blocks now hold the rotational derivative
+    S->grad_rot = true;
+    hipStream_t st = S->c->stream;
+    std::vector<double> H = S->d_H.download(st), LS = S->d_LS.download(st), gH = S->gr.gH.download(st),
+                        gL = S->gr.gLHN.download(st);
+    const int p = S->p, P2 = p * p;
+    auto ip = [&](const double *a, const double *b) {  // inprod (expansion.h:706-741)
+      double sum = 0.0;
+      for (int n = 0; n < p; n++) {
+        double t = 0.0;
+        for (int mm = 1; mm < 2 * n + 1; mm++) t += a[n * n + mm] * b[n * n + mm];
+        sum += 2.0 * t + a[n * n] * b[n * n];
+      }
+      return sum;
+    };
+    for (int i = 0; i < S->nmol; i++) {
+      const MolInst &M = S->mols[i];
+      double t[3] = {0, 0, 0};
+      for (int k = 0; k < M.mt->ns; k++) {
+        if (M.interPol[k].empty()) continue;
+        const size_t g = (size_t)M.base + k;
+        for (int a = 0; a < 3; a++)
+          t[a] -= ip(&H[g * P2], &gL[(g * 3 + a) * P2]) + ip(&gH[(g * 3 + a) * P2], &LS[g * P2]);
+      }
+      for (int a = 0; a < 3; a++) torque[3 * i + a] = t[a];
+    }
     return PBSAM_OK;
   });
 }
@@ -2654,7 +2720,7 @@ int pbsam_system_get_grad(const pbsam_system *S, int i, int k, double *gH, doubl
   if (S && S->mixed && i >= 0 && i < S->nmol)
     return pbsam_system_get_grad(S->parts[S->mixed->part_of_mol[i]].get(), S->mixed->local_mol[i], k, gH, gLHN);
   if (!S || i < 0 || i >= S->nmol || k < 0 || k >= S->mols[i].mt->ns) return PBSAM_ERR_BADARG;
-  if (!S->grad_ok) return PBSAM_ERR_STATE;
+  if (!S->grad_ok && !S->grad_rot) return PBSAM_ERR_STATE;
   return guarded(&S->c->err, [&]() {
     hipStream_t s = S->c->stream;
     const size_t B3 = 3 * (size_t)S->p * S->p, g = (size_t)S->mols[i].base + k;

@@ -1591,10 +1591,7 @@ __global__ void __launch_bounds__(256) k_surfgemm_k4(SGArgs A) {
   }
 }
 
-}  // namespace pbsam
 
-#include "pbsam_kernels_imat.hip"
-#include "pbsam_kernels_pot.hip"
 
 // blocks listed in blk (B doubles each) multiplied by -1: one workgroup per (block, chunk)
 __global__ void __launch_bounds__(256) k_negblk(const int *blk, int B, double *out) {
@@ -1604,3 +1601,85 @@ __global__ void __launch_bounds__(256) k_negblk(const int *blk, int B, double *o
     *o = -*o;
   }
 }
+
+// ---- rotational derivative of the gradient sweeps (no counterpart in the reference) ----
+// Generator of a lab-frame rotation about axis a acting on a block (d/de of the rotation by
+// the quaternion (cos e/2, sin e/2 e_a) * q of the rotated block): on the complex
+// coefficients c(n,m) (c(n,-m) = conj c(n,m)) it is i conj(L_a), L_+- with the factors
+// s_m/s_{m+-1} sqrt((n -+ m)(n +- m + 1)) (s_m = (-1)^m for m >= 0, 1 below; the harmonics
+// carry no Condon-Shortley sign) and L_z = m.
+__device__ __forceinline__ cd coef_c(const double *B, int n, int m) {
+  if (m < -n || m > n) return {0.0, 0.0};
+  const int am = m < 0 ? -m : m;
+  const cd v = am == 0 ? cd{B[n * n], 0.0} : cd{B[n * n + 2 * am - 1], B[n * n + 2 * am]};
+  return m < 0 ? cconj(v) : v;
+}
+__device__ __forceinline__ double sgm(int m) { return (m >= 0 && (m & 1)) ? -1.0 : 1.0; }
+// out[b][a][p^2] = G_a in[blk ? blk[b] : b]; grid (blocks, chunks of 256 coefficients)
+__global__ void __launch_bounds__(256) k_rotgen(int p, const int *blk, const double *in, double *out) {
+  const int P2 = p * p, i = blockIdx.y * 256 + threadIdx.x;
+  if (i >= P2) return;
+  const int b = blockIdx.x;
+  const double *B = in + (size_t)(blk ? blk[b] : b) * P2;
+  int n = (int)sqrt((double)i);
+  while (n * n > i) n--;
+  while ((n + 1) * (n + 1) <= i) n++;
+  const int r = i - n * n, m = (r + 1) / 2;  // r = 0: m = 0 (real); odd: Re c(n,m); even: Im c(n,m)
+  const bool im = r > 0 && (r % 2 == 0);
+  const cd cl = coef_c(B, n, m - 1), cr = coef_c(B, n, m + 1), c0 = coef_c(B, n, m);
+  const double fp = (m - 1 >= -n) ? sgm(m - 1) / sgm(m) * sqrt((double)((n - m + 1) * (n + m))) : 0.0;
+  const double fm = (m + 1 <= n) ? sgm(m + 1) / sgm(m) * sqrt((double)((n + m + 1) * (n - m))) : 0.0;
+  const cd sx = {fp * cl.x + fm * cr.x, fp * cl.y + fm * cr.y};  // L_+ from m-1, L_- from m+1
+  const cd wx = {-0.5 * sx.y, 0.5 * sx.x};                         // i/2 (...)
+  const cd wy = {-0.5 * fp * cl.x + 0.5 * fm * cr.x, -0.5 * fp * cl.y + 0.5 * fm * cr.y};
+  const cd wz = {-m * c0.y, m * c0.x};                             // i m c
+  double *o = out + (size_t)b * 3 * P2 + i;
+  o[0] = im ? wx.y : wx.x;
+  o[P2] = im ? wy.y : wy.x;
+  o[2 * P2] = im ? wz.y : wz.x;
+}
+// per pair k: the three gradient planes G of its slot become u_k x G (the derivative along
+// the displacement omega_a x u_k), plus, where gen[k] >= 0, the three planes of slot2[k]
+__global__ void __launch_bounds__(256) k_rot_contract(int p, const double *u, const int *gen, const double *slot2,
+                                                      double *slot) {
+  const int P2 = p * p, i = blockIdx.y * 256 + threadIdx.x, k = blockIdx.x;
+  if (i >= P2) return;
+  double *G = slot + (size_t)k * 3 * P2 + i;
+  const double g0 = G[0], g1 = G[P2], g2 = G[2 * P2];
+  const double ux = u[3 * k], uy = u[3 * k + 1], uz = u[3 * k + 2];
+  double r0 = uy * g2 - uz * g1, r1 = uz * g0 - ux * g2, r2 = ux * g1 - uy * g0;
+  if (gen[k] >= 0) {
+    const double *S2 = slot2 + (size_t)k * 3 * P2 + i;
+    r0 += S2[0];
+    r1 += S2[P2];
+    r2 += S2[2 * P2];
+  }
+  G[0] = r0;
+  G[P2] = r1;
+  G[2 * P2] = r2;
+}
+// re-expansion jobs of the generator planes: pair k's job with source block 3 gen[k] and
+// output planes at 3 k
+__global__ void __launch_bounds__(256) k_rot_genjobs(int n, const XJob *xj, const int *gen, XJob *xj2) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  XJob J = xj[k];
+  J.src = 3 * (gen[k] >= 0 ? gen[k] : 0);
+  J.slot = 3 * k;
+  xj2[k] = J;
+}
+// out[blk[e]][a] -= G_a LSrot[sph[e]] (the turn of a sphere's own back-rotation)
+__global__ void __launch_bounds__(256) k_rotgen_sub(int p, const int *blk, const int *sph, const double *gen3,
+                                                    double *out) {
+  const int P2 = p * p, i = blockIdx.y * 256 + threadIdx.x, e = blockIdx.x;
+  if (i >= P2) return;
+  const double *g = gen3 + (size_t)sph[e] * 3 * P2 + i;
+  double *o = out + (size_t)blk[e] * 3 * P2 + i;
+  o[0] -= g[0];
+  o[P2] -= g[P2];
+  o[2 * P2] -= g[2 * P2];
+}
+
+}  // namespace pbsam
+#include "pbsam_kernels_imat.hip"
+#include "pbsam_kernels_pot.hip"

@@ -40,6 +40,7 @@ SYMBOLS = [
     "pbsam_read_pqr", "pbsam_write_imat", "pbsam_read_imat", "pbsam_write_expansion", "pbsam_read_expansion",
     "pbsam_read_config", "pbsam_write_config", "pbsam_read_imats", "pbsam_read_expansions",
     "pbsam_potential", "pbsam_total_int_energy", "pbsam_system_set_flags", "pbsam_autotune",
+    "pbsam_torques_exact",
     "pbsam_xform_h_batch_noalloc", "pbsam_xform_h_batch_noalloc_bytes",
 ]
 POT_REFERENCE_FRAME = 1
@@ -124,6 +125,7 @@ def load():
         "pbsam_system_set_cutoffs": ([P, D, D], I),
         "pbsam_system_set_flags": ([P, ctypes.c_uint], I),
         "pbsam_autotune": ([P, I, P, P], I),
+        "pbsam_torques_exact": ([P, D, I, P], I),
         "pbsam_solve_mutual": ([P, D, I, ctypes.c_uint, P], I),
         "pbsam_mutual_rounds": ([P, I, ctypes.c_uint, P], I),
         "pbsam_system_get": ([P, I, I, P, P, P, P], I),
@@ -478,6 +480,12 @@ class System:
         r = ctypes.c_int()
         self.ctx.check(self.ctx.L.pbsam_gradient(self.h, tol, max_rounds, ctypes.byref(r)))
         return r.value
+
+    def torques_exact(self, tol=1e-16, max_rounds=2000):
+        """pbsam_torques_exact: -dE/domega of each molecule (lab frame), exact-derivative mode"""
+        t = np.zeros(3 * self.n)
+        self.ctx.check(self.ctx.L.pbsam_torques_exact(self.h, tol, max_rounds, _p(t)))
+        return t.reshape(-1, 3)
 
     def forces(self):
         f, t = np.zeros(3 * self.n), np.zeros(3 * self.n)

@@ -3,7 +3,7 @@
     python -m pbsam_amd.rocprof <run_kernel_trace.csv> [run_marker_api_trace.csv [bench.json]]
 
 Prints a Markdown table of per-kernel dispatch statistics over the whole trace and, when
-the marker trace holds bench.py's "timed_region" (and "batch_region", "bd_region",
+the marker trace holds bench.py's "timed_region" (and "batch_region", "mixed_region", "bd_region",
 "kernel_timing_region") range, over the dispatches that ran inside it; with bench.py's JSON line it also
 lists bench.py's own HIP-event averages of the instrumented kernels (timed in the instrumented repeat of
 the timed evaluations) beside rocprofv3's over the same dispatches.
@@ -70,7 +70,8 @@ def main(argv):
     table(rows, "All dispatches of the run")
     if len(argv) < 3:
         return
-    for label, what in (("batch_region", "bench.py's batched leg"), ("bd_region", "bench.py's Brownian-dynamics leg")):
+    for label, what in (("batch_region", "bench.py's batched leg"), ("mixed_region", "bench.py's mixed-order batch leg"),
+                        ("bd_region", "bench.py's Brownian-dynamics leg")):
         brng = find_range(argv[2], label)
         if brng is not None:
             inb = [r for r in rows if r[1] >= brng[0] and r[2] <= brng[1]]

@@ -200,3 +200,61 @@ def test_torque_off_centre_sphere(orc, exact, d, tol):
     tau = quat_rot(conj(q0), np.cross(offs[0], f0_lab))
     assert np.abs(t[0] - tau).max() < tol * np.linalg.norm(tau), (t[0], tau)
     assert np.abs(t[1]).max() < 1e-12 * np.linalg.norm(tau)
+
+
+def test_rotation_generator_is_the_derivative_of_the_rotation(orc):
+    """The generator used by the rotational sweeps (orc_rot_generator) against central
+    differences of the reference's rotation (rotateWithXi with (cos e/2, sin e/2 e_a) * q)."""
+    import ctypes
+    import pyoracle
+
+    def qmul(a, b):
+        w1, x1, y1, z1 = a
+        w2, x2, y2, z2 = b
+        return np.array([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                         w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2])
+    L = pyoracle.lib()
+    p = 8   # higher rows: the rotation's rounding (~1e-10 at row 11) over 2e-5 swamps the difference
+    rng = np.random.default_rng(3)
+    B = rng.normal(size=p * p)
+    q0 = rng.normal(size=4)
+    q0 /= np.linalg.norm(q0)
+    B0 = orc.rotate_quat(q0, p, 1, B)
+    for a in range(3):
+        e = np.zeros(3)
+        e[a] = 1.0
+        h = 1e-5
+        qp = qmul(np.r_[np.cos(h / 2), np.sin(h / 2) * e], q0)
+        qm = qmul(np.r_[np.cos(h / 2), -np.sin(h / 2) * e], q0)
+        fd = (orc.rotate_quat(qp, p, 1, B) - orc.rotate_quat(qm, p, 1, B)) / (2 * h)
+        out = np.zeros(p * p)
+        L.orc_rot_generator(a, B0.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p), p)
+        assert np.abs(out - fd).max() < 1e-7 * np.abs(fd).max(), a
+
+
+def test_exact_torque_is_the_energy_derivative(orc, exact):
+    """The gradient sweeps for a rotation of each molecule about its centre (orc_gradient_rot)
+    give -dE/domega of each molecule's energy: central differences under lab-frame rotations
+    of 1e-4 rad agree to 2e-9 (2 x 3 spheres, p = 10)."""
+    mts, pos, quat = moltypes(orc, 3, 10, 400)
+
+    def qmul(a, b):
+        w1, x1, y1, z1 = a
+        w2, x2, y2, z2 = b
+        return np.array([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                         w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2])
+    S, _ = solve(orc, mts, pos, quat, True, forces=False)
+    S.gradient_rot(1e-16, 5000)
+    h = 1e-4
+    for mol in range(2):
+        tq = S.grad_energy(mol)
+        fd = np.zeros(3)
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = 1.0
+            qp, qm = quat.copy(), quat.copy()
+            qp[mol] = qmul(np.r_[np.cos(h / 2), np.sin(h / 2) * e], quat[mol])
+            qm[mol] = qmul(np.r_[np.cos(h / 2), -np.sin(h / 2) * e], quat[mol])
+            fd[a] = -(solve(orc, mts, pos, qp, True, False)[0].pot(mol) -
+                      solve(orc, mts, pos, qm, True, False)[0].pot(mol)) / (2 * h)
+        assert np.abs(tq - fd).max() < 1e-6 * np.linalg.norm(fd), (mol, tq, fd)
