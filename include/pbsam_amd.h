@@ -46,6 +46,16 @@ typedef struct pbsam_system pbsam_system;
 int pbsam_ctx_create(double kappa, double eps_out, int device, pbsam_ctx **out);
 void pbsam_ctx_destroy(pbsam_ctx *ctx);
 const char *pbsam_last_error(const pbsam_ctx *ctx);
+/* Memory plan.  The reference recomputes every re-expansion operator per use ("LowMemory",
+ * src/molecule.cpp:1286-1293); this library stores two kinds of them when they fit: the
+ * far-pair translation matrices of a molecule type (made for its frozen far values and
+ * reused by the gradient phase) and its numerical re-expansion maps (gradient phase).
+ * Stores that would take the context's total past `bytes` are not made (default: a quarter
+ * of the device's memory); their users then make the operators again in every step that
+ * needs them, with results equal bit for bit.  Applies to molecule types whose far values
+ * or maps are built after the call.  pbsam_ctx_stored reports the bytes stored and the cap. */
+int pbsam_ctx_set_store_limit(pbsam_ctx *ctx, long long bytes);
+int pbsam_ctx_stored(const pbsam_ctx *ctx, long long *bytes, long long *limit);
 /* Debye kappa (1/A) from a salt concentration in mol/L, as src/selfp_makemat.cpp:59-60 */
 double pbsam_kappa_from_salt(double salt_molar, double eps_out, double temp_k);
 

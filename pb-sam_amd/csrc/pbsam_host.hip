@@ -195,6 +195,9 @@ struct Tune {
   int mgemv_wg = 4096;    // target workgroups of a stored-map GEMV launch (sets its point split)
   int pgemm_min = 4;      // jobs sharing a stored map from which it runs as a matrix-core product
   int grad_window = 64;   // gradient sweep steps enqueued between read-backs at most
+  int p2l_wg = 1024;      // target workgroups of a point-to-expansion launch (sets its point split)
+  int surf_wg = 512;      // surface-charge launches below this many workgroups give each
+                          // point more lanes
 };
 
 struct pbsam_ctx {
@@ -208,12 +211,18 @@ struct pbsam_ctx {
   bool prof = false;
   DBuf<unsigned long long> d_work;  // [kernel][flops, bytes]
   DBuf<double> d_qpart;             // per-workgroup partial surface-charge totals
+  DBuf<double> d_ppart, d_ppart2;   // partial sums of split point-to-expansion launches (per stream)
   DBuf<int> d_xbin;                 // re-expansion orders, order classes and their counts
   DBuf<int> d_xbin2;                // the same for launches on stream2
   KProf kp[PBSAM_K_COUNT];
   // a second stream (default priority) for independent launch chains, joined by events
   hipStream_t stream0 = nullptr, stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // Operators that can be stored or made again per use (far-pair translation matrices of
+  // a molecule type, its numerical re-expansion maps): bytes stored so far and the cap
+  // (a quarter of the device's memory unless pbsam_ctx_set_store_limit sets another).
+  // Stores that would pass the cap are not made; their users recompute per step.
+  size_t stored = 0, store_limit = 0;
   Tune tune;                                          // settings of the system being run
   std::map<std::array<long long, 3>, Tune> tuned;     // pbsam_autotune results by (p, spheres, systems)
 };
@@ -454,14 +463,39 @@ static void launch_xform(pbsam_ctx *c, int njobs, XArgs a) {
     launch_xform_q(c, njobs, b, std::min(qmax[k], a.p));
   }
 }
-static void launch_p2l(pbsam_ctx *c, int njobs, PArgs a) {
+// Point-to-expansion sums of njobs jobs with at most maxpts points each.  A launch of few
+// jobs splits each job's points into ranges of whole tiles so that about tune.p2l_wg
+// workgroups run (one job alone in one workgroup walks hundreds of tiles one after the
+// other); the ranges' partial sums are added in range order by k_p2lpsum.
+static void launch_p2l(pbsam_ctx *c, int njobs, PArgs a, int maxpts) {
   if (!njobs) return;
   a.tile = p2l_tile(a.p);
   size_t lds = p2l_lds(a.p);
+  const int tiles = std::max(1, (maxpts + a.tile - 1) / a.tile);
+  int nsplit = std::min(tiles, std::max(1, (c->tune.p2l_wg + njobs - 1) / njobs));
+  a.nsplit = 1;
+  if (nsplit > 1) {
+    a.spl = (tiles + nsplit - 1) / nsplit * a.tile;
+    a.nsplit = (maxpts + a.spl - 1) / a.spl;
+  }
+  if (a.nsplit > 1) {
+    const size_t need = (size_t)njobs * a.nsplit * 2 * a.p * a.p;
+    DBuf<double> &pb = (c->stream == c->stream2) ? c->d_ppart2 : c->d_ppart;  // one per stream
+    if (need > pb.n) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      pb.alloc(std::max(need, (size_t)1 << 20));
+    }
+    a.part = pb.p;
+  }
   a.work = prof_begin(c, PBSAM_K_P2L);
-  PLAUNCH(c, PBSAM_K_P2L, k_p2l, dim3(njobs), dim3(PT), lds, c->stream, c->tabs, a);
+  PLAUNCH(c, PBSAM_K_P2L, k_p2l, dim3(njobs, a.nsplit), dim3(PT), lds, c->stream, c->tabs, a);
   HIPCHK(hipGetLastError());
   prof_end(c, PBSAM_K_P2L);
+  if (a.nsplit > 1) {
+    const int nset = a.q2 ? 2 : 1;
+    hipLaunchKernelGGL(k_p2lpsum, dim3(njobs, (nset * a.p * a.p + 255) / 256), dim3(256), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+  }
 }
 static void launch_solve(pbsam_ctx *c, int n, SArgs a) {
   if (!n) return;
@@ -474,7 +508,13 @@ static void launch_solve(pbsam_ctx *c, int n, SArgs a) {
 // surface charges of n listed spheres with at most maxcnt exposed points each
 static void launch_surface(pbsam_ctx *c, int n, QArgs a, int maxcnt) {
   if (!n) return;
-  a.nchunk = std::max(1, (maxcnt + ST - 1) / ST);
+  // lanes per point: enough workgroups for the chip when few spheres are listed, and
+  // no more lanes than columns
+  a.lanes = 1;
+  while (a.lanes < 16 && 2 * a.lanes <= a.p &&
+         (long long)n * ((maxcnt + ST / a.lanes - 1) / (ST / a.lanes)) < c->tune.surf_wg)
+    a.lanes *= 2;
+  a.nchunk = std::max(1, (maxcnt + ST / a.lanes - 1) / (ST / a.lanes));
   const size_t need = (size_t)2 * n * a.nchunk;
   if (need > c->d_qpart.n) {
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -623,6 +663,8 @@ struct pbsam_moltype {
   std::vector<long long> p2l_off;
   DBuf<double> d_p2lm;
   DBuf<double> d_ysurf;  // harmonics of every exposed point about its sphere centre [point][p^2]
+  bool p2l_stored = false;  // numerical maps stored (else made per gradient step)
+  size_t stored = 0;        // bytes of optional stores counted against the context's cap
   // near entries per target (source, analytical job or numerical job) and by source
   struct Entry {
     int src, xj, pj;
@@ -778,11 +820,14 @@ static void moltype_setup(pbsam_moltype *m, const pbsam_moltype_desc *d) {
   pa.jobs = djm.p;
   pa.out1 = dmul.p;
   pa.multipole = 1;
-  launch_p2l(c, (int)jm.size(), pa);
+  int maxq = 0;
+  for (const std::vector<PJob> *v : {&jm, &jl})
+    for (const PJob &J : *v) maxq = std::max(maxq, J.npts);
+  launch_p2l(c, (int)jm.size(), pa, maxq);
   pa.jobs = djl.p;
   pa.out1 = dloc.p;
   pa.multipole = 0;
-  launch_p2l(c, (int)jl.size(), pa);
+  launch_p2l(c, (int)jl.size(), pa, maxq);
   std::vector<double> mul = dmul.download(s), loc = dloc.download(s);
   // cal_DVFixed expcenter.cpp:1042-1060
   std::vector<double> Dfix((size_t)ns * P2), Vfix((size_t)ns * P2), E0((size_t)ns * P2);
@@ -993,7 +1038,7 @@ static double moltype_round(pbsam_moltype *m) {
   pa.kap2 = 1;
   pa.multipole = 0;
   pa.p = p;
-  launch_p2l(c, m->n_pj, pa);
+  launch_p2l(c, m->n_pj, pa, m->max_nexp);
   SArgs sa{};
   sa.sph = m->d_all.p;
   sa.lst_off = m->d_lst_off.p;
@@ -1125,7 +1170,7 @@ static void moltype_gs_pass(pbsam_moltype *m, const std::vector<int> &S) {
     xa.sel = m->d_selx.p + J.offx[seg];
     launch_xform(c, J.offx[seg + 1] - J.offx[seg], xa);
     pa.sel = m->d_selp.p + J.offp[seg];
-    launch_p2l(c, J.offp[seg + 1] - J.offp[seg], pa);
+    launch_p2l(c, J.offp[seg + 1] - J.offp[seg], pa, m->max_nexp);
   }
 }
 
@@ -1216,32 +1261,30 @@ static void moltype_values(pbsam_moltype *m) {
     }
     foff[ki + 1] = (int)flst.size();
   }
-  m->d_Tfar.alloc((size_t)tj.size() * TS);
-  for (size_t i = 0; i < fj.size(); i++) {
-    fj[i].TH = m->d_Tfar.p + (size_t)(2 * i) * TS;
-    fj[i].TF = m->d_Tfar.p + (size_t)(2 * i + 1) * TS;
+  // The far pairs' translation matrices are kept for the gradient phase when they fit the
+  // context's store cap; otherwise they are made chunk by chunk here and again per step
+  // there (tpair null).
+  const size_t needT = tj.size() * TS * sizeof(double);
+  const bool keepT = m->d_Tfar.n == tj.size() * (size_t)TS || c->stored + needT <= c->store_limit;
+  if (keepT && m->d_Tfar.n != tj.size() * (size_t)TS) {
+    m->d_Tfar.alloc(tj.size() * TS);
+    c->stored += needT;
+    m->stored += needT;
   }
+  const size_t CHF = keepT ? std::max<size_t>(fj.size(), 1)
+                           : std::max<size_t>(1, ((size_t)1 << 30) / (2 * (size_t)TS * sizeof(double)));
+  DBuf<double> Tch;
+  if (!keepT) Tch.alloc(std::min(CHF, std::max<size_t>(fj.size(), 1)) * 2 * TS);
   DBuf<XJob> dfj;
   DBuf<TJob> dtj;
   DBuf<int> dfoff, dflst;
   DBuf<double> fsH, fsF;
-  dfj.upload(fj, s);
-  dtj.upload(tj, s);
   dfoff.upload(foff, s);
   if (flst.empty()) flst.push_back(0);
   dflst.upload(flst, s);
   fsH.alloc((size_t)std::max<size_t>(fj.size(), 1) * P2);
   fsF.alloc((size_t)std::max<size_t>(fj.size(), 1) * P2);
-  launch_trans(c, (int)tj.size(), dtj.p, p, m->d_Tfar.p);
-  m->tpair.assign((size_t)ns * ns, nullptr);
-  for (int ki = 0, f = 0; ki < ns; ki++) {
-    const SphHost &Si = m->sph[ki];
-    for (size_t e = 0; e < Si.near.size(); e++)
-      if (m->ana[ki][e] >= 0) m->tpair[(size_t)ki * ns + Si.near[e]] = m->d_T.p + (size_t)(2 * m->ana[ki][e]) * TS;
-    for (int kj : Si.far) m->tpair[(size_t)ki * ns + kj] = m->d_Tfar.p + (size_t)(2 * f++) * TS;
-  }
   XArgs xa{};
-  xa.jobs = dfj.p;
   xa.H = m->d_Hself.p;
   xa.F = m->d_Fself.p;
   xa.TQH = m->d_TQHself.p;
@@ -1250,7 +1293,34 @@ static void moltype_values(pbsam_moltype *m) {
   xa.slotF = fsF.p;
   xa.p = p;
   xa.full = 1;
-  launch_xform(c, (int)fj.size(), xa);
+  for (size_t c0 = 0; c0 < fj.size(); c0 += CHF) {
+    const size_t n = std::min(CHF, fj.size() - c0);
+    double *Tb = keepT ? m->d_Tfar.p + c0 * 2 * TS : Tch.p;
+    std::vector<XJob> cj(fj.begin() + c0, fj.begin() + c0 + n);
+    std::vector<TJob> ct(tj.begin() + 2 * c0, tj.begin() + 2 * (c0 + n));
+    for (size_t i = 0; i < n; i++) {
+      cj[i].TH = Tb + (2 * i) * TS;
+      cj[i].TF = Tb + (2 * i + 1) * TS;
+      ct[2 * i].off = (long long)(2 * i) * TS;
+      ct[2 * i + 1].off = (long long)(2 * i + 1) * TS;
+    }
+    dfj.upload(cj, s);
+    dtj.upload(ct, s);
+    launch_trans(c, (int)ct.size(), dtj.p, p, Tb);
+    xa.jobs = dfj.p;
+    launch_xform(c, (int)n, xa);
+    HIPCHK(hipStreamSynchronize(s));  // the chunk's tables and matrices are reused
+  }
+  m->tpair.assign((size_t)ns * ns, nullptr);
+  for (int ki = 0, f = 0; ki < ns; ki++) {
+    const SphHost &Si = m->sph[ki];
+    for (size_t e = 0; e < Si.near.size(); e++)
+      if (m->ana[ki][e] >= 0) m->tpair[(size_t)ki * ns + Si.near[e]] = m->d_T.p + (size_t)(2 * m->ana[ki][e]) * TS;
+    for (int kj : Si.far) {
+      if (keepT) m->tpair[(size_t)ki * ns + kj] = m->d_Tfar.p + (size_t)(2 * f) * TS;
+      f++;
+    }
+  }
   m->d_LFfar.alloc((size_t)ns * P2);
   m->d_LHfar.alloc((size_t)ns * P2);
   LArgs la{m->d_all.p, dfoff.p, dflst.p, fsH.p, nullptr, m->d_LHfar.p, nullptr, p};
@@ -1275,7 +1345,7 @@ static void moltype_values(pbsam_moltype *m) {
   pa.kap1 = 0;
   pa.kap2 = 1;
   pa.p = p;
-  launch_p2l(c, m->n_pj, pa);
+  launch_p2l(c, m->n_pj, pa, m->max_nexp);
   m->d_LFintra.alloc((size_t)ns * P2);
   m->d_LHintra.alloc((size_t)ns * P2);
   LArgs lb{m->d_all.p, m->d_lst_off.p, m->d_lst.p, m->d_slotH.p, m->d_LHfar.p, m->d_LHintra.p, nullptr, p};
@@ -1331,6 +1401,9 @@ struct GradState {
   DBuf<int> is_off, is_lst, ps_off, ps_lst, ident, esph, eitem, sbuf;
   DBuf<MJob> mj;
   DBuf<double> mpart;
+  DBuf<double> Tr, Mr;                     // per-step intra translation matrices and numerical maps
+  DBuf<TJob> trj;                          //   of molecule types that do not store them, and
+  DBuf<MBuild> mbj;                        //   their jobs
   DBuf<double> s3spec;                     // k_solve3 part workgroups: [row][F, H][3][p^2]
   DBuf<unsigned long long> s3mask;         // [row][parts][2]
   DBuf<unsigned int> s3cnt;                // [row], zero between launches
@@ -1815,7 +1888,7 @@ static double system_round(pbsam_system *S, bool round1, bool refresh) {
     pa.kap1 = 0;
     pa.kap2 = 1;
     pa.p = p;
-    launch_p2l(c, S->n_pintra, pa);
+    launch_p2l(c, S->n_pintra, pa, S->max_nexp);
     sa.lst_off = S->d_intra_off.p;
     sa.lst = S->d_intra_lst.p;
     sa.farH = S->d_LHfar.p;
@@ -1978,7 +2051,7 @@ static void system_gs_round(pbsam_system *S, const std::vector<double> &dev, con
     xb.sel = sb + r[8];
     launch_xform(c, r[9], xb);
     pa.sel = sb + r[10];
-    launch_p2l(c, r[11], pa);
+    launch_p2l(c, r[11], pa, S->max_nexp);
   }
 }
 
@@ -2096,6 +2169,9 @@ int pbsam_ctx_create(double kappa, double eps_out, int device, pbsam_ctx **out) 
     int lds = 0;
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     c->lds_max = (size_t)lds;
+    size_t mfree = 0, mtotal = 0;
+    HIPCHK(hipMemGetInfo(&mfree, &mtotal));
+    c->store_limit = mtotal / 4;
     build_tables(c.get());
     return PBSAM_OK;
   });
@@ -2113,14 +2189,14 @@ void pbsam_ctx_destroy(pbsam_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &P : c->kp)
     for (hipEvent_t e : P.ev) (void)hipEventDestroy(e);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   c->d_work.release();
   c->d_qpart.release();
   c->d_xbin.release();
   c->d_xbin2.release();
-  if (c->stream2) {
-    (void)hipStreamSynchronize(c->stream2);
-    (void)hipStreamDestroy(c->stream2);
-  }
+  c->d_ppart.release();
+  c->d_ppart2.release();
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -2128,6 +2204,18 @@ void pbsam_ctx_destroy(pbsam_ctx *c) {
 }
 
 const char *pbsam_last_error(const pbsam_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int pbsam_ctx_set_store_limit(pbsam_ctx *c, long long bytes) {
+  if (!c || bytes < 0) return PBSAM_ERR_BADARG;
+  c->store_limit = (size_t)bytes;
+  return PBSAM_OK;
+}
+int pbsam_ctx_stored(const pbsam_ctx *c, long long *bytes, long long *limit) {
+  if (!c) return PBSAM_ERR_BADARG;
+  if (bytes) *bytes = (long long)c->stored;
+  if (limit) *limit = (long long)c->store_limit;
+  return PBSAM_OK;
+}
 
 int pbsam_profile(pbsam_ctx *c, int on) {
   if (!c) return PBSAM_ERR_BADARG;
@@ -2202,7 +2290,11 @@ int pbsam_moltype_create(pbsam_ctx *c, const pbsam_moltype_desc *d, pbsam_moltyp
   return rc;
 }
 
-void pbsam_moltype_destroy(pbsam_moltype *m) { delete m; }
+void pbsam_moltype_destroy(pbsam_moltype *m) {
+  if (!m) return;
+  m->c->stored -= std::min(m->c->stored, m->stored);
+  delete m;
+}
 
 int pbsam_moltype_info(const pbsam_moltype *m, int *n_sph, int *p, long long *n_points) {
   if (!m) return PBSAM_ERR_BADARG;
@@ -2927,7 +3019,7 @@ int pbsam_xform_num_batch(pbsam_ctx *c, int n, int p, const double *P, const dou
     pa.kap1 = 0;
     pa.kap2 = 1;
     pa.p = p;
-    launch_p2l(c, n, pa);
+    launch_p2l(c, n, pa, npts);
     std::vector<double> r1 = o1.download(s), r2 = o2.download(s);
     memcpy(outF, r1.data(), sizeof(double) * n * P2);
     memcpy(outH, r2.data(), sizeof(double) * n * P2);

@@ -563,8 +563,10 @@ __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
 // point-to-expansion sums: CLocalExpan / CMulExpan of point charges
 // (expansion.cpp:203-340), used for the numerical re-expansion of surface charges
 // (CXFormNIntra::xformFH, xforms.cpp:528-550) and for fixed-charge expansions.
-// One workgroup per job; points are processed in tiles held in LDS, one thread per
-// coefficient accumulates over the points in their input order.
+// One workgroup per job, or (nsplit > 1) per (job, contiguous range of spl points); points
+// are processed in tiles held in LDS, one thread per coefficient accumulates over the
+// points of its range in their input order.  Split launches write per-range partial sums
+// that k_p2lpsum adds in range order (a fixed order, so results do not vary run to run).
 // ----------------------------------------------------------------------------
 constexpr int PT = 256;
 
@@ -590,6 +592,10 @@ struct PArgs {
   // gradient charges: per-point squared norms of the 3-vector charge of each set; a point
   // whose norm is below 1e-15 is skipped (triexpan.cpp:127-153).  Null: skip zero charges.
   const double *n1, *n2;
+  // point split (nsplit > 1): workgroup (j, s) takes points [s*spl, (s+1)*spl) of job j and
+  // writes its partial sums to part[(j*nsplit + s)*2*P2 ...] (set 1, then set 2)
+  int nsplit, spl;
+  double *part;
 };
 
 __global__ void __launch_bounds__(PT) k_p2l(Tabs tb, PArgs A) {
@@ -603,7 +609,10 @@ __global__ void __launch_bounds__(PT) k_p2l(Tabs tb, PArgs A) {
   if (two && J.src >= 0 && A.TQ2) p2 = compute_order(rho0, A.TQ2[J.src], J.a_src, p);
   const int q = p1 > p2 ? p1 : p2;
   const int T = A.tile;
-  if (A.work && tid == 0) {
+  const bool split = A.nsplit > 1;
+  const int pt_lo = split ? min(J.npts, (int)blockIdx.y * A.spl) : 0;
+  const int pt_hi = split ? min(J.npts, pt_lo + A.spl) : J.npts;
+  if (A.work && tid == 0 && blockIdx.y == 0) {
     const double w = work_p2l_point(p1) + (two ? work_p2l_point(p2) : 0.0);
     atomicAdd(&A.work[0], (unsigned long long)(w * J.npts));
     atomicAdd(&A.work[1], (unsigned long long)(8 * ((size_t)J.npts * (two ? 5 : 4) + (two ? 2 : 1) * P2)));
@@ -614,8 +623,8 @@ __global__ void __launch_bounds__(PT) k_p2l(Tabs tb, PArgs A) {
   double *geo = w2 + T * q;          // [T][3]: cost, sint, phi
   double acc1[4] = {0, 0, 0, 0}, acc2[4] = {0, 0, 0, 0};
   const int q2 = q * q;
-  for (int t0 = 0; t0 < J.npts; t0 += T) {
-    const int nt = min(T, J.npts - t0);
+  for (int t0 = pt_lo; t0 < pt_hi; t0 += T) {
+    const int nt = min(T, pt_hi - t0);
     // phase 0: geometry and radial factors of each point
     for (int t = tid; t < nt; t += PT) {
       const double *x = A.xyz + 3 * (size_t)(J.pt_off + t0 + t);
@@ -708,8 +717,8 @@ __global__ void __launch_bounds__(PT) k_p2l(Tabs tb, PArgs A) {
     }
     __syncthreads();
   }
-  double *o1 = A.out1 + (size_t)J.slot * P2;
-  double *o2 = two ? A.out2 + (size_t)J.slot * P2 : nullptr;
+  double *o1 = split ? A.part + ((size_t)blockIdx.x * A.nsplit + blockIdx.y) * 2 * P2 : A.out1 + (size_t)J.slot * P2;
+  double *o2 = two ? (split ? o1 + P2 : A.out2 + (size_t)J.slot * P2) : nullptr;
   for (int i = tid; i < P2; i += PT) {
     o1[i] = 0.0;
     if (o2) o2[i] = 0.0;
@@ -724,6 +733,21 @@ __global__ void __launch_bounds__(PT) k_p2l(Tabs tb, PArgs A) {
     if (n < p1) o1[c] = acc1[k];
     if (o2 && n < p2) o2[c] = acc2[k];
   }
+}
+
+// Partial sums of a split k_p2l launch added in range order: one thread per coefficient of
+// each job's output slot(s).
+__global__ void __launch_bounds__(256) k_p2lpsum(PArgs A) {
+  const int jb = blockIdx.x;
+  const PJob J = A.jobs[A.sel ? A.sel[jb] : jb];
+  const int P2 = A.p * A.p, nset = A.q2 ? 2 : 1;
+  const int c = blockIdx.y * 256 + threadIdx.x;
+  if (c >= nset * P2) return;
+  const double *src = A.part + (size_t)jb * A.nsplit * 2 * P2 + c;
+  double acc = 0.0;
+  for (int k = 0; k < A.nsplit; k++) acc += src[(size_t)k * 2 * P2];
+  if (c < P2) A.out1[(size_t)J.slot * P2 + c] = acc;
+  else A.out2[(size_t)J.slot * P2 + c - P2] = acc;
 }
 
 // ----------------------------------------------------------------------------
@@ -1440,19 +1464,23 @@ struct QArgs {
   int p;
   unsigned long long *work;  // instrumentation: [flops, bytes] (null: off)
   double *part;              // [workgroup][2] partial sums of |qF|, |qH|
-  int nchunk;                // workgroups per sphere (ST points each)
+  int nchunk;                // workgroups per sphere (ST / lanes points each)
+  int lanes;                 // lanes per point (1, 2, 4, 8 or 16; 0 means 1)
 };
-// One workgroup per (sphere, chunk of ST points); one point per thread.  The totals
-// TQF/TQH are finished by k_surface_totals from the per-workgroup partial sums.
+// One workgroup per (sphere, chunk of ST/lanes points); `lanes` adjacent threads per point
+// share its harmonic columns m (lane l takes m = kL + l for even k and kL + L-1-l for odd
+// k, which balances the columns' lengths p - m) and add their sums with lane shuffles.
+// The totals TQF/TQH are finished by k_surface_totals from the per-workgroup partial sums.
 __global__ void __launch_bounds__(ST) k_surface(Tabs tb, QArgs A) {
   extern __shared__ double sh[];
   const int p = A.p, P2 = p * p, tid = threadIdx.x;
+  const int L = A.lanes > 1 ? A.lanes : 1, lane = tid % L, PW = ST / L;
   const int ch = blockIdx.x % A.nchunk;
   const int g = A.sph[blockIdx.x / A.nchunk];
   double *Fs = sh, *Hs = Fs + P2, *red = Hs + P2;
   const double *cChgH = A.consts + (size_t)g * 10 * p + 9 * p;
   const int off = A.pt_off[g], cnt = A.pt_cnt[g];
-  if (ch * ST >= cnt) {  // uniform across the workgroup
+  if (ch * PW >= cnt) {  // uniform across the workgroup
     if (tid == 0) {
       A.part[2 * blockIdx.x] = 0.0;
       A.part[2 * blockIdx.x + 1] = 0.0;
@@ -1466,14 +1494,16 @@ __global__ void __launch_bounds__(ST) k_surface(Tabs tb, QArgs A) {
   __syncthreads();
   const double dA = 4 * M_PI / double(A.nspx[g]);
   double tF = 0.0, tH = 0.0;
-  const int h = ch * ST + tid;
-  if (h < cnt) {
+  const int h = ch * PW + tid / L;
+  if (h < cnt) {  // uniform across each point's lanes
     const double *x = A.xyz + 3 * (size_t)(off + h);
     double rho, th, ph;
     cart_to_sph(x[0], x[1], x[2], rho, th, ph);
     const double cost = cos(th), sint = sin(th);
     double sF = 0.0, sH = 0.0;
-    for (int m = 0; m < p; m++) {
+    for (int k = 0; k * L < p; k++) {
+      const int m = k * L + ((k & 1) ? L - 1 - lane : lane);
+      if (m >= p) continue;
       sh_column(tb, p, m, cost, sint, ph, [&](int n, double re, double im) {
         const double c2 = (2 * n + 1) / (4.0 * M_PI), chH = cChgH[n];
         if (m == 0) {
@@ -1486,11 +1516,17 @@ __global__ void __launch_bounds__(ST) k_surface(Tabs tb, QArgs A) {
         }
       });
     }
-    const double qf = sF * dA, qh = sH * dA;
-    A.qF[off + h] = qf;
-    A.qH[off + h] = qh;
-    tF = fabs(qf);
-    tH = fabs(qh);
+    for (int o = 1; o < L; o <<= 1) {  // the point's lanes are adjacent and all active
+      sF += __shfl_xor(sF, o, 64);
+      sH += __shfl_xor(sH, o, 64);
+    }
+    if (lane == 0) {
+      const double qf = sF * dA, qh = sH * dA;
+      A.qF[off + h] = qf;
+      A.qH[off + h] = qh;
+      tF = fabs(qf);
+      tH = fabs(qh);
+    }
   }
   tF = block_sum(tF, red);
   tH = block_sum(tH, red);

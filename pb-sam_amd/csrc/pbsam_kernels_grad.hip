@@ -949,11 +949,12 @@ struct MBuild {
   double a_tgt;
 };
 // one wavefront per point; lane m evaluates harmonic column m
-__global__ void __launch_bounds__(256) k_p2lmat(Tabs tb, const MBuild *jobs, const double *xyz, double *mats,
-                                                int p) {
+__global__ void __launch_bounds__(256) k_p2lmat(Tabs tb, const MBuild *jobs, const int *sel, const double *xyz,
+                                                double *mats, int p) {
   __shared__ double Y[4][PMAX * PMAX];
   __shared__ double wF[4][PMAX], wH[4][PMAX];
-  const MBuild J = jobs[blockIdx.y];
+  const MBuild J = jobs[sel ? sel[blockIdx.y] : blockIdx.y];
+  if (J.npts <= 0) return;  // a source with no exposed point has an empty map
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64, P2 = p * p;
   const int h0 = blockIdx.x * 4 + w;
   const bool valid = h0 < J.npts;

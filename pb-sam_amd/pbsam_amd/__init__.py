@@ -42,6 +42,7 @@ SYMBOLS = [
     "pbsam_potential", "pbsam_total_int_energy", "pbsam_system_set_flags", "pbsam_autotune",
     "pbsam_torques_exact",
     "pbsam_xform_h_batch_noalloc", "pbsam_xform_h_batch_noalloc_bytes",
+    "pbsam_ctx_set_store_limit", "pbsam_ctx_stored",
 ]
 POT_REFERENCE_FRAME = 1
 BD_ESCAPED, BD_DOCKED, BD_RUNNING, BD_STUCK, BD_FAILED = range(5)
@@ -89,6 +90,8 @@ def load():
         "pbsam_ctx_create": ([D, D, I, P], I),
         "pbsam_ctx_destroy": ([P], None),
         "pbsam_last_error": ([P], ctypes.c_char_p),
+        "pbsam_ctx_set_store_limit": ([P, ctypes.c_longlong], I),
+        "pbsam_ctx_stored": ([P, P, P], I),
         "pbsam_kappa_from_salt": ([D, D, D], D),
         "pbsam_moltype_create": ([P, ctypes.POINTER(MolDesc), P], I),
         "pbsam_moltype_destroy": ([P], None),
@@ -262,6 +265,17 @@ class Context:
 
     def sync(self):
         self.check(self.L.pbsam_sync(self.h))
+
+    def set_store_limit(self, nbytes):
+        """Cap on the optional operator stores (far-pair translation matrices, numerical
+        maps) of molecule types set up after the call; 0 makes them all per use."""
+        self.check(self.L.pbsam_ctx_set_store_limit(self.h, int(nbytes)))
+
+    def stored(self):
+        """(bytes of optional operator stores, their cap)"""
+        b, lim = ctypes.c_longlong(), ctypes.c_longlong()
+        self.check(self.L.pbsam_ctx_stored(self.h, ctypes.byref(b), ctypes.byref(lim)))
+        return b.value, lim.value
 
     def profile(self, on):
         """Start (reset) or stop the per-kernel HIP-event timing and work counters."""
