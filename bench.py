@@ -18,7 +18,10 @@ mutual solve alone (CMolecule::polarize_mutual, src/molecule.cpp:1989-2075), tim
 beside the plain one-system-per-call path timed on a sample of the same snapshots.
 `bd` is batched Brownian dynamics (CBDnam trajectories, one per system of a batch): 1024
 trajectories of two 100-sphere molecules at p=10, forces every step, reported as
-trajectory-steps/s beside the oracle's single trajectory on one host core.
+trajectory-steps/s beside the oracle's trajectories on the host cores.  `large` is the
+large-system workload: eight distinct 500-sphere molecules at p=30 with 2000 points per
+sphere in one system (set-up, mutual solves and force evaluations; operators past the
+context's store cap are made per use).
 
 Prints one JSON line.  `roofline` is the kernel with the largest share of device time,
 timed with HIP events around each of its launches on the library's stream, with the
@@ -328,6 +331,84 @@ def bd_leg(pbsam_amd, synth, ctx, kappa, ntraj, nsteps, cpu_budget):
     return out
 
 
+LARGE_MOLS, LARGE_P, LARGE_NPTS, LARGE_SEED = 8, 30, 2000, 7000
+
+
+def device_memory():
+    """(free, total) bytes of the current HIP device"""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    fr, tot = ctypes.c_size_t(), ctypes.c_size_t()
+    hip.hipMemGetInfo(ctypes.byref(fr), ctypes.byref(tot))
+    return fr.value, tot.value
+
+
+def large_leg(pbsam_amd, synth, ctx, nsph, solves, evals):
+    """The large-system workload: eight distinct synthetic molecules of `nsph` spheres at
+    p = 30 with 2000 points per sphere, centred on a 2 x 2 x 2 grid whose spacing leaves 10 A
+    between the molecules' packing balls (a 200 A cube cannot hold eight 500-sphere balls
+    of radius 58.5 A: the grid spacing is 2R + 10 A), random orientations; spheres of
+    different molecules closer than 20 A are polarized (the reference's 10 A would polarize
+    none across 10 A gaps), interactions to 100 A.  Set-up (types, self-polarization, frozen
+    values) is timed apart; then `solves` mutual solves (stock tolerance) and `evals` complete
+    force+torque evaluations.  Operators beyond the context's store cap are made per use."""
+    import numpy as np
+    rng = np.random.default_rng(LARGE_SEED)
+    R = synth.ball_radius(nsph)
+    a = 2 * R + 10.0
+    t0 = time.perf_counter()
+    mols, mts = [], []
+    for i in range(LARGE_MOLS):
+        m = synth.molecule(nsph, seed=LARGE_SEED + 1 + i)
+        g = pbsam_amd.MolType(ctx, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], LARGE_P, eps_in=synth.EPS_IN,
+                              rcen=m["rcen"], npts=[LARGE_NPTS] * nsph)
+        g.selfpol(tol=0, max_rounds=0)
+        g.values()
+        mols.append(m)
+        mts.append(g)
+        print("large system: molecule type %d set up at %.1f s" % (i, time.perf_counter() - t0), file=sys.stderr,
+              flush=True)
+    t_setup = time.perf_counter() - t0
+    pos = np.array([[a * (i & 1), a * ((i >> 1) & 1), a * ((i >> 2) & 1)] for i in range(LARGE_MOLS)], dtype=float)
+    quat = np.array([synth.random_quat(rng) for _ in range(LARGE_MOLS)])
+    S = pbsam_amd.System(ctx, mts, pos, quat)
+    S.set_cutoffs(20.0, 100.0)
+    t0 = time.perf_counter()
+    r0, _ = S.solve(tol=0, max_rounds=0)
+    ctx.sync()
+    t_first = time.perf_counter() - t0
+    n_inter, n_pol = S.counts()
+    t0 = time.perf_counter()
+    for _ in range(solves):
+        rounds, _ = S.solve(tol=0, max_rounds=0)
+    ctx.sync()
+    t_solve = (time.perf_counter() - t0) / max(solves, 1)
+    t0 = time.perf_counter()
+    _, _, f, t = S.compute_forces(tol=0, max_rounds=0)  # first evaluation: stored operators built
+    ctx.sync()
+    t_first_eval = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        _, _, f, t = S.compute_forces(tol=0, max_rounds=0)
+    ctx.sync()
+    t_eval = (time.perf_counter() - t0) / max(evals, 1)
+    st, passes = S.status()[0], S.status()[2]
+    stored, limit = ctx.stored()
+    free, total = device_memory()
+    out = dict(molecules=LARGE_MOLS, spheres_per_molecule=nsph, p=LARGE_P, points_per_sphere=LARGE_NPTS,
+               grid_spacing_A=a, inter_rcutoff_A=20.0, interact_rcutoff_A=100.0,
+               setup_s=t_setup, first_solve_s=t_first, solves_per_s=1.0 / t_solve, mutual_rounds=rounds,
+               first_eval_s=t_first_eval, evals_per_s=1.0 / t_eval, grad_passes=int(np.sum(passes)),
+               inter_pairs=n_inter, polarized_spheres=n_pol, status=int(st[0]),
+               force_norm_max=float(np.abs(f).max()),
+               stored_GB=stored / 1e9, store_limit_GB=limit / 1e9, device_used_GB=(total - free) / 1e9,
+               device_total_GB=total / 1e9)
+    S.close()
+    for g in mts:
+        g.close()
+    return out
+
+
 def cpu_info():
     """host cores this process may use and the CPU model"""
     try:
@@ -493,6 +574,9 @@ def main():
     ap.add_argument("--tight-solves", type=int, default=1, help="mutual solves timed at the tight tolerance")
     ap.add_argument("--bd", type=int, default=1024, help="trajectories in the Brownian-dynamics leg (0: skip it)")
     ap.add_argument("--bd-steps", type=int, default=3, help="timed steps of the Brownian-dynamics leg")
+    ap.add_argument("--large", type=int, default=500, metavar="NSPH",
+                    help="the large-system workload: 8 molecules of NSPH spheres at p=30 (500: full size; 0: skip)")
+    ap.add_argument("--large-evals", type=int, default=1, help="timed evaluations of the large system")
     ap.add_argument("--pot-grid", type=int, default=0, metavar="N",
                     help="also time pbsam_potential on an N^3 grid around the solved system (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -629,6 +713,12 @@ def main():
         potential = dict(points=len(pts), seconds=el, points_per_s=len(pts) / el,
                          grid="%d^3 points spanning 130 A around the two molecules" % args.pot_grid)
 
+    large = None
+    if args.large > 0:
+        large = large_leg(pbsam_amd, synth, ctx, args.large, 1, args.large_evals)
+        print("large system (8 x %d spheres): %.3f solves/s, %.3f evals/s" %
+              (args.large, large["solves_per_s"], large["evals_per_s"]), file=sys.stderr, flush=True)
+
     bd = None
     if args.bd > 0:
         bd = bd_leg(pbsam_amd, synth, ctx, kappa, args.bd, args.bd_steps, 0.0 if args.no_cpu else 5.0)
@@ -672,6 +762,7 @@ def main():
         "batch": batch,
         "mixed_batch": mixed,
         "bd": bd,
+        "large": large,
         "potential": potential,
         "roofline": roof,
         "roofline_hbm": roof_hbm,

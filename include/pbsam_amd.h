@@ -169,11 +169,21 @@ int pbsam_system_set_cutoffs(pbsam_system *sys, double inter_rcutoff, double int
  *                       (P = c_i - c_m); the reference adds +d/dP
  *                       (prepareDTA_LowMemory, src/molecule.cpp:1495-1521), which leaves its
  *                       forces 0.5-2 % off the energy gradient
+ *   PBSAM_SELF_FRAME    a sphere with interaction partners but none that polarize it takes
+ *                       its force from its rotated (lab-frame) self H against the lab-frame
+ *                       gradient of its partners' re-expanded self H; the reference pairs
+ *                       the unrotated self H with that lab-frame gradient
+ *                       (computeForceOn_0, src/expcenter.cpp:140, after
+ *                       prepareDTA_iself_LowMemory, src/molecule.cpp:1585-1629), so its
+ *                       forces change when the whole system is rotated
+ * Interactions of such spheres are not part of computePot's energy (their LS stays zero),
+ * so the exact-derivative statement covers the polarized spheres' energy.
  * Setting flags discards the last solution (as pbsam_system_set_pose does). */
 #define PBSAM_FULL_ORDERS 2
 #define PBSAM_NO_GRAD_SKIP 4
 #define PBSAM_EXT_SIGN 8
-#define PBSAM_EXACT_DERIVATIVE 15
+#define PBSAM_SELF_FRAME 16
+#define PBSAM_EXACT_DERIVATIVE 31
 int pbsam_system_set_flags(pbsam_system *sys, unsigned flags);
 int pbsam_solve_mutual(pbsam_system *sys, double tol, int max_rounds, unsigned flags, int *rounds_out);
 /* round 1 from self values, then a fixed number of simultaneous (Jacobi) rounds */

@@ -94,6 +94,8 @@ def lib():
         L.orc_set_grad_skip.restype = None
         L.orc_set_ext_sign.argtypes = [D]
         L.orc_set_ext_sign.restype = None
+        L.orc_set_self_frame.argtypes = [ctypes.c_int]
+        L.orc_set_self_frame.restype = None
         L.orc_rot_generator.argtypes = [I, P, P, I]
         L.orc_rot_generator.restype = None
         L.orc_gradient_rot.argtypes = [P, D, I]
@@ -282,6 +284,13 @@ def exact_derivative(on):
     L.orc_debug_full_order(1 if on else 0)
     L.orc_set_grad_skip(0.0 if on else 1e-15)
     L.orc_set_ext_sign(-1.0 if on else 1.0)
+    L.orc_set_self_frame(1 if on else 0)
+
+
+def self_frame(on):
+    """Forces of spheres with interaction partners only from their rotated self H (on=True)
+    or, as the reference does (expcenter.cpp:140), the unrotated one.  Process-wide."""
+    lib().orc_set_self_frame(1 if on else 0)
 
 
 class MolType:

@@ -17,8 +17,8 @@ LIBPATH = os.path.join(os.path.dirname(HERE), "libpbsam_amd.so")
 
 OK, ERR_BADARG, ERR_OVERLAP, ERR_NOCONV, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED, ERR_CHARGE, ERR_STATE = range(9)
 HROT_REFRESH = 1
-FULL_ORDERS, NO_GRAD_SKIP, EXT_SIGN = 2, 4, 8
-EXACT_DERIVATIVE = HROT_REFRESH | FULL_ORDERS | NO_GRAD_SKIP | EXT_SIGN
+FULL_ORDERS, NO_GRAD_SKIP, EXT_SIGN, SELF_FRAME = 2, 4, 8, 16
+EXACT_DERIVATIVE = HROT_REFRESH | FULL_ORDERS | NO_GRAD_SKIP | EXT_SIGN | SELF_FRAME
 PMAX = 32
 
 D = ctypes.c_double

@@ -14,6 +14,9 @@
   p = 10 against the oracle; 8 x 60 spheres at p = 30 with 2000 points per sphere, checked
   by what a rigid motion of the whole system must leave unchanged (energies, returned
   forces and torques are invariant under a global rotation and translation).
+* The large-system workload at full size: eight distinct 500-sphere molecules at p = 30
+  with 2000 points per sphere on a grid with 10 A gaps (about 200 GB of device memory;
+  operators past the store cap are made per use), by the same invariances.
 """
 import numpy as np
 import pytest
@@ -194,3 +197,55 @@ def test_eight_by_60_p30_rigid_motion_invariance(gpu, orc):
     _, rc3, f3, t3 = S3.compute_forces()
     assert rc3 == 0 and relerr(f3, f0) < 1e-3 and relerr(t3, t0) < 1e-3, (t3, t0)
     assert np.abs(f0).max() > 0
+
+
+def test_eight_by_500_p30_full_size():
+    """Eight distinct 500-sphere molecules at p = 30, 2000 points per sphere, packing balls
+    10 A apart, spheres of different molecules within 20 A polarized, interactions to 100 A:
+    the whole set-up and the force evaluation fit one device with part of the far-pair
+    translation matrices and all numerical maps made per use (the store cap of a quarter of
+    the device's memory is reached), and a rigid motion of the whole system leaves the
+    energies after two simultaneous rounds unchanged to 1e-9 and the returned forces (stock
+    schedule) to 1e-3.  Most spheres here interact without being polarized; their forces are
+    rotation covariant with PBSAM_SELF_FRAME (the reference's pairing of the unrotated self
+    H with a lab-frame gradient is not, tests/test_oracle_forces.py), which is set."""
+    import pbsam_amd
+    from pbsam_amd import synth
+    from conftest import KAPPA
+    ctx = pbsam_amd.Context(KAPPA)
+    gs = []
+    for i in range(8):
+        m = synth.molecule(500, seed=7001 + i)
+        g = pbsam_amd.MolType(ctx, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], 30, eps_in=4.0, rcen=m["rcen"],
+                              npts=[2000] * 500)
+        g.selfpol(tol=0, max_rounds=0)
+        g.values()
+        gs.append(g)
+    stored, limit = ctx.stored()
+    assert 0 < stored <= limit
+    a = 2 * synth.ball_radius(500) + 10.0
+    pos = np.array([[a * (i & 1), a * ((i >> 1) & 1), a * ((i >> 2) & 1)] for i in range(8)], dtype=float)
+    rng = np.random.default_rng(7000)
+    quat = np.array([synth.random_quat(rng) for _ in range(8)])
+    Q = synth.random_quat(np.random.default_rng(12))
+    shift = np.array([13.0, -7.0, 21.0])
+    res = []
+    for P_, Q_ in ((pos, quat), (np.array([quat_rot(Q, x) for x in pos]) + shift,
+                                 np.array([quat_mul(Q, q) for q in quat]))):
+        S = pbsam_amd.System(ctx, gs, P_, Q_)
+        S.set_cutoffs(20.0, 100.0)
+        S.set_flags(pbsam_amd.SELF_FRAME)
+        S.rounds(2)
+        n_inter, n_pol = S.counts()
+        assert n_inter > 0 and n_pol > 0
+        j = S.energy()
+        _, rc, f, t = S.compute_forces()
+        assert rc == 0
+        res.append((j, f))
+        S.close()
+    assert relerr(res[1][0], res[0][0]) < 1e-9
+    assert np.abs(res[0][1]).max() > 0 and np.isfinite(res[0][1]).all()
+    assert relerr(res[1][1], res[0][1]) < 1e-3
+    for g in gs:
+        g.close()
+    ctx.close()

@@ -51,3 +51,30 @@ def test_forces_reference_order(gpu, orc, n_sph, p, npts):
     # forces and torques: 1e-6 relative is required, far tighter is expected
     assert relerr(fg, fo) < 1e-8, (fg, fo)
     assert relerr(tg, to) < 1e-8, (tg, to)
+
+
+@pytest.mark.parametrize("self_frame", [False, True])
+def test_forces_with_interaction_only_spheres(gpu, orc, self_frame):
+    """Spheres whose partners interact but do not polarize them (interRcutoff 5 A,
+    interactRcutoff 100 A) take their forces from self values; the device matches the
+    oracle with the reference's pairing of the unrotated self H and with PBSAM_SELF_FRAME
+    (the rotated self H)."""
+    import pbsam_amd
+    import pyoracle
+    sg, so, keep = systems(gpu, orc, 6, 10, 400, gap=8.0)
+    for s in (sg, so):
+        s.set_cutoffs(5.0, 100.0)
+    if self_frame:
+        sg.set_flags(pbsam_amd.SELF_FRAME)
+    pyoracle.self_frame(self_frame)
+    try:
+        _, rc, fg, tg = sg.compute_forces(tol=0, max_rounds=0)
+        rco, fo, to = so.compute_forces(0, 0.0, 0)
+    finally:
+        pyoracle.self_frame(False)
+    assert rc == 0 and rco == 0
+    n_inter, n_pol = sg.counts()
+    assert n_pol < 12  # some spheres with partners are not polarized
+    assert np.abs(fo).max() > 0
+    assert relerr(fg, fo) < 1e-8, (fg, fo)
+    assert relerr(tg, to) < 1e-8, (tg, to)
