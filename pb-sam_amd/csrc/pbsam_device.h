@@ -121,6 +121,8 @@ struct XJob {       // analytical re-expansion of one source sphere onto one tar
   double a_src, a_tgt;
   int tord;         // order the translation matrices were made at (0: this job's H order)
   int pad;
+  const double *rot;  // stored rotation coefficients of the pair's direction (k_rotcoeff
+                      // layout, xi = pi, rows n < p), or null: generated in the kernel
 };
 struct NJob {       // numerical P2L of a source sphere's surface charges onto a target
   int src, tgt, slot, pad;

@@ -413,7 +413,7 @@ static void launch_xform_qp(pbsam_ctx *c, int n, const XArgs &a, int qmax) {
 // sharing for parallelism.
 static void launch_xform_q(pbsam_ctx *c, int n, XArgs a, int qmax) {
   a.work = prof_begin(c, PBSAM_K_XFORM);
-  if (a.planes == 3 && n < c->tune.xsplit_max && !a.cls) {
+  if (a.planes == 3 && n < c->tune.xsplit_max && !a.cls && !a.nosplit) {
     a.split = 1;
     launch_xform_qp<1>(c, n, a, qmax);
   } else if (a.planes == 3) {
@@ -664,6 +664,12 @@ struct pbsam_moltype {
   DBuf<double> d_p2lm;
   DBuf<double> d_ysurf;  // harmonics of every exposed point about its sphere centre [point][p^2]
   bool p2l_stored = false;  // numerical maps stored (else made per gradient step)
+  // rotation coefficients of the analytical intra pairs' directions (k_rotcoeff layout,
+  // xi = pi), stored when they fit the store cap: [ki*ns+kj] pointer or null (not stored,
+  // numerical, or a singular direction: generated in the re-expansion kernel)
+  bool rot_built = false;
+  DBuf<double> d_rpair;
+  std::vector<const double *> rpair;
   size_t stored = 0;        // bytes of optional stores counted against the context's cap
   // near entries per target (source, analytical job or numerical job) and by source
   struct Entry {
@@ -1401,6 +1407,7 @@ struct GradState {
   DBuf<int> is_off, is_lst, ps_off, ps_lst, ident, esph, eitem, sbuf;
   DBuf<MJob> mj;
   DBuf<double> mpart;
+  DBuf<double> Tfull;                      // inter translation matrices of a plan at full order
   DBuf<double> Tr, Mr;                     // per-step intra translation matrices and numerical maps
   DBuf<TJob> trj;                          //   of molecule types that do not store them, and
   DBuf<MBuild> mbj;                        //   their jobs

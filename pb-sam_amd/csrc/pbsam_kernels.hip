@@ -157,6 +157,8 @@ struct XArgs {
   int planes;          // consecutive source blocks / output slots per job (0 or 1: one)
   int split;           // 1: one plane per workgroup, plane = blockIdx.y (single-plane kernel)
   const int *cnt;      // with cls: device count of the class (workgroups past it leave)
+  int nosplit;         // host hint: keep the planes of a job in one workgroup (jobs with
+                       // stored rotations read their coefficients once for all planes)
 };
 
 // Orders of a batch of re-expansion jobs (the same rule as k_xform), per batch position,
@@ -212,6 +214,208 @@ __device__ __forceinline__ void sing_signs(int sing, int n, int m, double &sg, d
   }
 }
 
+// A job whose rotation coefficients are stored (XJob::rot, the pair direction's ROT(n,s,m)
+// for n < p as k_rotcoeff writes them, with xi = pi): forward-rotate every level at once,
+// translate, then back-rotate, three barriers in all instead of the in-LDS recursion's
+// four per level.  Each sum runs over the same terms in the same order as the generating
+// path (XG-lane segments for the forward rotation and the translation, m in order for the
+// back rotation), and the stored coefficients are the recursion's own values, so the
+// outputs are the same.  LDS: a/b values [NPL][F,H][q(q+1)/2] complex, then the source
+// blocks (fits the host's xform_lds for any q no larger than the launch's).
+template <int XT, int NPL>
+__device__ __forceinline__ void xapply_job(const XArgs &A, const XJob &J, int pH, int pF, int q, int tq) {
+  extern __shared__ double sh[];
+  constexpr int QM = (XT <= 128) ? 8 : ((XT == 256) ? 16 : PMAX);
+  constexpr int NO = (QM * (QM + 1) / 2 + XT - 1) / XT;
+  const int p = A.p, P2 = p * p, tid = threadIdx.x;
+  const bool intra = (J.kind == 1);
+  const int nfb = A.F ? NPL : 0;
+  const int nl = q * (q + 1) / 2;  // (n, m) values with m <= n < q, index n(n+1)/2 + m
+  cd *aH = reinterpret_cast<cd *>(sh);  // [NPL][nl], then [nfb][nl] for F
+  cd *aF = aH + NPL * nl;
+  double *hs = reinterpret_cast<double *>(aF + nfb * nl);  // [NPL][q^2]
+  double *fs = hs + NPL * q * q;                            // [nfb][q^2]
+  for (int i = tid; i < NPL * q * q; i += XT) {
+    const int r = i / (q * q), c = i % (q * q);
+    hs[i] = (c < pH * pH) ? A.H[(size_t)(J.src + r) * P2 + c] : 0.0;
+    if (nfb) fs[i] = (intra && c < pF * pF) ? A.F[(size_t)(J.src + r) * P2 + c] : 0.0;
+  }
+  const cd *ROT = reinterpret_cast<const cd *>(J.rot);
+  __syncthreads();
+  const int g = tid / XG, lane = tid % XG;
+  // (a) forward rotation of every level: a(n, m) = sum_s ROT(n,s,m) c(n,s)
+  for (int t = g; t < nl; t += XT / XG) {
+    int n = (int)((sqrtf(8.0f * t + 1.0f) - 1.0f) * 0.5f);
+    while (n * (n + 1) / 2 > t) n--;
+    while ((n + 1) * (n + 2) / 2 <= t) n++;
+    const int m = t - n * (n + 1) / 2;
+    const cd *row = ROT + rot_off(n) + m * (2 * n + 1) + n;  // row[s] = ROT(n,s,m)
+    cd sH[NPL], sF[NPL];
+#pragma unroll
+    for (int r = 0; r < NPL; r++) sH[r] = sF[r] = {0.0, 0.0};
+    for (int s = lane; s <= n; s += XG) {
+      const cd rp = row[s], rm = row[-s];
+#pragma unroll
+      for (int r = 0; r < NPL; r++) {
+        const double *h = hs + r * q * q, *f = fs + r * q * q;
+        if (s == 0) {
+          if (n < pH) sH[r] = cadd(sH[r], cmul(rp, cd{h[n * n], 0.0}));
+          if (intra && n < pF) sF[r] = cadd(sF[r], cmul(rp, cd{f[n * n], 0.0}));
+        } else {
+          if (n < pH) {
+            const cd c = {h[n * n + 2 * s - 1], h[n * n + 2 * s]};
+            sH[r] = cadd(sH[r], cadd(cmul(rp, c), cmul(rm, cconj(c))));
+          }
+          if (intra && n < pF) {
+            const cd c = {f[n * n + 2 * s - 1], f[n * n + 2 * s]};
+            sF[r] = cadd(sF[r], cadd(cmul(rp, c), cmul(rm, cconj(c))));
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NPL; r++) {
+      sH[r].x = seg_reduce(sH[r].x);
+      sH[r].y = seg_reduce(sH[r].y);
+      if (nfb) {
+        sF[r].x = seg_reduce(sF[r].x);
+        sF[r].y = seg_reduce(sF[r].y);
+      }
+      if (lane == 0) {
+        if (m == 0) { sH[r].y = 0.0; sF[r].y = 0.0; }  // Mout(n,0) keeps the real part
+        aH[r * nl + t] = sH[r];
+        if (nfb) aF[r * nl + t] = sF[r];
+      }
+    }
+  }
+  __syncthreads();
+  // (b) translation, one level per XG-lane group: the level's a values move to the lanes'
+  // registers (lane k holds l = m + k, m + k + XG, ...), then b(n, m) = sum_l M_m[n][l] a(l, m)
+  // for n in order is written over a(n, m) (only this group touches level m)
+  constexpr int NL = (QM + XG - 1) / XG;
+  for (int m = g; m < q; m += XT / XG) {
+    cd rH[NL][NPL], rF[NL][NPL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+      const int l = m + lane + k * XG;
+#pragma unroll
+      for (int r = 0; r < NPL; r++) {
+        rH[k][r] = (l < q) ? aH[r * nl + l * (l + 1) / 2 + m] : cd{0.0, 0.0};
+        rF[k][r] = (nfb && l < q) ? aF[r * nl + l * (l + 1) / 2 + m] : cd{0.0, 0.0};
+      }
+    }
+    const int w = tq - m;
+    for (int n = m; n < q; n++) {
+      cd sH[NPL], sF[NPL];
+#pragma unroll
+      for (int r = 0; r < NPL; r++) sH[r] = sF[r] = {0.0, 0.0};
+      if (n < pH) {
+        const double *Mm = J.TH + tmat_off(tq, m) + (size_t)(n - m) * w;
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+          const int l = m + lane + k * XG;
+          if (l < pH) {
+            const double tv = Mm[l - m];
+#pragma unroll
+            for (int r = 0; r < NPL; r++) sH[r] = cadd(sH[r], cscale(tv, rH[k][r]));
+          }
+        }
+      }
+      if (intra && n < pF) {
+        const double *Mm = J.TF + tmat_off(tq, m) + (size_t)(n - m) * w;
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+          const int l = m + lane + k * XG;
+          if (l < pF) {
+            const double tv = Mm[l - m];
+#pragma unroll
+            for (int r = 0; r < NPL; r++) sF[r] = cadd(sF[r], cscale(tv, rF[k][r]));
+          }
+        }
+      }
+      const int t = n * (n + 1) / 2 + m;
+#pragma unroll
+      for (int r = 0; r < NPL; r++) {
+        sH[r].x = seg_reduce(sH[r].x);
+        sH[r].y = seg_reduce(sH[r].y);
+        if (nfb) {
+          sF[r].x = seg_reduce(sF[r].x);
+          sF[r].y = seg_reduce(sF[r].y);
+        }
+        if (lane == 0) {
+          if (m == 0) { sH[r].y = 0.0; sF[r].y = 0.0; }
+          aH[r * nl + t] = sH[r];
+          if (nfb) aF[r * nl + t] = sF[r];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // (c) back rotation (rotcoeff.cpp:420-438): each thread's outputs (n, s), m in order
+  const int nout = nl;
+  for (int k = 0; k < NO; k++) {
+    const int ek = tid + k * XT;
+    if (ek >= nout) break;
+    int n = (int)((sqrtf(8.0f * ek + 1.0f) - 1.0f) * 0.5f);
+    while (n * (n + 1) / 2 > ek) n--;
+    while ((n + 1) * (n + 2) / 2 <= ek) n++;
+    const int s = ek - n * (n + 1) / 2;
+    cd accH[NPL], accF[NPL];
+#pragma unroll
+    for (int r = 0; r < NPL; r++) accH[r] = accF[r] = {0.0, 0.0};
+    const cd *rown = ROT + rot_off(n) + n;
+    for (int m = 0; m <= n; m++) {
+      const cd *row = rown + m * (2 * n + 1);
+      const cd rp = cconj(row[s]), rm = row[-s];
+      const int t = n * (n + 1) / 2 + m;
+#pragma unroll
+      for (int r = 0; r < NPL; r++) {
+        if (n < pH) {
+          const cd b = aH[r * nl + t];
+          if (m == 0) accH[r] = cadd(accH[r], cmul(cscale(1.0, rm), cd{b.x, 0.0}));
+          else accH[r] = cadd(accH[r], cadd(cmul(rp, b), cmul(rm, cconj(b))));
+        }
+        if (intra && n < pF) {
+          const cd b = aF[r * nl + t];
+          if (m == 0) accF[r] = cadd(accF[r], cmul(cscale(1.0, rm), cd{b.x, 0.0}));
+          else accF[r] = cadd(accF[r], cadd(cmul(rp, b), cmul(rm, cconj(b))));
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NPL; r++) {
+      double *oH = A.slotH + (size_t)(J.slot + r) * P2;
+      if (n < pH) {
+        if (s == 0) oH[n * n] = accH[r].x;
+        else { oH[n * n + 2 * s - 1] = accH[r].x; oH[n * n + 2 * s] = accH[r].y; }
+      } else {
+        if (s == 0) oH[n * n] = 0.0;
+        else { oH[n * n + 2 * s - 1] = 0.0; oH[n * n + 2 * s] = 0.0; }
+      }
+      if (intra) {
+        double *oF = A.slotF + (size_t)(J.slot + r) * P2;
+        if (n < pF) {
+          if (s == 0) oF[n * n] = accF[r].x;
+          else { oF[n * n + 2 * s - 1] = accF[r].x; oF[n * n + 2 * s] = accF[r].y; }
+        } else {
+          if (s == 0) oF[n * n] = 0.0;
+          else { oF[n * n + 2 * s - 1] = 0.0; oF[n * n + 2 * s] = 0.0; }
+        }
+      }
+    }
+  }
+  // orders q..p-1 of the slots are zero
+#pragma unroll
+  for (int r = 0; r < NPL; r++) {
+    double *oH = A.slotH + (size_t)(J.slot + r) * P2;
+    double *oF = intra ? A.slotF + (size_t)(J.slot + r) * P2 : nullptr;
+    for (int i = q * q + tid; i < P2; i += XT) {
+      oH[i] = 0.0;
+      if (oF) oF[i] = 0.0;
+    }
+  }
+}
+
 // XT threads per workgroup: 64 (one wavefront) for q <= 8, 256 for q <= 16, 512 above
 // (each thread owns at most NO of the q(q+1)/2 back-rotated outputs)
 // one job: batch position e, plane offset pl (split launches)
@@ -244,10 +448,14 @@ __device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e,
   const int NR = 2 * q - 1;  // rotation rows n < 2q-1
   const int tq = J.tord ? J.tord : pH;  // layout order of the translation matrices
   if (A.work && tid == 0) {
-    double w = work_rotgen(q) + NPL * (2 * work_rotapply(pH) + work_transapply(pH));
+    double w = (J.rot ? 0.0 : work_rotgen(q)) + NPL * (2 * work_rotapply(pH) + work_transapply(pH));
     if (intra) w += NPL * (2 * work_rotapply(pF) + work_transapply(pF));
     atomicAdd(&A.work[0], (unsigned long long)w);
     atomicAdd(&A.work[1], (unsigned long long)(8 * NPL * (pH * pH + pF * pF + (intra ? 2 : 1) * P2)));
+  }
+  if (J.rot) {  // stored rotation coefficients (uniform across the workgroup)
+    xapply_job<XT, NPL>(A, J, pH, pF, q, tq);
+    return;
   }
   // LDS carve (the host's xform_lds): R, source blocks, level vectors
   const int nfb = A.F ? NPL : 0;

@@ -1661,6 +1661,22 @@ __global__ void __launch_bounds__(256) k_rot_contract(int p, const double *u, co
 }
 // re-expansion jobs of the generator planes: pair k's job with source block 3 gen[k] and
 // output planes at 3 k
+// Rotation jobs of pair directions (CXFormA's rotation, xi = pi): the angles from the pair
+// vectors with the same cart_to_sph as k_xform, so that k_rotcoeff stores the coefficients
+// k_xform would generate for the pair.
+__global__ void __launch_bounds__(256) k_pairrot_jobs(int n, const double *P, long long RS, RJob *jobs) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  double rho, th, ph;
+  cart_to_sph(P[3 * j], P[3 * j + 1], P[3 * j + 2], rho, th, ph);
+  RJob R{};
+  R.theta = th;
+  R.phi = ph;
+  R.xi = M_PI;
+  R.off = (long long)j * RS;
+  jobs[j] = R;
+}
+
 __global__ void __launch_bounds__(256) k_rot_genjobs(int n, const XJob *xj, const int *gen, XJob *xj2) {
   const int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= n) return;
