@@ -1640,11 +1640,13 @@ static void tune_select(pbsam_system *S) {
   c->tune = (it != c->tuned.end()) ? it->second : Tune{};
 }
 
+static void moltype_rot_build(pbsam_moltype *m);
 static int system_lists(pbsam_system *S) {
   pbsam_ctx *c = S->c;
   hipStream_t s = c->stream;
   tune_select(S);
   const int nmol = S->nmol, p = S->p, P2 = p * p;
+  for (MolInst &M : S->mols) moltype_rot_build(M.mt);
   system_reset_state(S);
   S->solved = false;
   S->grad_ok = false;
@@ -1746,6 +1748,7 @@ static int system_lists(pbsam_system *S) {
           const int idx = mt->ana[ki][e];  // shares the molecule type's translation matrices
           J.TH = mt->d_T.p + (size_t)(2 * idx) * TS;
           J.TF = mt->d_T.p + (size_t)(2 * idx + 1) * TS;
+          J.rot = mt->rot_built ? mt->rpair[(size_t)ki * mt->ns + kj] : nullptr;
           S->ent[g].push_back({Mi.base + kj, (int)xa.size(), -1});
           xa.push_back(J);
         } else {

@@ -952,8 +952,17 @@ __global__ void __launch_bounds__(256) k_p2lpsum(PArgs A) {
   const int c = blockIdx.y * 256 + threadIdx.x;
   if (c >= nset * P2) return;
   const double *src = A.part + (size_t)jb * A.nsplit * 2 * P2 + c;
+  const size_t st = 2 * (size_t)P2;
   double acc = 0.0;
-  for (int k = 0; k < A.nsplit; k++) acc += src[(size_t)k * 2 * P2];
+  int k = 0;
+  for (; k + 8 <= A.nsplit; k += 8) {  // eight loads in flight, added in range order
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = src[(k + u) * st];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc += v[u];
+  }
+  for (; k < A.nsplit; k++) acc += src[k * st];
   if (c < P2) A.out1[(size_t)J.slot * P2 + c] = acc;
   else A.out2[(size_t)J.slot * P2 + c - P2] = acc;
 }
@@ -1753,9 +1762,24 @@ __global__ void __launch_bounds__(64) k_surface_totals(const int *sph, int n, in
   const int e = blockIdx.x * 64 + threadIdx.x;
   if (e >= n) return;
   double f = 0.0, h = 0.0;
-  for (int c = 0; c < nchunk; c++) {
-    f += part[2 * ((size_t)e * nchunk + c)];
-    h += part[2 * ((size_t)e * nchunk + c) + 1];
+  const double *pe = part + 2 * (size_t)e * nchunk;
+  int c = 0;
+  for (; c + 8 <= nchunk; c += 8) {  // eight chunks' loads in flight, added in chunk order
+    double vf[8], vh[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      vf[u] = pe[2 * (c + u)];
+      vh[u] = pe[2 * (c + u) + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      f += vf[u];
+      h += vh[u];
+    }
+  }
+  for (; c < nchunk; c++) {
+    f += pe[2 * c];
+    h += pe[2 * c + 1];
   }
   TQF[sph[e]] = f;
   TQH[sph[e]] = h;
