@@ -56,6 +56,14 @@ const char *pbsam_last_error(const pbsam_ctx *ctx);
  * or maps are built after the call.  pbsam_ctx_stored reports the bytes stored and the cap. */
 int pbsam_ctx_set_store_limit(pbsam_ctx *ctx, long long bytes);
 int pbsam_ctx_stored(const pbsam_ctx *ctx, long long *bytes, long long *limit);
+/* HIP graphs (no counterpart in the reference, whose sweeps are host loops): the gradient
+ * phase's Gauss-Seidel sweep runs in windows of steps (every launch of a window is known
+ * before it starts); a window whose launches recur with the same buffers and sizes is
+ * captured as a HIP graph the second time and replayed after that.  Results are the same
+ * with graphs on (default) or off.  pbsam_system_graph_stats counts a system's graph
+ * launches and captures. */
+int pbsam_ctx_set_graphs(pbsam_ctx *ctx, int on);
+int pbsam_system_graph_stats(const pbsam_system *sys, long long *launches, long long *captures);
 /* Debye kappa (1/A) from a salt concentration in mol/L, as src/selfp_makemat.cpp:59-60 */
 double pbsam_kappa_from_salt(double salt_molar, double eps_out, double temp_k);
 

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -223,6 +224,8 @@ struct pbsam_ctx {
   // (a quarter of the device's memory unless pbsam_ctx_set_store_limit sets another).
   // Stores that would pass the cap are not made; their users recompute per step.
   size_t stored = 0, store_limit = 0;
+  bool graphs = true;      // gradient sweep windows replayed as captured HIP graphs
+  bool capturing = false;  // a stream capture is open (no buffer may grow)
   Tune tune;                                          // settings of the system being run
   std::map<std::array<long long, 3>, Tune> tuned;     // pbsam_autotune results by (p, spheres, systems)
 };
@@ -380,8 +383,17 @@ static size_t p2l_lds(int p) {
   int T = p2l_tile(p);
   return (size_t)8 * ((size_t)T * p * p + 2 * T * p + 3 * T);
 }
+// Raise a kernel's dynamic-LDS limit once per size (the call is not repeated inside a
+// captured graph).
 static void set_lds(const void *k, size_t bytes) {
-  if (bytes > 65536) HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  static std::mutex mu;
+  static std::map<const void *, size_t> done;
+  if (bytes <= 65536) return;
+  std::lock_guard<std::mutex> lock(mu);
+  size_t &d = done[k];
+  if (bytes <= d) return;
+  HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  d = bytes;
 }
 
 // translation matrices at order p, or (TQ given) at each pair's truncation order
@@ -1418,6 +1430,17 @@ struct GradState {
   PinBuf<int> h_sb;                        // step lists staged for upload
   PinBuf<int> h_pairs, h_lists;            // gradient re-expansion batches: pair and chunk lists
   PinBuf<double> h_dev;                    // per-step changes read back
+  // Gradient sweep windows as instantiated HIP graphs, keyed by everything their launches
+  // take (step lists, buffer addresses, settings); a window is captured the second time its
+  // key comes up (the first run sizes every buffer it grows), then replayed.
+  std::map<std::vector<unsigned long long>, hipGraphExec_t> graphs;
+  std::set<std::vector<unsigned long long>> seen;
+  long long graph_launches = 0, graph_captures = 0;
+  GradState() = default;
+  GradState(const GradState &) = delete;
+  ~GradState() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+  }
 };
 
 struct BdState;  // Brownian-dynamics state of a batch (pbsam_host_bd.inc)
@@ -2218,6 +2241,22 @@ const char *pbsam_last_error(const pbsam_ctx *c) { return c ? c->err.c_str() : "
 int pbsam_ctx_set_store_limit(pbsam_ctx *c, long long bytes) {
   if (!c || bytes < 0) return PBSAM_ERR_BADARG;
   c->store_limit = (size_t)bytes;
+  return PBSAM_OK;
+}
+int pbsam_ctx_set_graphs(pbsam_ctx *c, int on) {
+  if (!c) return PBSAM_ERR_BADARG;
+  c->graphs = on != 0;
+  return PBSAM_OK;
+}
+int pbsam_system_graph_stats(const pbsam_system *S, long long *launches, long long *captures) {
+  if (!S) return PBSAM_ERR_BADARG;
+  long long l = S->gr.graph_launches, k = S->gr.graph_captures;
+  for (const auto &P : S->parts) {
+    l += P->gr.graph_launches;
+    k += P->gr.graph_captures;
+  }
+  if (launches) *launches = l;
+  if (captures) *captures = k;
   return PBSAM_OK;
 }
 int pbsam_ctx_stored(const pbsam_ctx *c, long long *bytes, long long *limit) {

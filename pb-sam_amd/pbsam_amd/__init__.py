@@ -42,7 +42,7 @@ SYMBOLS = [
     "pbsam_potential", "pbsam_total_int_energy", "pbsam_system_set_flags", "pbsam_autotune",
     "pbsam_torques_exact",
     "pbsam_xform_h_batch_noalloc", "pbsam_xform_h_batch_noalloc_bytes",
-    "pbsam_ctx_set_store_limit", "pbsam_ctx_stored",
+    "pbsam_ctx_set_store_limit", "pbsam_ctx_stored", "pbsam_ctx_set_graphs", "pbsam_system_graph_stats",
 ]
 POT_REFERENCE_FRAME = 1
 BD_ESCAPED, BD_DOCKED, BD_RUNNING, BD_STUCK, BD_FAILED = range(5)
@@ -92,6 +92,8 @@ def load():
         "pbsam_last_error": ([P], ctypes.c_char_p),
         "pbsam_ctx_set_store_limit": ([P, ctypes.c_longlong], I),
         "pbsam_ctx_stored": ([P, P, P], I),
+        "pbsam_ctx_set_graphs": ([P, I], I),
+        "pbsam_system_graph_stats": ([P, P, P], I),
         "pbsam_kappa_from_salt": ([D, D, D], D),
         "pbsam_moltype_create": ([P, ctypes.POINTER(MolDesc), P], I),
         "pbsam_moltype_destroy": ([P], None),
@@ -270,6 +272,10 @@ class Context:
         """Cap on the optional operator stores (far-pair translation matrices, numerical
         maps) of molecule types set up after the call; 0 makes them all per use."""
         self.check(self.L.pbsam_ctx_set_store_limit(self.h, int(nbytes)))
+
+    def set_graphs(self, on):
+        """Replay recurring gradient sweep windows as captured HIP graphs (default on)."""
+        self.check(self.L.pbsam_ctx_set_graphs(self.h, int(bool(on))))
 
     def stored(self):
         """(bytes of optional operator stores, their cap)"""
@@ -547,6 +553,12 @@ class System:
         self.ctx.check(self.ctx.L.pbsam_batch_status(self.h, st.ctypes.data_as(P), r.ctypes.data_as(P),
                                                      g.ctypes.data_as(P)))
         return st, r, g
+
+    def graph_stats(self):
+        """(graph launches, graph captures) of this system's gradient phases so far"""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        self.ctx.check(self.ctx.L.pbsam_system_graph_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def counts(self):
         a, b = ctypes.c_int(), ctypes.c_int()
