@@ -562,7 +562,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=10.0,
                     help="seconds of oracle force evaluations to time (at least one runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")

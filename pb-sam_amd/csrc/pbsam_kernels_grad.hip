@@ -183,8 +183,7 @@ __global__ void __launch_bounds__(256) k_dta_jobs(int n, const int *pairs, const
   lst[k] = k;
 }
 
-// NT threads per pair; QMAX bounds the order (each thread owns up to two (n, s) outputs,
-// so q (q + 1) / 2 <= 2 NT)
+// NT threads per pair; QMAX bounds the order (q (q + 1) / 2 <= 2 NT)
 template <int NT, int QMAX>
 __global__ void __launch_bounds__(NT) k_xgrad(Tabs tb, GArgs A) {
   extern __shared__ double sh[];
@@ -319,11 +318,13 @@ __global__ void __launch_bounds__(NT) k_xgrad(Tabs tb, GArgs A) {
     dR[n * n + n - n] = dm;
     dR[n * n + n + n] = cconj(dm);
   }
-  // back-rotated accumulators: each thread owns up to two (n, s) outputs
+  // back-rotated accumulators: each thread owns up to KO (n, s) outputs (one where the
+  // workgroup has a thread per output of order QMAX, which keeps the registers down)
+  constexpr int KO = (QMAX * (QMAX + 1) / 2 + NT - 1) / NT;
   const int nout = q * (q + 1) / 2;
-  int on[2], os[2];
-  cd aGr[2], aGt1[2], aGt2[2], aGp1[2], aGp2[2];
-  for (int k = 0; k < 2; k++) {
+  int on[KO], os[KO];
+  cd aGr[KO], aGt1[KO], aGt2[KO], aGp1[KO], aGp2[KO];
+  for (int k = 0; k < KO; k++) {
     int e = tid + k * NT;
     on[k] = -1;
     os[k] = 0;
@@ -400,7 +401,7 @@ __global__ void __launch_bounds__(NT) k_xgrad(Tabs tb, GArgs A) {
     }
     __syncthreads();
     // (c) back-rotation contributions of level m to the owned outputs (n, s), n >= m
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < KO; k++) {
       const int n = on[k], s = os[k];
       if (n < m) continue;
       const cd *row = R + n * n + n, *drow = dR + n * n + n;
@@ -428,7 +429,11 @@ __global__ void __launch_bounds__(NT) k_xgrad(Tabs tb, GArgs A) {
       cd vR[NV], vD[NV];
       int vi[NV];
       int cnt = 0;
-      for (int e = tid; e < total; e += NT, cnt++) {
+#pragma unroll
+      for (int t0 = 0; t0 < NV; t0++) {  // static indices keep the staged values in registers
+        const int e = tid + t0 * NT;
+        if (e >= total) break;
+        cnt = t0 + 1;
         int t = e + base;
         int n = (int)sqrt((double)t);
         while (n * n > t) n--;
@@ -442,24 +447,26 @@ __global__ void __launch_bounds__(NT) k_xgrad(Tabs tb, GArgs A) {
         cd d2 = cscale(r2 * e2, row[s + 1]);
         cd d3 = cscale(r3 * z3, row[s]);
         cd v = cmul(exxi, cadd(cadd(cmul(d1, exiphi), cmul(d2, exphi)), d3));
-        vR[cnt] = {v.x / em, v.y / em};
+        vR[t0] = {v.x / em, v.y / em};
         cd g1 = cscale(e1, cadd(cscale(r1, drow[s - 1]), cscale(dr1, row[s - 1])));
         cd g2 = cscale(e2, cadd(cscale(r2, drow[s + 1]), cscale(dr2, row[s + 1])));
         cd g3 = cscale(z3, cadd(cscale(r3, drow[s]), cscale(dr3, row[s])));
         cd u = cmul(exxi, cadd(cadd(cmul(g1, exiphi), cmul(g2, exphi)), g3));
-        vD[cnt] = {u.x / em, u.y / em};
-        vi[cnt] = (n - 1) * (n - 1) + (n - 1) + s;
+        vD[t0] = {u.x / em, u.y / em};
+        vi[t0] = (n - 1) * (n - 1) + (n - 1) + s;
       }
       __syncthreads();
-      for (int k = 0; k < cnt; k++) {
-        R[vi[k]] = vR[k];
-        dR[vi[k]] = vD[k];
-      }
+#pragma unroll
+      for (int k = 0; k < NV; k++)
+        if (k < cnt) {
+          R[vi[k]] = vR[k];
+          dR[vi[k]] = vD[k];
+        }
     }
     __syncthreads();
   }
   // outputs: Gt = dR^H T R M + R^H T dR M, Gp likewise; G3 = JR (Gr, Gt, Gp)
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < KO; k++) {
     const int n = on[k], s = os[k];
     if (n < 0) continue;
     cd gr = aGr[k], gt = cadd(aGt1[k], aGt2[k]), gp = cadd(aGp1[k], aGp2[k]);
