@@ -218,7 +218,7 @@ struct pbsam_ctx {
   KProf kp[PBSAM_K_COUNT];
   // a second stream (default priority) for independent launch chains, joined by events
   hipStream_t stream0 = nullptr, stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_mid = nullptr;
   // Operators that can be stored or made again per use (far-pair translation matrices of
   // a molecule type, its numerical re-expansion maps): bytes stored so far and the cap
   // (a quarter of the device's memory unless pbsam_ctx_set_store_limit sets another).
@@ -2199,6 +2199,7 @@ int pbsam_ctx_create(double kappa, double eps_out, int device, pbsam_ctx **out) 
     c->stream0 = c->stream;
     HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_mid, hipEventDisableTiming));
     int lds = 0;
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     c->lds_max = (size_t)lds;
@@ -2232,6 +2233,7 @@ void pbsam_ctx_destroy(pbsam_ctx *c) {
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_mid) (void)hipEventDestroy(c->ev_mid);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
