@@ -509,7 +509,7 @@ struct S3Args {
                            // result (k_surfgemm's operand), or null
   // identity-matrix entries as S3_PARTS workgroups (grid y; see solve3_split)
   double *spec;                // [row][F, H][3][p^2] tentative states
-  unsigned long long *mask;    // [row][S3_PARTS][2] certified-sweep masks
+  unsigned long long *mask;    // [row][S3_PARTS][4] certified-sweep masks and outer-change sums
   unsigned int *cnt;           // [row] arrivals, zero between launches
 };
 
@@ -605,13 +605,14 @@ __device__ void solve3_identity(const S3Args &A, int g, int e, int lb, double *r
 // Part workgroup blockIdx.y of an identity-matrix entry: half h = y % S3_HALF of column
 // d = y / S3_HALF's (n, m) pairs, one per thread, through all 100 sweeps of the joint loop
 // without its stopping test; bit s of the part's mask is set when one of its pairs alone
-// shows that sweep s continues (the test of id_sweep_fast).  States go to the row's
-// tentative block.  The last of the S3_PARTS workgroups to arrive (a device-scope counter;
-// nothing waits) finishes the entry:
-// when every sweep 0..98 is certified by some column, the joint loop runs all 100 sweeps and
-// the tentative states are its result, taken through solve3_identity's epilogue; otherwise
-// it runs the joint loop itself from the entry's inputs, which no column workgroup writes.
-// Either way the outputs are the joint loop's, bit for bit, from S3_PARTS CUs instead of one.
+// shows that sweep s continues (the test of id_sweep_fast).  Each part writes its pairs'
+// results (F, H and the scaled surface right-hand sides E, which are the joint loop's when
+// it runs all 100 sweeps), keeps their inputs in the row's backup block, and leaves its
+// share of the outer-change sums.  The last of the S3_PARTS workgroups to arrive (a
+// device-scope counter; nothing waits) finishes the entry: when every sweep 0..98 is
+// certified by some column the joint loop runs all 100 sweeps, so only dev is left (the
+// parts' sums in part order); otherwise it restores the inputs and runs the joint loop
+// itself.  Either way F, H and E are the joint loop's, bit for bit, from S3_PARTS CUs.
 template <int KW>
 __device__ void solve3_split(const S3Args &A, int g, int e, int lb, double *red) {
   const int p = A.p, P2 = p * p, tid = threadIdx.x, lane = tid % 64, w = tid / 64;
@@ -622,10 +623,11 @@ __device__ void solve3_split(const S3Args &A, int g, int e, int lb, double *red)
   const double *cs = A.consts + (size_t)g * 10 * p;
   const double *cLH1 = cs + 2 * p, *cLF1 = cs + 3 * p, *cLH2 = cs + 6 * p, *cLF2 = cs + 7 * p;
   const size_t o = ((size_t)e * 3 + d) * P2, ol = ((size_t)lb * 3 + d) * P2;
-  double *spec = A.spec + (size_t)blockIdx.x * 6 * P2;
+  double *spec = A.spec + (size_t)blockIdx.x * 6 * P2;  // inputs: [F, H][3][p^2]
   IdState<1> S;
   id_clear(S, 0);
   int n = 0, i = 0;
+  double iFr = 0.0, iFi = 0.0;
   if (mine) {
     pair_nm(ep, n, i);
     S.nn[0] = n;
@@ -636,10 +638,12 @@ __device__ void solve3_split(const S3Args &A, int g, int e, int lb, double *red)
     S.hbr[0] = cLH2[n] * A.LH[ol + i] + cLF2[n] * A.LF[ol + i];
     S.fbi[0] = two ? cLH1[n] * A.LH[ol + i + 1] + cLF1[n] * A.LF[ol + i + 1] : 0.0;
     S.hbi[0] = two ? cLH2[n] * A.LH[ol + i + 1] + cLF2[n] * A.LF[ol + i + 1] : 0.0;
-    S.Fr[0] = A.F[o + i];
+    S.Fr[0] = iFr = A.F[o + i];
     S.Hr[0] = A.H[o + i];
-    S.Fi[0] = two ? A.F[o + i + 1] : 0.0;
+    S.Fi[0] = iFi = two ? A.F[o + i + 1] : 0.0;
     S.Hi[0] = two ? A.H[o + i + 1] : 0.0;
+    S.uHr[0] = S.Hr[0];
+    S.uHi[0] = S.Hi[0];
     const double cthr = 4.0 * p * p * 1e-8 * (1.0 + 1e-6);  // as id_iterate
     S.c2[0] = two ? 0.5 * cthr : cthr;
   }
@@ -650,51 +654,99 @@ __device__ void solve3_split(const S3Args &A, int g, int e, int lb, double *red)
     for (int t = 64; t < 100; t++)
       if (__ballot(id_sweep_fast(S))) m1 |= 1ull << (t - 64);
   }
+  double td = 0.0, ts = 0.0;
   if (mine) {
-    spec[(size_t)d * P2 + i] = S.Fr[0];
-    spec[(size_t)(3 + d) * P2 + i] = S.Hr[0];
-    if (i != n * n) {
-      spec[(size_t)d * P2 + i + 1] = S.Fi[0];
-      spec[(size_t)(3 + d) * P2 + i + 1] = S.Hi[0];
+    const bool two = (i != n * n);
+    spec[(size_t)d * P2 + i] = iFr;  // the inputs, for a rerun of the joint loop
+    spec[(size_t)(3 + d) * P2 + i] = S.uHr[0];
+    if (two) {
+      spec[(size_t)d * P2 + i + 1] = iFi;
+      spec[(size_t)(3 + d) * P2 + i + 1] = S.uHi[0];
     }
+    if (!two) {  // as the joint loop's epilogue: m = 0 pairs keep zero imaginary parts
+      S.Fi[0] = 0.0;
+      S.Hi[0] = 0.0;
+    }
+    A.F[o + i] = S.Fr[0];
+    A.H[o + i] = S.Hr[0];
+    if (two) {
+      A.F[o + i + 1] = S.Fi[0];
+      A.H[o + i + 1] = S.Hi[0];
+    }
+    if (A.E) {  // w (2n+1)/4pi F and w cChgH[n] H, w = 2 for m > 0
+      const double tw = two ? 2.0 : 1.0, CF = (2 * n + 1) / (4.0 * M_PI), CH = cs[9 * p + n];
+      double *Eo = A.E + ((size_t)lb * P2 + i) * 6;
+      Eo[d] = tw * (CF * S.Fr[0]);
+      Eo[3 + d] = tw * (CH * S.Hr[0]);
+      if (two) {
+        Eo[6 + d] = tw * (CF * S.Fi[0]);
+        Eo[6 + 3 + d] = tw * (CH * S.Hi[0]);
+      }
+    }
+    id_dev2(S, td, ts);
   }
+  td = wave_sum(td);
+  ts = wave_sum(ts);
   unsigned long long *wm = reinterpret_cast<unsigned long long *>(red);
-  int *flag = reinterpret_cast<int *>(wm + 2 * (S3T / 64));
+  double *wd = red + 2 * (S3T / 64);
+  int *flag = reinterpret_cast<int *>(wd + 2 * (S3T / 64));
   if (lane == 0) {
     wm[2 * w] = m0;
     wm[2 * w + 1] = m1;
+    wd[2 * w] = td;
+    wd[2 * w + 1] = ts;
   }
   __syncthreads();
-  unsigned long long *mk = A.mask + (size_t)blockIdx.x * 2 * S3_PARTS;
+  unsigned long long *mk = A.mask + (size_t)blockIdx.x * 4 * S3_PARTS;  // [part][m0, m1, td, ts]
   if (tid == 0) {
     unsigned long long a0 = 0, a1 = 0;
+    double sd = 0.0, ss = 0.0;
     for (int x = 0; x < S3T / 64; x++) {
       a0 |= wm[2 * x];
       a1 |= wm[2 * x + 1];
+      sd += wd[2 * x];
+      ss += wd[2 * x + 1];
     }
-    mk[2 * blockIdx.y] = a0;
-    mk[2 * blockIdx.y + 1] = a1;
+    mk[4 * blockIdx.y] = a0;
+    mk[4 * blockIdx.y + 1] = a1;
+    mk[4 * blockIdx.y + 2] = (unsigned long long)__double_as_longlong(sd);
+    mk[4 * blockIdx.y + 3] = (unsigned long long)__double_as_longlong(ss);
   }
-  __threadfence();  // this workgroup's states and mask, device-wide, before it arrives
+  __threadfence();  // this workgroup's states, backups and sums, device-wide, before it arrives
   __syncthreads();
   if (tid == 0) flag[0] = (atomicAdd(&A.cnt[blockIdx.x], 1u) == (unsigned)(S3_PARTS - 1));
   __syncthreads();
   if (!flag[0]) return;  // uniform
-  __threadfence();       // the other columns' states and masks
+  __threadfence();       // the other parts' states, backups and sums
   if (tid == 0) {
     (void)atomicExch(&A.cnt[blockIdx.x], 0u);  // for the next launch
     unsigned long long a0 = 0, a1 = 0;
+    double TD = 0.0, TS = 0.0;
     for (int x = 0; x < S3_PARTS; x++) {
-      a0 |= mk[2 * x];
-      a1 |= mk[2 * x + 1];
+      a0 |= mk[4 * x];
+      a1 |= mk[4 * x + 1];
+      TD += __longlong_as_double((long long)mk[4 * x + 2]);
+      TS += __longlong_as_double((long long)mk[4 * x + 3]);
     }
     const unsigned long long need1 = (1ull << 35) - 1;  // sweeps 64..98 (sweep 99 ends the loop)
     flag[1] = (a0 == ~0ull) && ((a1 & need1) == need1);
+    if (flag[1]) {
+      A.dev[e] = TD / (4 * TS) / double(3);
+      if (A.work) {
+        atomicAdd(&A.work[0], (unsigned long long)(100 * 3 * 12.0 * P2));
+        atomicAdd(&A.work[1], (unsigned long long)(8.0 * 3 * 6 * P2));
+      }
+    }
   }
   __syncthreads();
-  const bool ok = flag[1] != 0;
+  if (flag[1]) return;  // uniform
+  // the loop stops before sweep 100: restore the inputs and run it jointly
+  for (int x = tid; x < 3 * P2; x += S3T) {
+    A.F[(size_t)e * 3 * P2 + x] = spec[x];
+    A.H[(size_t)e * 3 * P2 + x] = spec[3 * P2 + x];
+  }
   __syncthreads();  // red is the joint loop's from here
-  solve3_identity<KW>(A, g, e, lb, red, ok ? spec : nullptr);
+  solve3_identity<KW>(A, g, e, lb, red, nullptr);
 }
 
 __global__ void __launch_bounds__(S3T) k_solve3(S3Args A) {
