@@ -649,10 +649,14 @@ __device__ void solve3_split(const S3Args &A, int g, int e, int lb, double *red)
   }
   unsigned long long m0 = 0, m1 = 0;  // sweeps 0..63, 64..99
   if (w * 64 < per) {                 // wavefronts with pairs (uniform per wavefront)
-    for (int t = 0; t < 64; t++)
-      if (__ballot(id_sweep_fast(S))) m0 |= 1ull << t;
-    for (int t = 64; t < 100; t++)
-      if (__ballot(id_sweep_fast(S))) m1 |= 1ull << (t - 64);
+    // each lane records its own certificates; one OR across the wavefront afterwards (no
+    // cross-lane step inside the sweep chain)
+    for (int t = 0; t < 64; t++) m0 |= (unsigned long long)id_sweep_fast(S) << t;
+    for (int t = 64; t < 100; t++) m1 |= (unsigned long long)id_sweep_fast(S) << (t - 64);
+    for (int off = 1; off < 64; off <<= 1) {
+      m0 |= (unsigned long long)__shfl_xor((long long)m0, off, 64);
+      m1 |= (unsigned long long)__shfl_xor((long long)m1, off, 64);
+    }
   }
   double td = 0.0, ts = 0.0;
   if (mine) {
