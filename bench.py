@@ -387,11 +387,16 @@ def large_leg(pbsam_amd, synth, ctx, nsph, solves, evals):
     _, _, f, t = S.compute_forces(tol=0, max_rounds=0)  # first evaluation: stored operators built
     ctx.sync()
     t_first_eval = time.perf_counter() - t0
+    mark = roctx()
+    if mark:
+        mark.roctxRangePushA(b"large_region")
     t0 = time.perf_counter()
     for _ in range(evals):
         _, _, f, t = S.compute_forces(tol=0, max_rounds=0)
     ctx.sync()
     t_eval = (time.perf_counter() - t0) / max(evals, 1)
+    if mark:
+        mark.roctxRangePop()
     st, passes = S.status()[0], S.status()[2]
     stored, limit = ctx.stored()
     free, total = device_memory()
@@ -649,12 +654,16 @@ def main():
     # reference's order; the Gauss-Seidel schedule converges slowly, so a few hundred rounds
     tight = None
     if args.tight_solves > 0:
+        if mark:
+            mark.roctxRangePushA(b"tight_region")
         t0 = time.perf_counter()
         for _ in range(args.tight_solves):
             trounds, trc = system.solve(tol=1e-8, max_rounds=1000)
         ctx.sync()
         tight = dict(solves_per_s=args.tight_solves / (time.perf_counter() - t0), tol=1e-8, mutual_rounds=trounds,
                      status=trc)
+        if mark:
+            mark.roctxRangePop()
 
     dom = max(prof, key=lambda k: prof[k]["ms"])
     d = prof[dom]

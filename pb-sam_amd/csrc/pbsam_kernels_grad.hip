@@ -1566,8 +1566,9 @@ __global__ void __launch_bounds__(256) k_surfgemm(SGArgs A) {
 // are added in order, so each wavefront's dependent chain is a quarter as long.
 constexpr int SG1_ROWS = 16;
 // grid (16-point tile, entry pair of the bundle, bundle); 256 threads
-__global__ void __launch_bounds__(256) k_surfgemm_k4(SGArgs A) {
-  __shared__ double Cp[4][16 * 16];
+template <int NW>  // wavefronts per workgroup, each a K range of the product
+__global__ void __launch_bounds__(64 * NW) k_surfgemm_k4(SGArgs A) {
+  __shared__ double Cp[NW][16 * 16];
   __shared__ int ent_s[2];
   __shared__ double dA_s[2];
   const int p = A.p, P2 = p * p, tid = threadIdx.x, w = tid / 64, lane = tid % 64;
@@ -1596,7 +1597,7 @@ __global__ void __launch_bounds__(256) k_surfgemm_k4(SGArgs A) {
   const bool cok = es < nent;
   const double *eb = A.E + ((size_t)A.pbuf[ent_s[cok ? es : 0]] * P2 + kq) * 6 + col % 6;
   // this wavefront's K range, in steps of 4
-  const int ksteps = (P2 + 3) / 4, kper = (ksteps + 3) / 4;
+  const int ksteps = (P2 + 3) / 4, kper = (ksteps + NW - 1) / NW;
   const int k0 = 4 * w * kper, k1 = min(P2, 4 * (w + 1) * kper);
   v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
   // UNR steps of 4 per round trip, all loads issued before the products; steps past the
@@ -1629,8 +1630,13 @@ __global__ void __launch_bounds__(256) k_surfgemm_k4(SGArgs A) {
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         const int cf = lane * 16 + 6 * t + d, ch = cf + 3;
-        const double gf = (((Cp[0][cf] + Cp[1][cf]) + Cp[2][cf]) + Cp[3][cf]) * dA;
-        const double gh = (((Cp[0][ch] + Cp[1][ch]) + Cp[2][ch]) + Cp[3][ch]) * dA;
+        double sf = Cp[0][cf], shv = Cp[0][ch];
+#pragma unroll
+        for (int x = 1; x < NW; x++) {  // the K ranges' partial products in order
+          sf += Cp[x][cf];
+          shv += Cp[x][ch];
+        }
+        const double gf = sf * dA, gh = shv * dA;
         A.gSF[(size_t)d * A.npts_all + pt] = gf;
         A.gSH[(size_t)d * A.npts_all + pt] = gh;
         q[d] = fabs(gf);
