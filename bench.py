@@ -568,6 +568,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--autotune", type=int, default=2, metavar="REPS",
+                    help="tune the evaluation's launch shapes first (pbsam_autotune, REPS timed calls per setting; "
+                         "0: the defaults)")
     ap.add_argument("--cpu-budget", type=float, default=10.0,
                     help="seconds of oracle force evaluations to time (at least one runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
@@ -608,6 +611,12 @@ def main():
     t_setup = time.perf_counter() - t_setup
     print("set-up (self-polarization of both molecule types) %.1f s" % t_setup, file=sys.stderr, flush=True)
 
+    tuned = None
+    if args.autotune > 0:  # set-up: launch shapes for this system's shape (pbsam_autotune)
+        t_tune = time.perf_counter()
+        tms, tuned = system.autotune(reps=args.autotune)
+        print("autotuned launch shapes in %.1f s (%.1f ms per evaluation): %s" % (time.perf_counter() - t_tune, tms,
+                                                                                  tuned), file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         system.compute_forces(tol=0, max_rounds=0)
     ctx.sync()
@@ -764,7 +773,7 @@ def main():
                         "2 molecules x 60 spheres, p=30, 1800 points/sphere, centres 50 A apart, random "
                         "orientations, reference Gauss-Seidel schedule",
             "molecules": 2, "spheres_per_molecule": N_SPH, "p": P, "points_per_sphere": NPTS,
-            "separation_A": DIST, "inter_rcutoff_A": 100.0, "mutual_rounds": rounds,
+            "separation_A": DIST, "inter_rcutoff_A": 100.0, "mutual_rounds": rounds, "launch_shapes": tuned,
         },
         "solves_per_s": solves_per_s,
         "tight_solve": tight,

@@ -239,13 +239,15 @@ int pbsam_system_get_grad(const pbsam_system *sys, int mol, int k, double *gH, d
  * evaluations of this system (pbsam_compute_forces, stock tolerances, reps calls each) while
  * it varies, one at a time, the settings that only change how the work is split over
  * workgroups and launches (the sweep look-ahead window, the multi-plane split threshold,
- * the order-class binning threshold, the point split of the stored-map products and the
- * bundle size from which they run on matrix cores), and keeps the fastest.  The choice is
- * used by this system and by every later system of the context with the same pole order,
- * sphere count and number of systems.  Results stay within rounding of the untuned ones
- * (the point split changes a summation order).  ms_per_eval: the best time per evaluation
- * (summed over the parts of a mixed-order batch); settings (5 ints, may be NULL): window,
- * split threshold, binning threshold, GEMV workgroup target, matrix-core bundle minimum. */
+ * the order-class binning threshold, the point split of the stored-map products, the
+ * bundle size from which they run on matrix cores, the point split of the numerical
+ * re-expansions and the lanes per point of the surface charges), and keeps the fastest.  The
+ * choice is used by this system and by every later system of the context with the same pole
+ * order, sphere count and number of systems.  Results stay within rounding of the untuned
+ * ones (the point splits change summation orders).  ms_per_eval: the best time per
+ * evaluation (summed over the parts of a mixed-order batch); settings (7 ints, may be NULL):
+ * window, split threshold, binning threshold, GEMV workgroup target, matrix-core bundle
+ * minimum, point-to-expansion workgroup target, surface-charge workgroup target. */
 int pbsam_autotune(pbsam_system *sys, int reps, double *ms_per_eval, int *settings);
 
 /* ---- energy and potential queries ----

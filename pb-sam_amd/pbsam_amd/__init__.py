@@ -464,9 +464,9 @@ class System:
 
     def autotune(self, reps=1):
         """pbsam_autotune: (best ms per evaluation, dict of the chosen settings)"""
-        ms, st = ctypes.c_double(), np.zeros(5, dtype=np.int32)
+        ms, st = ctypes.c_double(), np.zeros(7, dtype=np.int32)
         self.ctx.check(self.ctx.L.pbsam_autotune(self.h, reps, ctypes.byref(ms), st.ctypes.data_as(P)))
-        keys = ("grad_window", "xsplit_max", "xbin_min", "mgemv_wg", "pgemm_min")
+        keys = ("grad_window", "xsplit_max", "xbin_min", "mgemv_wg", "pgemm_min", "p2l_wg", "surf_wg")
         return ms.value, dict(zip(keys, (int(v) for v in st)))
 
     def set_pose(self, i, pos, quat):
