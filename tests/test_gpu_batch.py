@@ -227,7 +227,8 @@ def test_autotune_keeps_results(gpu, orc):
     assert rc == 0
     b1 = batch()
     ms, settings = b1.autotune(reps=1)
-    assert ms > 0 and set(settings) == {"grad_window", "xsplit_max", "xbin_min", "mgemv_wg", "pgemm_min"}
+    assert ms > 0 and set(settings) == {"grad_window", "xsplit_max", "xbin_min", "mgemv_wg", "pgemm_min",
+                                        "p2l_wg", "surf_wg"}
     _, rc, f1, t1 = b1.compute_forces()
     assert rc == 0 and relerr(f1, f0) < 1e-12 and relerr(t1, t0) < 1e-12
     _, rc, f2, _ = batch().compute_forces()   # same shape: the tuned settings
