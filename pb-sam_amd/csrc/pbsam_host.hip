@@ -1445,6 +1445,16 @@ struct GradState {
 
 struct BdState;  // Brownian-dynamics state of a batch (pbsam_host_bd.inc)
 
+// Buffers of the adjoint solve (pbsam_adjoint_forces)
+struct AdjState {
+  DBuf<double> aF, aH, wF, wH, zeta, rot, inter;  // [sphere][p^2]: adjoint blocks, w = M^T a, R w, sums
+  DBuf<double> qTF, qTH, lqF, lqH, sT;            // numerical part: per job point, per point, harmonic sums
+  DBuf<double> src, out, dev, md;                 // contraction: source planes, sums; sweep changes
+  DBuf<int> ptgt, joff, jlist, tsph, town, ext;
+  DBuf<long long> qoff;
+  DBuf<PJob> sjobs;
+};
+
 struct pbsam_system {
   pbsam_ctx *c;
   int p, nmol, nsph = 0;
@@ -1488,6 +1498,7 @@ struct pbsam_system {
   DBuf<int> d_iota, d_selx, d_selp, d_seli, d_selk, d_gsbuf;
   double itot = 0.0;
   GradState gr;
+  AdjState adj;
   bool solved = false, grad_ok = false;
   bool grad_rot = false;  // the gradient blocks hold the rotational derivative (pbsam_torques_exact)
   // independent systems solved together: groups of consecutive molecules that interact
@@ -3116,3 +3127,4 @@ int pbsam_rotate_quat_batch(pbsam_ctx *c, int n, int p, const double *quat, int 
 #include "pbsam_host_energy.inc"
 #include "pbsam_host_io.inc"
 #include "pbsam_host_noalloc.inc"
+#include "pbsam_host_adjoint.inc"

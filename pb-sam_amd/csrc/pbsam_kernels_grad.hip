@@ -1769,3 +1769,4 @@ __global__ void __launch_bounds__(256) k_rotgen_sub(int p, const int *blk, const
 }  // namespace pbsam
 #include "pbsam_kernels_imat.hip"
 #include "pbsam_kernels_pot.hip"
+#include "pbsam_kernels_adjoint.hip"

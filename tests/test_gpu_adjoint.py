@@ -1,0 +1,166 @@
+"""Forces and torques from the adjoint of the converged fixed point (pbsam_adjoint_forces).
+
+Under PBSAM_EXACT_DERIVATIVE the adjoint solve gives -dE_i/dr_i and -dE_i/domega_i of every
+molecule's energy sum_k <H_k, LS_k>.  Checked against: the forward-mode gradient phase in the
+same mode (orientation * pbsam_forces, and pbsam_torques_exact), which the oracle pins
+(tests/test_gpu_exact.py, tests/test_gpu_torque.py); central differences of the device energy
+under 1e-4 A moves and 1e-4 rad turns; the Debye-Hueckel closed forms of two charged spheres;
+the same systems solved alone against a batch; and torch.autograd.gradcheck through
+pbsam_amd.autograd.RigidEnergy."""
+import numpy as np
+import pytest
+
+from conftest import relerr
+from test_gpu_exact import EPS_OUT, conj, pair, quat_rot
+
+pytestmark = pytest.mark.gpu
+
+ADJ_TOL = 1e-22
+
+
+def system(gpu, gs, pos, quat, tol=1e-18):
+    import pbsam_amd
+    S = pbsam_amd.System(gpu, gs, pos, quat)
+    S.set_cutoffs(100.0, 100.0)
+    S.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+    rounds, rc = S.solve(tol=tol, max_rounds=4000)
+    assert rc == 0, rounds
+    return S
+
+
+def turned(q, a, h):
+    from test_gpu_torque import turned as t
+    return t(q, a, h)
+
+
+@pytest.mark.parametrize("ns,p,npts", [(4, 10, 400), (6, 10, 400), (3, 30, 1200)])
+def test_adjoint_equals_forward_mode(gpu, orc, ns, p, npts):
+    """every polarized pair of these molecules is within the intra cutoff, so the forward
+    sweeps and the adjoint linearize the same map"""
+    gs, _, pos, quat = pair(gpu, orc, ns, p, npts)
+    S = system(gpu, gs, pos, quat)
+    fa, ta, sweeps = S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+    print("adjoint sweeps", sweeps, flush=True)
+    S.gradient(tol=1e-20, max_rounds=4000)
+    f, _ = S.forces()
+    ff = np.array([quat_rot(quat[i], f[i]) for i in range(2)])
+    tf = S.torques_exact(tol=1e-20, max_rounds=4000)
+    assert relerr(fa, ff) < 1e-8, (fa, ff)
+    assert relerr(ta, tf) < 1e-8, (ta, tf)
+    # Newton's third law holds only approximately for the per-molecule energies: not tested
+
+
+@pytest.mark.parametrize("ns,p,npts", [(5, 10, 400), (3, 30, 1200)])
+def test_adjoint_is_the_energy_gradient(gpu, orc, ns, p, npts):
+    gs, _, pos, quat = pair(gpu, orc, ns, p, npts, seeds=(7, 8))
+    S = system(gpu, gs, pos, quat)
+    fa, ta, _ = S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+    h = 1e-4
+    for mol in range(2):
+        fd, td = np.zeros(3), np.zeros(3)
+        for d in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[mol, d] += h
+            pm[mol, d] -= h
+            ep = system(gpu, gs, pp, quat).energy()[mol]
+            em = system(gpu, gs, pm, quat).energy()[mol]
+            fd[d] = -(ep - em) / (2 * h)
+            qp, qm = quat.copy(), quat.copy()
+            qp[mol] = turned(quat[mol], d, h)
+            qm[mol] = turned(quat[mol], d, -h)
+            ep = system(gpu, gs, pos, qp).energy()[mol]
+            em = system(gpu, gs, pos, qm).energy()[mol]
+            td[d] = -(ep - em) / (2 * h)
+        assert np.abs(fa[mol] - fd).max() < 1e-6 * np.linalg.norm(fd), (mol, fa[mol], fd)
+        assert np.abs(ta[mol] - td).max() < 1e-6 * np.linalg.norm(td), (mol, ta[mol], td)
+
+
+@pytest.mark.parametrize("d,tol", [(40.0, 1e-3), (80.0, 1e-4)])
+def test_adjoint_debye_hueckel(gpu, d, tol):
+    """two one-sphere molecules (radii 3 and 2 A, +1 and -1 at the centres, the first sphere
+    off its molecule centre): force -f1 on molecule 0, torque (R off) x (-f1)"""
+    import pbsam_amd
+    from test_gpu_exact import dh_pair
+    q0 = np.array([0.9, 0.3, -0.2, 0.1])
+    q0 /= np.linalg.norm(q0)
+    off = (1.0, -0.5, 0.7)
+    _, _, f1, offs, quat, mts = dh_pair(gpu, d, pbsam_amd.EXACT_DERIVATIVE, q0, (1.0, 0.0, 0.0, 0.0), off)
+    c0, c1 = np.zeros(3), np.array([d, 0.0, 0.0])
+    pos = np.array([c0 - quat_rot(quat[0], offs[0]), c1])
+    S = pbsam_amd.System(gpu, mts, pos, quat)
+    S.set_cutoffs(1e4, 1e4)
+    S.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+    S.solve(tol=1e-24, max_rounds=400)
+    fa, ta, _ = S.adjoint_forces(tol=ADJ_TOL, max_rounds=2000)
+    assert np.abs(fa[0] + f1).max() < tol * np.linalg.norm(f1)
+    assert np.abs(fa[1] - f1).max() < tol * np.linalg.norm(f1)
+    tau = np.cross(quat_rot(quat[0], offs[0]), -f1)
+    assert np.abs(ta[0] - tau).max() < tol * np.linalg.norm(f1) * np.linalg.norm(offs[0])
+
+
+def test_adjoint_batch_equals_single_systems(gpu, orc):
+    import pbsam_amd
+    from pbsam_amd import synth
+    gs, _, _, _ = pair(gpu, orc, 4, 10, 400)
+    d = 2 * synth.ball_radius(4) + 6.0
+    mols = [synth.molecule(4, seed=s) for s in (4, 5)]
+    poses = [synth.place_pair(mols[0], mols[1], d + b, seed=30 + b) for b in range(3)]
+    alone = []
+    for pos, quat in poses:
+        S = system(gpu, gs, pos, quat)
+        fa, ta, _ = S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+        alone.append((fa, ta))
+    pos = np.vstack([pp for pp, _ in poses])
+    quat = np.vstack([qq for _, qq in poses])
+    B = pbsam_amd.System(gpu, gs * 3, pos, quat, groups=[2] * 3)
+    B.set_cutoffs(100.0, 100.0)
+    B.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+    B.solve(tol=1e-18, max_rounds=4000)
+    fb, tb, _ = B.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+    for g, (fa, ta) in enumerate(alone):
+        assert relerr(fb[2 * g:2 * g + 2], fa) < 1e-9
+        assert relerr(tb[2 * g:2 * g + 2], ta) < 1e-9
+
+
+def test_adjoint_needs_the_exact_mode(gpu, orc):
+    import pbsam_amd
+    gs, _, pos, quat = pair(gpu, orc, 3, 10, 400)
+    S = pbsam_amd.System(gpu, gs, pos, quat)
+    S.set_cutoffs(100.0, 100.0)
+    S.solve(tol=1e-12, max_rounds=400)
+    with pytest.raises(pbsam_amd.PbsamError) as e:
+        S.adjoint_forces()
+    assert e.value.code == pbsam_amd.ERR_STATE
+    S3 = pbsam_amd.System(gpu, gs + gs[:1], np.vstack([pos, pos[:1] + 200.0]), np.vstack([quat, quat[:1]]))
+    S3.set_cutoffs(100.0, 100.0)
+    S3.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+    S3.solve(tol=1e-12, max_rounds=400)
+    with pytest.raises(pbsam_amd.PbsamError) as e:
+        S3.adjoint_forces()
+    assert e.value.code == pbsam_amd.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("ns,p,npts", [(4, 10, 400), (3, 30, 1200)])
+def test_gradcheck_rigid_energy(gpu, ns, p, npts):
+    import pbsam_amd
+    pbsam_amd.load()   # the library's HIP runtime first, then torch
+    import torch
+    from pbsam_amd import autograd, synth
+    mols = [synth.molecule(ns, seed=s) for s in (24, 25)]
+    mts = []
+    for m in mols:
+        mt = pbsam_amd.MolType(gpu, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], p, eps_in=4.0, rcen=m["rcen"],
+                               npts=[npts] * ns)
+        mt.selfpol(tol=0, max_rounds=0)
+        mt.values()
+        mts.append(mt)
+    pos, quat = synth.place_pair(mols[0], mols[1], 2 * synth.ball_radius(ns) + 5.0, seed=22)
+    for j in range(2):
+        ev = autograd.Evaluator(gpu, mts, pos, quat, j, tol=1e-18, max_rounds=4000, adj_tol=ADJ_TOL,
+                                adj_sweeps=20000)
+        x = torch.zeros(6, dtype=torch.float64, requires_grad=True)
+        _, g = autograd.solve_rigid(ev, np.zeros(6), True)
+        scale = float(np.linalg.norm(g))
+        assert scale > 0
+        assert torch.autograd.gradcheck(lambda y: autograd.RigidEnergy.apply(y, ev), (x,), eps=1e-4,
+                                        atol=1e-6 * scale, rtol=1e-6)
