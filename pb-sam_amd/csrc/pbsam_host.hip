@@ -1446,10 +1446,12 @@ struct GradState {
 struct BdState;  // Brownian-dynamics state of a batch (pbsam_host_bd.inc)
 
 // Buffers of the adjoint solve (pbsam_adjoint_forces)
+constexpr int KRYLOV_RESTART = 30;  // GMRES basis size before a restart
 struct AdjState {
   DBuf<double> aF, aH, wF, wH, zeta, rot, inter;  // [sphere][p^2]: adjoint blocks, w = M^T a, R w, sums
   DBuf<double> qTF, qTH, lqF, lqH, sT;            // numerical part: per job point, per point, harmonic sums
-  DBuf<double> src, out, dev, md;                 // contraction: source planes, sums; sweep changes
+  DBuf<double> src, out;                          // contraction: source planes, sums
+  DBuf<double> V, w, x, b, part, h;               // Krylov basis, work vector, solution, right-hand side
   DBuf<int> ptgt, joff, jlist, tsph, town, ext;
   DBuf<long long> qoff;
   DBuf<PJob> sjobs;

@@ -161,34 +161,31 @@ __global__ void __launch_bounds__(256) k_adj_qsum(const int *pol, const int *jof
   lqH[pt_off[m] + h] = q;
 }
 
-// One adjoint Jacobi sweep's new blocks of each listed sphere (one workgroup each):
-//   aH = LS + (inter part) + (analytic intra slots) + dA cChgH_n sT_H
-//   aF =                     (analytic intra slots) + dA (2n+1)/(4 pi) sT_F
+// The adjoint operator applied to x, as y = x - A~ x, for each listed sphere (one workgroup
+// each), x and y compact [entry][F, H][p^2]:
+//   (A~ x)_H = (inter part) + (analytic intra slots) + dA cChgH_n sT_H
+//   (A~ x)_F =                (analytic intra slots) + dA (2n+1)/(4 pi) sT_F
 // (sT: the harmonic sums of the adjoint surface charges, computeExposedSurfaceChargesFH
-// expcenter.cpp:1420-1440 transposed), and dev = |change|^2 / |new|^2 over both blocks.
+// expcenter.cpp:1420-1440 transposed).
 struct CArgs {
   const int *pol;
   const int *lst_off, *lst;   // analytic and numerical intra slots of each listed sphere
   const double *slotH, *slotF;
-  const double *rhs;          // [sphere][p^2] right-hand side of the H block
   const double *inter;        // [sphere][p^2] inter-molecular part
   const double *sTF, *sTH;    // [list entry][p^2]
   const double *consts;
   const int *nspx;
-  double *aF, *aH;            // [sphere][p^2] in/out
-  double *dev;                // [sphere]
+  const double *x;            // [entry][2][p^2]
+  double *y;                  // [entry][2][p^2]
   int p;
 };
-__global__ void __launch_bounds__(256) k_adj_combine(CArgs A) {
-  __shared__ double red[4];
+__global__ void __launch_bounds__(256) k_adj_apply(CArgs A) {
   const int e = blockIdx.x, g = A.pol[e], p = A.p, P2 = p * p;
   const double dA = 4 * M_PI / double(A.nspx[g]);
   const double *cChgH = A.consts + (size_t)g * 10 * p + 9 * p;
-  double d2 = 0.0, n2 = 0.0;
   for (int c = threadIdx.x; c < P2; c += 256) {
     const int n = coef_level(c);
-    const size_t i = (size_t)g * P2 + c;
-    double h = A.rhs[i] + A.inter[i], f = 0.0;
+    double h = A.inter[(size_t)g * P2 + c], f = 0.0;
     for (int k = A.lst_off[e]; k < A.lst_off[e + 1]; k++) {
       const size_t o = (size_t)A.lst[k] * P2 + c;
       h += A.slotH[o];
@@ -196,15 +193,61 @@ __global__ void __launch_bounds__(256) k_adj_combine(CArgs A) {
     }
     h += dA * cChgH[n] * A.sTH[(size_t)e * P2 + c];
     f += dA * ((2 * n + 1) / (4.0 * M_PI)) * A.sTF[(size_t)e * P2 + c];
-    const double dh = h - A.aH[i], df = f - A.aF[i];
-    d2 += dh * dh + df * df;
-    n2 += h * h + f * f;
-    A.aH[i] = h;
-    A.aF[i] = f;
+    const size_t i = (size_t)e * 2 * P2 + c;
+    A.y[i] = A.x[i] - f;
+    A.y[i + P2] = A.x[i + P2] - h;
   }
-  d2 = block_sum(d2, red);
-  n2 = block_sum(n2, red);
-  if (threadIdx.x == 0) A.dev[g] = n2 > 0.0 ? d2 / n2 : d2;
+}
+
+// compact [entry][F, H][p^2] -> the sphere-indexed blocks the re-expansion jobs read
+__global__ void __launch_bounds__(256) k_adj_scatter(const int *pol, const double *x, double *aF, double *aH,
+                                                     int P2) {
+  const int e = blockIdx.x, c = blockIdx.y * 256 + threadIdx.x;
+  if (c >= P2) return;
+  const size_t g = (size_t)pol[e] * P2 + c, i = (size_t)e * 2 * P2 + c;
+  aF[g] = x[i];
+  aH[g] = x[i + P2];
+}
+// the right-hand side in compact form: F part zero, H part LS
+__global__ void __launch_bounds__(256) k_adj_rhs(const int *pol, const double *LS, double *b, int P2) {
+  const int e = blockIdx.x, c = blockIdx.y * 256 + threadIdx.x;
+  if (c >= P2) return;
+  const size_t i = (size_t)e * 2 * P2 + c;
+  b[i] = 0.0;
+  b[i + P2] = LS[(size_t)pol[e] * P2 + c];
+}
+
+// Krylov vector kernels of the restarted GMRES (columns of V are consecutive vectors of N).
+// part[k][b] = sum over block b's range of V_k . w (KB blocks, ranges in order); the host adds
+// the blocks in order, so results do not vary run to run.  A non-finite value is passed on.
+constexpr int KB = 256;
+__global__ void __launch_bounds__(256) k_kdots(const double *V, int ncol, long long N, const double *w,
+                                               double *part) {
+  __shared__ double red[4];
+  const int b = blockIdx.x, k = blockIdx.y;
+  const long long per = (N + KB - 1) / KB, lo = b * per, hi = min(N, lo + per);
+  const double *v = V + (size_t)k * N;
+  double s = 0.0;
+  for (long long i = lo + threadIdx.x; i < hi; i += 256) s += v[i] * w[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) part[(size_t)k * KB + b] = s;
+}
+// w = alpha w - sum_k h[k] V_k (h on the device, ncol <= 64)
+__global__ void __launch_bounds__(256) k_kaxpy(const double *V, int ncol, long long N, const double *h, double alpha,
+                                               double *w) {
+  __shared__ double hs[64];
+  if (threadIdx.x < ncol) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  double s = alpha * w[i];
+  for (int k = 0; k < ncol; k++) s -= hs[k] * V[(size_t)k * N + i];
+  w[i] = s;
+}
+// out = w * s
+__global__ void __launch_bounds__(256) k_kscale(long long N, const double *w, double s, double *out) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < N) out[i] = w[i] * s;
 }
 
 // out[t][a] = <zeta_g + own_t Hrot_g, src_t[a]>_W for target t (sphere g = sph[t]), a = 0..2:

@@ -18,7 +18,7 @@ def ball_radius(n):
     return 20.0 * (n / 20.0) ** (1.0 / 3.0)
 
 
-def molecule(n_sph, seed, rmin=2.0, rmax=6.0, qmin=10, qmax=40, ball=None, max_tries=200000):
+def molecule(n_sph, seed, rmin=2.0, rmax=6.0, qmin=10, qmax=40, ball=None, max_tries=200000, gap=0.0):
     rng = np.random.default_rng(seed)
     R = ball_radius(n_sph) if ball is None else ball
     cen, rad = [], []
@@ -35,7 +35,7 @@ def molecule(n_sph, seed, rmin=2.0, rmax=6.0, qmin=10, qmax=40, ball=None, max_t
             continue
         ok = True
         for c2, r2 in zip(cen, rad):
-            if np.linalg.norm(c - c2) < r + r2:
+            if np.linalg.norm(c - c2) < r + r2 + gap:
                 ok = False
                 break
         if ok:
