@@ -231,6 +231,18 @@ int pbsam_compute_forces(pbsam_system *sys, double tol, int max_rounds, double *
  * again).  PBSAM_ERR_STATE without the flags or the solve; PBSAM_ERR_UNSUPPORTED when a
  * sphere has interaction partners but no polarization partner. */
 int pbsam_torques_exact(pbsam_system *sys, double tol, int max_rounds, double *torque);
+/* The fixed point of the mutual solve under PBSAM_EXACT_DERIVATIVE by restarted GMRES (no
+ * counterpart in the reference, which iterates polarize_mutual's Gauss-Seidel rounds,
+ * src/molecule.cpp:1989-2075): pair lists and phase 0 as pbsam_solve_mutual, then GMRES on the
+ * linear system of the polarized spheres' F and H blocks (one operator application = one
+ * Jacobi round of the mutual solve without its constant terms, with the inner solve's limit)
+ * until the squared residual <= tol times the squared right-hand side, or max_rounds
+ * applications.  Leaves the state pbsam_energy, pbsam_gradient and pbsam_adjoint_forces read.
+ * The reference's iteration need not converge (on molecules with nearly touching spheres its
+ * Jacobi rounds diverge and its Gauss-Seidel rounds wander); GMRES needs only the linear
+ * system to be invertible.  PBSAM_ERR_STATE without the flags; PBSAM_ERR_UNSUPPORTED for
+ * polarized spheres with buried points; PBSAM_ERR_NOCONV at the limit. */
+int pbsam_solve_exact(pbsam_system *sys, double tol, int max_rounds, int *applications_out);
 /* Forces and torques from the adjoint of the converged mutual-polarization fixed point (no
  * counterpart in the reference, which differentiates by forward sweeps,
  * src/molecule.cpp:2077-2228): after a mutual solve with PBSAM_EXACT_DERIVATIVE set, one

@@ -1452,6 +1452,7 @@ struct AdjState {
   DBuf<double> qTF, qTH, lqF, lqH, sT;            // numerical part: per job point, per point, harmonic sums
   DBuf<double> src, out;                          // contraction: source planes, sums
   DBuf<double> V, w, x, b, part, h;               // Krylov basis, work vector, solution, right-hand side
+  DBuf<double> tq;                                // surface-charge totals of the exact solve's rounds
   DBuf<int> ptgt, joff, jlist, tsph, town, ext;
   DBuf<long long> qoff;
   DBuf<PJob> sjobs;

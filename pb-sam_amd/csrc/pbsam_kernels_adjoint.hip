@@ -199,6 +199,55 @@ __global__ void __launch_bounds__(256) k_adj_apply(CArgs A) {
   }
 }
 
+// The exact-mode fixed-point map of each listed sphere (one workgroup each): its local
+// expansions LH = LS + (intra slots) [+ far], LF = (intra slots) [+ far] and the inner solve's
+// limit per level (solveFH, expcenter.cpp:1080-1135, with the identity surface matrix)
+//   (F, H) = A_n^-1 [B_n (LF, LH) + (c3 Vfix, cInvH Dfix)]
+// (A_n, B_n as in k_adj_local; the fixed terms only for the right-hand side).  With x given,
+// y = x - (F, H) (the operator I - A of the linear part), else y = (F, H); x, y compact.
+struct EArgs {
+  const int *pol;
+  const int *lst_off, *lst;
+  const double *slotH, *slotF;
+  const double *inter;          // [sphere][p^2] LS
+  const double *farH, *farF;    // [sphere][p^2] frozen far-pair expansions, or null
+  const double *Vfix, *Dfix;    // [sphere][p^2] fixed-charge terms, or null
+  const double *consts;
+  const double *x;              // [entry][2][p^2] or null
+  double *y;                    // [entry][2][p^2]
+  int p;
+};
+__global__ void __launch_bounds__(256) k_exact_apply(EArgs A) {
+  const int e = blockIdx.x, g = A.pol[e], p = A.p, P2 = p * p;
+  const double *k = A.consts + (size_t)g * 10 * p;
+  for (int c = threadIdx.x; c < P2; c += 256) {
+    const int n = coef_level(c);
+    const size_t gi = (size_t)g * P2 + c;
+    double lh = A.inter[gi], lf = 0.0;
+    for (int s = A.lst_off[e]; s < A.lst_off[e + 1]; s++) {
+      const size_t o = (size_t)A.lst[s] * P2 + c;
+      lh += A.slotH[o];
+      lf += A.slotF[o];
+    }
+    if (A.farH) {
+      lh += A.farH[gi];
+      lf += A.farF[gi];
+    }
+    const double c3 = 1.0 / double(2 * n + 1);
+    const double cH1 = k[n], cF1 = k[p + n], cLH1 = k[2 * p + n], cLF1 = k[3 * p + n];
+    const double cH2 = k[4 * p + n], cF2 = k[5 * p + n], cLH2 = k[6 * p + n], cLF2 = k[7 * p + n];
+    const double cInvH = k[8 * p + n];
+    const double a11 = 1.0 - c3 * cF1, a12 = -c3 * cH1, a21 = -cInvH * cF2, a22 = 1.0 - cInvH * cH2;
+    const double det = a11 * a22 - a12 * a21;
+    const double r1 = c3 * ((A.Vfix ? A.Vfix[gi] : 0.0) + cLH1 * lh + cLF1 * lf);
+    const double r2 = cInvH * ((A.Dfix ? A.Dfix[gi] : 0.0) + cLH2 * lh + cLF2 * lf);
+    const double F = (a22 * r1 - a12 * r2) / det, H = (a11 * r2 - a21 * r1) / det;
+    const size_t i = (size_t)e * 2 * P2 + c;
+    A.y[i] = A.x ? A.x[i] - F : F;
+    A.y[i + P2] = A.x ? A.x[i + P2] - H : H;
+  }
+}
+
 // compact [entry][F, H][p^2] -> the sphere-indexed blocks the re-expansion jobs read
 __global__ void __launch_bounds__(256) k_adj_scatter(const int *pol, const double *x, double *aF, double *aH,
                                                      int P2) {
@@ -207,6 +256,15 @@ __global__ void __launch_bounds__(256) k_adj_scatter(const int *pol, const doubl
   const size_t g = (size_t)pol[e] * P2 + c, i = (size_t)e * 2 * P2 + c;
   aF[g] = x[i];
   aH[g] = x[i + P2];
+}
+// sphere-indexed F, H blocks of the listed spheres -> compact [entry][F, H][p^2]
+__global__ void __launch_bounds__(256) k_adj_gather(const int *pol, const double *F, const double *H, double *x,
+                                                    int P2) {
+  const int e = blockIdx.x, c = blockIdx.y * 256 + threadIdx.x;
+  if (c >= P2) return;
+  const size_t g = (size_t)pol[e] * P2 + c, i = (size_t)e * 2 * P2 + c;
+  x[i] = F[g];
+  x[i + P2] = H[g];
 }
 // the right-hand side in compact form: F part zero, H part LS
 __global__ void __launch_bounds__(256) k_adj_rhs(const int *pol, const double *LS, double *b, int P2) {

@@ -40,7 +40,7 @@ SYMBOLS = [
     "pbsam_read_pqr", "pbsam_write_imat", "pbsam_read_imat", "pbsam_write_expansion", "pbsam_read_expansion",
     "pbsam_read_config", "pbsam_write_config", "pbsam_read_imats", "pbsam_read_expansions",
     "pbsam_potential", "pbsam_total_int_energy", "pbsam_system_set_flags", "pbsam_autotune",
-    "pbsam_torques_exact", "pbsam_adjoint_forces",
+    "pbsam_torques_exact", "pbsam_adjoint_forces", "pbsam_solve_exact",
     "pbsam_xform_h_batch_noalloc", "pbsam_xform_h_batch_noalloc_bytes",
     "pbsam_ctx_set_store_limit", "pbsam_ctx_stored", "pbsam_ctx_set_graphs", "pbsam_system_graph_stats",
 ]
@@ -132,6 +132,7 @@ def load():
         "pbsam_autotune": ([P, I, P, P], I),
         "pbsam_torques_exact": ([P, D, I, P], I),
         "pbsam_adjoint_forces": ([P, D, I, P, P, P], I),
+        "pbsam_solve_exact": ([P, D, I, P], I),
         "pbsam_solve_mutual": ([P, D, I, ctypes.c_uint, P], I),
         "pbsam_mutual_rounds": ([P, I, ctypes.c_uint, P], I),
         "pbsam_system_get": ([P, I, I, P, P, P, P], I),
@@ -507,6 +508,15 @@ class System:
         t = np.zeros(3 * self.n)
         self.ctx.check(self.ctx.L.pbsam_torques_exact(self.h, tol, max_rounds, _p(t)))
         return t.reshape(-1, 3)
+
+    def solve_exact(self, tol=1e-24, max_rounds=2000):
+        """pbsam_solve_exact: the exact-mode fixed point by GMRES; returns (applications, status)
+        with status OK or ERR_NOCONV"""
+        n = ctypes.c_int()
+        rc = self.ctx.L.pbsam_solve_exact(self.h, tol, max_rounds, ctypes.byref(n))
+        if rc not in (OK, ERR_NOCONV) and not (len(self.groups) > 1 and rc == ERR_OVERLAP):
+            self.ctx.check(rc)
+        return n.value, rc
 
     def adjoint_forces(self, tol=1e-20, max_rounds=5000):
         """pbsam_adjoint_forces: (force, torque, operator applications) of each molecule from one
