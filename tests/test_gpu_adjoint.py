@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from conftest import relerr
-from test_gpu_exact import EPS_OUT, conj, pair, quat_rot
+from test_gpu_exact import pair, quat_rot
 
 pytestmark = pytest.mark.gpu
 
@@ -164,3 +164,99 @@ def test_gradcheck_rigid_energy(gpu, ns, p, npts):
         assert scale > 0
         assert torch.autograd.gradcheck(lambda y: autograd.RigidEnergy.apply(y, ev), (x,), eps=1e-4,
                                         atol=1e-6 * scale, rtol=1e-6)
+
+
+# ---- the exact-mode fixed point by GMRES (pbsam_solve_exact) ----
+
+def test_solve_exact_equals_converged_rounds(gpu, orc):
+    """where the reference's Gauss-Seidel rounds converge, GMRES lands on the same state"""
+    import pbsam_amd
+    gs, _, pos, quat = pair(gpu, orc, 5, 30, 1200)
+    S = system(gpu, gs, pos, quat, tol=1e-24)
+    e0 = S.energy()
+    H0 = [S.get(i, k)["H"] for i in range(2) for k in range(5)]
+    f0, t0, _ = S.adjoint_forces(tol=ADJ_TOL, max_rounds=2000)
+    n, rc = S.solve_exact(tol=1e-26, max_rounds=500)
+    assert rc == pbsam_amd.OK and n > 0
+    assert relerr(S.energy(), e0) < 1e-10
+    H1 = [S.get(i, k)["H"] for i in range(2) for k in range(5)]
+    assert max(relerr(a, b) for a, b in zip(H1, H0)) < 1e-9
+    f1, t1, _ = S.adjoint_forces(tol=ADJ_TOL, max_rounds=2000)
+    assert relerr(f1, f0) < 1e-9 and relerr(t1, t0) < 1e-9
+
+
+def exact_state(gpu, mts, pos, quat, tol=1e-24):
+    import pbsam_amd
+    S = pbsam_amd.System(gpu, mts, pos, quat)
+    S.set_cutoffs(100.0, 100.0)
+    S.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+    n, rc = S.solve_exact(tol=tol, max_rounds=1000)
+    assert rc == pbsam_amd.OK, n
+    return S
+
+
+def test_full_size_forces_and_torques_by_central_differences(gpu):
+    """The benchmark's two 60-sphere molecules (p = 30, 1800 points per sphere, 50 A apart):
+    nearly touching spheres (gaps down to 0.004 A) make the reference's mutual rounds wander
+    (Jacobi rounds diverge), so no converged fixed point comes from them; GMRES converges the
+    exact-mode system, and the adjoint's forces and torques are central differences (1e-4 A,
+    1e-4 rad) of that solved energy to 1e-6."""
+    import pbsam_amd
+    from pbsam_amd import synth
+    mols = [synth.molecule(60, seed=s) for s in (101, 102)]
+    mts = []
+    for m in mols:
+        mt = pbsam_amd.MolType(gpu, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], 30, eps_in=synth.EPS_IN,
+                               rcen=m["rcen"], npts=[1800] * 60)
+        mt.selfpol(tol=0, max_rounds=0)
+        mt.values()
+        mts.append(mt)
+    pos, quat = synth.place_pair(mols[0], mols[1], 50.0, seed=7)
+    S = exact_state(gpu, mts, pos, quat)
+    e = S.energy()
+    assert relerr(e[0], e[1]) < 1e-9  # the two molecules' energies agree at the fixed point
+    fa, ta, n = S.adjoint_forces(tol=ADJ_TOL, max_rounds=1000)
+    print("adjoint applications", n, flush=True)
+    assert relerr(fa[0], -fa[1]) < 1e-9
+    h = 1e-4
+    for mol in range(2):
+        fd, td = np.zeros(3), np.zeros(3)
+        for d in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[mol, d] += h
+            pm[mol, d] -= h
+            ep, em = (exact_state(gpu, mts, x, quat).energy()[mol] for x in (pp, pm))
+            fd[d] = -(ep - em) / (2 * h)
+            qp, qm = quat.copy(), quat.copy()
+            qp[mol] = turned(quat[mol], d, h)
+            qm[mol] = turned(quat[mol], d, -h)
+            ep, em = (exact_state(gpu, mts, pos, x).energy()[mol] for x in (qp, qm))
+            td[d] = -(ep - em) / (2 * h)
+        print("molecule", mol, fa[mol], fd, ta[mol], td, flush=True)
+        assert np.abs(fa[mol] - fd).max() < 1e-6 * np.linalg.norm(fd)
+        assert np.abs(ta[mol] - td).max() < 1e-6 * np.linalg.norm(td)
+
+
+def test_solve_exact_batch_equals_single_systems(gpu, orc):
+    import pbsam_amd
+    from pbsam_amd import synth
+    gs, _, _, _ = pair(gpu, orc, 4, 10, 400)
+    d = 2 * synth.ball_radius(4) + 6.0
+    mols = [synth.molecule(4, seed=s) for s in (4, 5)]
+    poses = [synth.place_pair(mols[0], mols[1], d + b, seed=40 + b) for b in range(3)]
+    alone = []
+    for pos, quat in poses:
+        S = exact_state(gpu, gs, pos, quat, tol=1e-26)
+        alone.append((S.energy(), S.adjoint_forces(tol=ADJ_TOL, max_rounds=1000)[:2]))
+    B = pbsam_amd.System(gpu, gs * 3, np.vstack([pp for pp, _ in poses]), np.vstack([qq for _, qq in poses]),
+                         groups=[2] * 3)
+    B.set_cutoffs(100.0, 100.0)
+    B.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+    n, rc = B.solve_exact(tol=1e-26, max_rounds=1000)
+    assert rc == pbsam_amd.OK
+    eb = B.energy()
+    fb, tb, _ = B.adjoint_forces(tol=ADJ_TOL, max_rounds=1000)
+    for g, (e, (fa, ta)) in enumerate(alone):
+        assert relerr(eb[2 * g:2 * g + 2], e) < 1e-10
+        assert relerr(fb[2 * g:2 * g + 2], fa) < 1e-9
+        assert relerr(tb[2 * g:2 * g + 2], ta) < 1e-9
