@@ -1453,6 +1453,9 @@ struct AdjState {
   DBuf<double> src, out;                          // contraction: source planes, sums
   DBuf<double> V, w, x, b, part, h;               // Krylov basis, work vector, solution, right-hand side
   DBuf<double> tq;                                // surface-charge totals of the exact solve's rounds
+  DBuf<double> rP, rst;                           // inter pairs' directions and stored rotations
+  DBuf<RJob> rj;
+  DBuf<XJob> xinter;                              // inter jobs applying them
   DBuf<int> ptgt, joff, jlist, tsph, town, ext;
   DBuf<long long> qoff;
   DBuf<PJob> sjobs;
