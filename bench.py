@@ -21,7 +21,10 @@ trajectories of two 100-sphere molecules at p=10, forces every step, reported as
 trajectory-steps/s beside the oracle's trajectories on the host cores.  `large` is the
 large-system workload: eight distinct 500-sphere molecules at p=30 with 2000 points per
 sphere in one system (set-up, mutual solves and force evaluations; operators past the
-context's store cap are made per use).
+context's store cap are made per use).  `exact` is the exact derivative on the main system:
+its fixed point by GMRES (pbsam_solve_exact; the reference's rounds do not converge on these
+molecules, whose spheres nearly touch) and both molecules' forces and torques from the
+adjoint (pbsam_adjoint_forces).
 
 Prints one JSON line.  `roofline` is the kernel with the largest share of device time,
 timed with HIP events around each of its launches on the library's stream, with the
@@ -139,6 +142,43 @@ def batch_leg(pbsam_amd, synth, ctx, mols, mts, nsys, reps, nplain):
                 max_rel_err_force_batch_vs_plain=err, groups_ok=int((st == 0).sum()),
                 mutual_rounds_max=int(rounds.max()), grad_passes_mean=float(passes.mean()),
                 snapshots=(P, Q))
+
+
+def exact_leg(pbsam_amd, ctx, mts, pos, quat, reps):
+    """The exact derivative of the solved energy on the same system: its fixed point by GMRES
+    (pbsam_solve_exact) and every molecule's force and torque from the adjoint
+    (pbsam_adjoint_forces), both to a squared relative residual of 1e-16, timed over `reps`
+    evaluations after a 1e-24 evaluation (warm-up and the reference the timed forces are
+    compared with).  The reference's own rounds do not converge on these molecules (nearly
+    touching spheres), so no forward-mode exact evaluation exists to time beside it."""
+    import numpy as np
+    S = pbsam_amd.System(ctx, mts, pos, quat)
+    S.set_cutoffs(100.0, 100.0)
+    S.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+    S.solve_exact(tol=1e-24, max_rounds=2000)
+    f0, t0_, _ = S.adjoint_forces(tol=1e-24, max_rounds=2000)
+    ctx.sync()
+    mark = roctx()
+    if mark:
+        mark.roctxRangePushA(b"exact_region")
+    ts = ta = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        napp, rc = S.solve_exact(tol=1e-16, max_rounds=1000)
+        t1 = time.perf_counter()
+        f, t, nadj = S.adjoint_forces(tol=1e-16, max_rounds=1000)
+        t2 = time.perf_counter()
+        ts += t1 - t0
+        ta += t2 - t1
+    if mark:
+        mark.roctxRangePop()
+    S.close()
+    return dict(evals_per_s=reps / (ts + ta), solve_s=ts / reps, adjoint_s=ta / reps, reps=reps, tol=1e-16,
+                applications_solve=napp, applications_adjoint=nadj, status=rc,
+                max_rel_err_force_vs_1e24=float(np.abs(f - f0).max() / np.abs(f0).max()),
+                max_rel_err_torque_vs_1e24=float(np.abs(t - t0_).max() / np.abs(t0_).max()),
+                workload="the same 2 x 60-sphere p=30 system: exact-mode fixed point by GMRES + adjoint forces and "
+                         "torques of both molecules (-dE/dr, -dE/domega, lab frame)")
 
 
 BD_SPH, BD_P, BD_NPTS, BD_SEEDS = 100, 10, 400, (301, 302)
@@ -580,6 +620,8 @@ def main():
     ap.add_argument("--solves", type=int, default=20, help="mutual solves timed for solves_per_s")
     ap.add_argument("--mixed", type=int, default=256, help="systems in the mixed-order batch leg (0: skip it)")
     ap.add_argument("--tight-solves", type=int, default=1, help="mutual solves timed at the tight tolerance")
+    ap.add_argument("--exact", type=int, default=2, metavar="REPS",
+                    help="exact evaluations (GMRES solve + adjoint) timed on the same system (0: skip)")
     ap.add_argument("--bd", type=int, default=1024, help="trajectories in the Brownian-dynamics leg (0: skip it)")
     ap.add_argument("--bd-steps", type=int, default=3, help="timed steps of the Brownian-dynamics leg")
     ap.add_argument("--large", type=int, default=500, metavar="NSPH",
@@ -673,6 +715,13 @@ def main():
                      status=trc)
         if mark:
             mark.roctxRangePop()
+
+    exact = None
+    if args.exact > 0:
+        exact = exact_leg(pbsam_amd, ctx, mts, pos, quat, args.exact)
+        print("exact derivative (GMRES solve %d applications + adjoint %d): %.2f evals/s" %
+              (exact["applications_solve"], exact["applications_adjoint"], exact["evals_per_s"]), file=sys.stderr,
+              flush=True)
 
     dom = max(prof, key=lambda k: prof[k]["ms"])
     d = prof[dom]
@@ -777,6 +826,7 @@ def main():
         },
         "solves_per_s": solves_per_s,
         "tight_solve": tight,
+        "exact": exact,
         "batch": batch,
         "mixed_batch": mixed,
         "bd": bd,
