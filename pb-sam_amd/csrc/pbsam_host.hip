@@ -1456,9 +1456,9 @@ struct AdjState {
   DBuf<double> rP, rst;                           // inter pairs' directions and stored rotations
   DBuf<RJob> rj;
   DBuf<XJob> xinter;                              // inter jobs applying them
+  DBuf<GVJob> gnum, gsurf;                        // stored-matrix products: numerical pairs, surfaces
   DBuf<int> ptgt, joff, jlist, tsph, town, ext;
   DBuf<long long> qoff;
-  DBuf<PJob> sjobs;
 };
 
 struct pbsam_system {

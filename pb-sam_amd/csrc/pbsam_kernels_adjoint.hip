@@ -143,6 +143,45 @@ __global__ void __launch_bounds__(ST) k_p2lT(Tabs tb, TArgs A) {
   }
 }
 
+// Products of stored point matrices with two charge sets: out1[slot][c] = sum_h M1[h][c] q1[h],
+// out2[slot][c] = sum_h M2[h][c] q2[h] over the job's points h (point-major npts x p^2
+// matrices, charges at pt_off + h), points added in order.  Used for the numerical
+// re-expansion with a molecule type's stored maps (M1 the F map, M2 the H map) and for the
+// harmonic sums of a sphere's adjoint surface charges (M1 = M2 = the stored harmonics of its
+// exposed points).  Grid (jobs, chunks of 256 coefficients); each thread streams one column.
+struct GVJob {
+  const double *M1, *M2;
+  int npts, pt_off, slot, pad;
+};
+__global__ void __launch_bounds__(256) k_mapgemv(const GVJob *jobs, const double *q1, const double *q2, double *out1,
+                                                 double *out2, int P2) {
+  const GVJob J = jobs[blockIdx.x];
+  const int c = blockIdx.y * 256 + threadIdx.x;
+  if (c >= P2) return;
+  const double *m1 = J.M1 + c, *m2 = J.M2 + c, *a = q1 + J.pt_off, *b = q2 + J.pt_off;
+  double s1 = 0.0, s2 = 0.0;
+  int h = 0;
+  for (; h + 4 <= J.npts; h += 4) {  // eight loads in flight, added in point order
+    double x[4], y[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      x[u] = m1[(size_t)(h + u) * P2];
+      y[u] = m2[(size_t)(h + u) * P2];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      s1 += x[u] * a[h + u];
+      s2 += y[u] * b[h + u];
+    }
+  }
+  for (; h < J.npts; h++) {
+    s1 += m1[(size_t)h * P2] * a[h];
+    s2 += m2[(size_t)h * P2] * b[h];
+  }
+  out1[(size_t)J.slot * P2 + c] = s1;
+  out2[(size_t)J.slot * P2 + c] = s2;
+}
+
 // Adjoint surface charges of each listed sphere: at each of its exposed points, the sum over
 // the numerical jobs that read that sphere's charges, in job order.  Grid (spheres, chunks of
 // 256 points).
