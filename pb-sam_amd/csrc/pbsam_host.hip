@@ -1455,7 +1455,7 @@ struct AdjState {
   DBuf<double> tq;                                // surface-charge totals of the exact solve's rounds
   DBuf<double> rP, rst;                           // inter pairs' directions and stored rotations
   DBuf<RJob> rj;
-  DBuf<XJob> xinter;                              // inter jobs applying them
+  DBuf<XJob> xinter, xintra;                      // inter jobs applying them; intra jobs, stored first
   DBuf<GVJob> gnum, gsurf;                        // stored-matrix products: numerical pairs, surfaces
   DBuf<int> ptgt, joff, jlist, tsph, town, ext;
   DBuf<long long> qoff;

@@ -757,6 +757,16 @@ __device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e,
     }
   }
 }
+// The stored-rotation path of k_xform on its own, at full orders: without the level
+// recursion's registers more workgroups fit on a CU.  Every job of a launch must carry stored
+// rotations (J.rot), since only their LDS (source blocks and level vectors) is provided.
+template <int XT, int NPL>
+__global__ void __launch_bounds__(XT) k_xapply(XArgs A) {
+  const int e = blockIdx.x;
+  const XJob J = A.jobs[A.sel ? A.sel[e] : e];
+  const int p = A.p, pF = (J.kind == 1) ? p : 0;
+  xapply_job<XT, NPL>(A, J, p, pF, p, J.tord ? J.tord : p);
+}
 template <int XT, int NPL>
 __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
   // one job per workgroup; a class of a binned batch is launched for the whole batch and
