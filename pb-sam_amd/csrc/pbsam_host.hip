@@ -404,11 +404,31 @@ static void launch_trans(pbsam_ctx *c, int njobs, const TJob *jobs, int p, doubl
   hipLaunchKernelGGL(k_trans, dim3(njobs), dim3(64), lds, c->stream, c->tabs, jobs, sel, TQ, p, out);
   HIPCHK(hipGetLastError());
 }
-// one k_xform launch for n workgroups whose orders are all <= qmax
+// LDS of k_xapply: source blocks and level vectors of the planes (H, and F when given)
+static size_t xapply_lds(int q, int planes, bool with_f) {
+  const size_t nb = (size_t)planes * (with_f ? 2 : 1), nl = (size_t)q * (q + 1) / 2;
+  return nb * (nl * sizeof(cd) + (size_t)q * q * sizeof(double));
+}
+// one k_xform launch for n workgroups whose orders are all <= qmax (k_xapply when every job
+// carries stored rotations)
 template <int NPL>
 static void launch_xform_qp(pbsam_ctx *c, int n, const XArgs &a, int qmax) {
-  const size_t lds = xform_lds(qmax, NPL, a.F != nullptr);
   const dim3 grid(n, (NPL == 1 && a.split) ? a.planes : 1);
+  if (a.allrot) {
+    const size_t lds = xapply_lds(qmax, NPL, a.F != nullptr);
+    if (qmax <= 8) {
+      set_lds((const void *)k_xapply<64, NPL>, lds);
+      PLAUNCH(c, PBSAM_K_XFORM, (k_xapply<64, NPL>), grid, dim3(64), lds, c->stream, a);
+    } else if (qmax <= 16) {
+      set_lds((const void *)k_xapply<256, NPL>, lds);
+      PLAUNCH(c, PBSAM_K_XFORM, (k_xapply<256, NPL>), grid, dim3(256), lds, c->stream, a);
+    } else {
+      set_lds((const void *)k_xapply<512, NPL>, lds);
+      PLAUNCH(c, PBSAM_K_XFORM, (k_xapply<512, NPL>), grid, dim3(512), lds, c->stream, a);
+    }
+    return;
+  }
+  const size_t lds = xform_lds(qmax, NPL, a.F != nullptr);
   if (qmax <= 8) {
     set_lds((const void *)k_xform<64, NPL>, lds);
     PLAUNCH(c, PBSAM_K_XFORM, (k_xform<64, NPL>), grid, dim3(64), lds, c->stream, c->tabs, a);
@@ -1485,6 +1505,7 @@ struct pbsam_system {
   std::vector<int> pol;  // polarized spheres (global index)
   DBuf<int> d_pol, d_inter_off, d_inter_lst, d_intra_off, d_intra_lst, d_ones_off;
   int n_inter = 0, n_xintra = 0, n_pintra = 0, n_intra_slots = 0;
+  bool xintra_allrot = false;  // every analytical intra job carries stored rotations (k_xapply)
   DBuf<XJob> d_xinter, d_xintra;
   DBuf<PJob> d_pintra;
   DBuf<TJob> d_tinter;
@@ -1821,6 +1842,7 @@ static int system_lists(pbsam_system *S) {
   }
   S->n_inter = (int)xi.size();
   S->n_xintra = (int)xa.size();
+  S->xintra_allrot = std::all_of(xa.begin(), xa.end(), [](const XJob &J) { return J.rot != nullptr; });
   S->n_pintra = (int)pa.size();
   S->n_intra_slots = aslot;
   S->d_Tinter.alloc(std::max<size_t>(ti.size() * TS, 1));
@@ -1925,6 +1947,7 @@ static double system_round(pbsam_system *S, bool round1, bool refresh) {
     xb.slotH = S->d_slotH.p;
     xb.slotF = S->d_slotF.p;
     xb.p = p;
+    xb.allrot = S->xintra_allrot;
     launch_xform(c, S->n_xintra, xb);
     PArgs pa{};
     pa.jobs = S->d_pintra.p;
@@ -2045,6 +2068,7 @@ static void system_gs_round(pbsam_system *S, const std::vector<double> &dev, con
   xb.slotH = S->d_slotH.p;
   xb.slotF = S->d_slotF.p;
   xb.p = p;
+  xb.allrot = S->xintra_allrot;
   PArgs pa{};
   pa.jobs = S->d_pintra.p;
   pa.xyz = S->d_pts.p;

@@ -159,6 +159,7 @@ struct XArgs {
   const int *cnt;      // with cls: device count of the class (workgroups past it leave)
   int nosplit;         // host hint: keep the planes of a job in one workgroup (jobs with
                        // stored rotations read their coefficients once for all planes)
+  int allrot;          // host: every job of the launch carries stored rotations (k_xapply)
 };
 
 // Orders of a batch of re-expansion jobs (the same rule as k_xform), per batch position,
@@ -757,15 +758,43 @@ __device__ __forceinline__ void xform_job(const Tabs &tb, const XArgs &A, int e,
     }
   }
 }
-// The stored-rotation path of k_xform on its own, at full orders: without the level
-// recursion's registers more workgroups fit on a CU.  Every job of a launch must carry stored
-// rotations (J.rot), since only their LDS (source blocks and level vectors) is provided.
+// The stored-rotation path of k_xform on its own (launches whose jobs all carry stored
+// rotations, J.rot): without the level recursion's registers more workgroups fit on a CU.
+// Same batch positions, orders, planes and instrumentation as k_xform; the LDS holds only the
+// source blocks and level vectors.
 template <int XT, int NPL>
 __global__ void __launch_bounds__(XT) k_xapply(XArgs A) {
-  const int e = blockIdx.x;
-  const XJob J = A.jobs[A.sel ? A.sel[e] : e];
-  const int p = A.p, pF = (J.kind == 1) ? p : 0;
-  xapply_job<XT, NPL>(A, J, p, pF, p, J.tord ? J.tord : p);
+  if (A.cnt && (int)blockIdx.x >= *A.cnt) return;
+  const int e = A.cls ? A.cls[blockIdx.x] : blockIdx.x;
+  const int pl = (NPL == 1 && A.split) ? blockIdx.y : 0;
+  XJob J = A.jobs[A.sel ? A.sel[e] : e];
+  const int p = A.p;
+  const bool intra = (J.kind == 1);
+  const int tqsrc = J.src;
+  J.src += pl;
+  J.slot += pl;
+  int pH, pF = 0;
+  if (A.ordH) {
+    pH = A.ordH[e];
+    if (intra) pF = A.ordF[e];
+  } else if (A.full) {
+    pH = p;
+    if (intra) pF = p;
+  } else {
+    double rho, th, ph;
+    cart_to_sph(J.P[0], J.P[1], J.P[2], rho, th, ph);
+    const int tqi = A.tqmap ? A.tqmap[tqsrc] : tqsrc;
+    pH = compute_order(rho, A.TQH[tqi], J.a_src, p);
+    if (intra) pF = compute_order(rho, A.TQF[tqi], J.a_src, p);
+  }
+  const int q = pH > pF ? pH : pF;
+  if (A.work && threadIdx.x == 0) {
+    double w = NPL * (2 * work_rotapply(pH) + work_transapply(pH));
+    if (intra) w += NPL * (2 * work_rotapply(pF) + work_transapply(pF));
+    atomicAdd(&A.work[0], (unsigned long long)w);
+    atomicAdd(&A.work[1], (unsigned long long)(8 * NPL * (pH * pH + pF * pF + (intra ? 2 : 1) * p * p)));
+  }
+  xapply_job<XT, NPL>(A, J, pH, pF, q, J.tord ? J.tord : pH);
 }
 template <int XT, int NPL>
 __global__ void __launch_bounds__(XT) k_xform(Tabs tb, XArgs A) {
