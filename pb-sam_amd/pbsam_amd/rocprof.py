@@ -23,7 +23,7 @@ import re
 import sys
 from collections import defaultdict
 
-INSTRUMENTED = {"xform": ("k_xform",), "p2l": ("k_p2l", "k_p2lgemv", "k_p2lgemm"), "solve": ("k_solve", "k_solve3"),
+INSTRUMENTED = {"xform": ("k_xform", "k_xapply"), "p2l": ("k_p2l", "k_p2lgemv", "k_p2lgemm"), "solve": ("k_solve", "k_solve3"),
                 "surface": ("k_surface",), "xgrad": ("k_xgrad",), "surfgrad": ("k_surfgemm", "k_surfgemm_k4")}
 
 
