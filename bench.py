@@ -56,8 +56,8 @@ PEAK_HBM = 8.0e12        # MI355X HBM3E, B/s
 # dispatches from the system's pair lists on), each scaled by the same counter for one
 # 256 MiB device-to-device copy (FETCH_SIZE reads half the bytes: x2.00; WRITE_SIZE x1.00),
 # averaged over the group's dispatches in one evaluation.
-TRAFFIC = {"solve": 1325482, "xform": 2377414, "surfgrad": 13549723, "p2l": 36810805, "surface": 23225371,
-           "xgrad": 161754330}
+TRAFFIC = {"solve": 1325971, "xform": 2352671, "surfgrad": 13555872, "p2l": 36803948, "surface": 23240738,
+           "xgrad": 161750981}
 
 
 def roctx():
