@@ -246,17 +246,19 @@ int pbsam_solve_exact(pbsam_system *sys, double tol, int max_rounds, int *applic
 /* Forces and torques from the adjoint of the converged mutual-polarization fixed point (no
  * counterpart in the reference, which differentiates by forward sweeps,
  * src/molecule.cpp:2077-2228): after a mutual solve with PBSAM_EXACT_DERIVATIVE set, one
- * transposed solve for all systems of the handle (restarted GMRES until the squared residual
- * <= tol times the squared right-hand side, or max_rounds operator applications; each
- * application is one pass of the re-expansion kernels over every polarized sphere), then its
- * contraction with the source planes of the positions and orientations.  force and torque
+ * transposed solve for all two-molecule systems of the handle, and for a system of n > 2
+ * molecules one per molecule (solved for every such system of the handle at once, n passes
+ * for the largest), each restarted GMRES until the squared residual <= tol times the squared
+ * right-hand side, or max_rounds operator applications (each application one pass of the
+ * re-expansion kernels over every polarized sphere); then each solve's contraction with the
+ * source planes of the positions and orientations.  force and torque
  * (3 doubles per molecule, lab frame) receive -dE_i/dr_i and -dE_i/domega_i of each molecule's energy
  * sum_k <H_k, LS_k>, the derivative of the solved energy (pbsam_forces in this mode returns
  * conj(q_i) times the same force when every polarized pair of a molecule lies within the
- * intra cutoff; pbsam_torques_exact the same torque).  sweeps_out: operator applications.
- * PBSAM_ERR_STATE without the flags or the solve; PBSAM_ERR_UNSUPPORTED for systems of more
- * than two molecules, spheres with interaction partners beyond the polarization cutoff, and
- * polarized spheres with buried points; PBSAM_ERR_NOCONV at the limit. */
+ * intra cutoff; pbsam_torques_exact the same torque).  sweeps_out: operator applications,
+ * summed over the passes.  PBSAM_ERR_STATE without the flags or the solve;
+ * PBSAM_ERR_UNSUPPORTED for spheres with interaction partners beyond the polarization cutoff
+ * and polarized spheres with buried points; PBSAM_ERR_NOCONV at the limit (of any pass). */
 int pbsam_adjoint_forces(pbsam_system *sys, double tol, int max_rounds, double *force, double *torque,
                          int *sweeps_out);
 /* gradient blocks of sphere k of molecule mol: gH (3 p^2, polarized spheres) and gLHN

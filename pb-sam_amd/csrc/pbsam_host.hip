@@ -1478,6 +1478,7 @@ struct AdjState {
   DBuf<XJob> xinter, xintra;                      // inter jobs applying them; intra jobs, stored first
   DBuf<GVJob> gnum, gsurf;                        // stored-matrix products: numerical pairs, surfaces
   DBuf<int> ptgt, joff, jlist, tsph, town, ext;
+  DBuf<int> mode, sel;                            // per pass: right-hand side kinds, source spheres
   DBuf<long long> qoff;
 };
 

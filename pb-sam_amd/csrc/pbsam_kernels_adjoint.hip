@@ -305,13 +305,24 @@ __global__ void __launch_bounds__(256) k_adj_gather(const int *pol, const double
   x[i] = F[g];
   x[i + P2] = H[g];
 }
-// the right-hand side in compact form: F part zero, H part LS
-__global__ void __launch_bounds__(256) k_adj_rhs(const int *pol, const double *LS, double *b, int P2) {
+// the right-hand side in compact form: F part zero; H part of entry e the whole field LS at its
+// sphere (mode 0), the part of it from one molecule (mode 1, in part), or zero (mode 2: a group
+// without a solve in this pass)
+__global__ void __launch_bounds__(256) k_adj_rhs(const int *pol, const int *mode, const double *LS,
+                                                 const double *part, double *b, int P2) {
   const int e = blockIdx.x, c = blockIdx.y * 256 + threadIdx.x;
   if (c >= P2) return;
-  const size_t i = (size_t)e * 2 * P2 + c;
+  const size_t i = (size_t)e * 2 * P2 + c, g = (size_t)pol[e] * P2 + c;
+  const int m = mode[e];
   b[i] = 0.0;
-  b[i + P2] = LS[(size_t)pol[e] * P2 + c];
+  b[i + P2] = m == 0 ? LS[g] : (m == 1 ? part[g] : 0.0);
+}
+// H of the spheres flagged in sel, zero elsewhere (sphere-indexed blocks)
+__global__ void __launch_bounds__(256) k_adj_srcmask(const int *sel, const double *H, double *out, int P2) {
+  const int g = blockIdx.x, c = blockIdx.y * 256 + threadIdx.x;
+  if (c >= P2) return;
+  const size_t i = (size_t)g * P2 + c;
+  out[i] = sel[g] ? H[i] : 0.0;
 }
 
 // Krylov vector kernels of the restarted GMRES (columns of V are consecutive vectors of N).

@@ -519,9 +519,9 @@ class System:
         return n.value, rc
 
     def adjoint_forces(self, tol=1e-20, max_rounds=5000):
-        """pbsam_adjoint_forces: (force, torque, operator applications) of each molecule from one
-        transposed solve, -dE/dr and -dE/domega in the lab frame (exact-derivative mode, after a
-        solve)"""
+        """pbsam_adjoint_forces: (force, torque, operator applications) of each molecule from the
+        transposed solves (one for two molecules, one per molecule for more), -dE/dr and
+        -dE/domega in the lab frame (exact-derivative mode, after a solve)"""
         f, t, n = np.zeros(3 * self.n), np.zeros(3 * self.n), ctypes.c_int()
         self.ctx.check(self.ctx.L.pbsam_adjoint_forces(self.h, tol, max_rounds, _p(f), _p(t), ctypes.byref(n)))
         return f.reshape(-1, 3), t.reshape(-1, 3), n.value

@@ -131,13 +131,137 @@ def test_adjoint_needs_the_exact_mode(gpu, orc):
     with pytest.raises(pbsam_amd.PbsamError) as e:
         S.adjoint_forces()
     assert e.value.code == pbsam_amd.ERR_STATE
-    S3 = pbsam_amd.System(gpu, gs + gs[:1], np.vstack([pos, pos[:1] + 200.0]), np.vstack([quat, quat[:1]]))
-    S3.set_cutoffs(100.0, 100.0)
-    S3.set_flags(pbsam_amd.EXACT_DERIVATIVE)
-    S3.solve(tol=1e-12, max_rounds=400)
+    Si = pbsam_amd.System(gpu, gs, pos, quat)
+    Si.set_cutoffs(10.0, 100.0)  # interaction partners beyond the polarization cutoff
+    Si.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+    Si.solve(tol=1e-12, max_rounds=400)
     with pytest.raises(pbsam_amd.PbsamError) as e:
-        S3.adjoint_forces()
+        Si.adjoint_forces()
     assert e.value.code == pbsam_amd.ERR_UNSUPPORTED
+
+
+def triangle(gpu, orc, ns=4, p=10, npts=400, seed=21):
+    """three molecules (the two types of pair(), the first twice) at the corners of a triangle
+    of side 2 R + 6, R the radius of the ball holding each molecule's spheres (so no spheres
+    of different molecules overlap), random orientations"""
+    from pbsam_amd import synth
+    gs, _, _, _ = pair(gpu, orc, ns, p, npts)
+    d = 2 * synth.ball_radius(ns) + 6.0
+    rng = np.random.default_rng(seed)
+    pos = np.array([[0.0, 0.0, 0.0], [d, 0.0, 0.0], [0.5 * d, 0.5 * np.sqrt(3.0) * d, 0.4]])
+    quat = np.array([synth.random_quat(rng) for _ in range(3)])
+    return [gs[0], gs[1], gs[0]], pos, quat
+
+
+def exact_forward(S, quat):
+    """forces (lab frame) and torques of the forward gradient sweeps in the exact-derivative mode"""
+    S.gradient(tol=1e-20, max_rounds=4000)
+    f, _ = S.forces()
+    ff = np.array([quat_rot(quat[i], f[i]) for i in range(len(quat))])
+    return ff, S.torques_exact(tol=1e-20, max_rounds=4000)
+
+
+def test_forward_sweeps_drop_the_coupling_through_a_third_molecule(gpu, orc):
+    """quirk: when the gradient planes of molecule j's move are re-expanded into a sphere of
+    another molecule, the reference takes only the partners in molecule j
+    (reexpandGrad_LowMemory, molecule.cpp:1779-1788), so with three molecules the forward
+    sweeps leave out the change of the third molecule's H; the adjoint, the derivative of the
+    solved energy (central differences in the next test), keeps it: in a triangle the two
+    differ at 1e-5.  In a chain 0 - 1 - 2 with the cutoff between the neighbours' and the
+    ends' separations, the middle molecule's partners do not polarize each other and its
+    derivatives agree (the ends' differ only through the loop 1 - 2 - 1, at 1e-10 here)."""
+    from pbsam_amd import synth
+    gs, pos, quat = triangle(gpu, orc)
+    S = system(gpu, gs, pos, quat)
+    fa, ta, sweeps = S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+    print("triangle: adjoint applications over three solves", sweeps, flush=True)
+    ff, tf = exact_forward(S, quat)
+    gap = max(relerr(fa, ff), relerr(ta, tf))
+    print("triangle: forward sweeps vs adjoint", relerr(fa, ff), relerr(ta, tf), flush=True)
+    assert 1e-6 < gap < 1e-3, gap
+    # chain along x: neighbours' surface separations <= d + 2 R - 4, the ends' >= 2 d - 2 R
+    R = synth.ball_radius(4)
+    d = 50.0
+    cut = 0.5 * ((d + 2 * R - 4) + (2 * d - 2 * R))
+    pos = np.array([[0.0, 0.0, 0.0], [d, 0.0, 0.0], [2 * d, 0.0, 0.0]])
+    S = cut_system(gpu, gs, pos, quat, cut)
+    fa, ta, _ = S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+    ff, tf = exact_forward(S, quat)
+    print("chain: forward sweeps vs adjoint per molecule",
+          [(relerr(fa[m], ff[m]), relerr(ta[m], tf[m])) for m in range(3)], flush=True)
+    assert relerr(fa[1], ff[1]) < 1e-9 and relerr(ta[1], tf[1]) < 1e-9
+
+
+def cut_system(gpu, gs, pos, quat, cut, tol=1e-18):
+    """system() with both cutoffs at cut (no interaction-only partners)"""
+    import pbsam_amd
+    S = pbsam_amd.System(gpu, gs, pos, quat)
+    S.set_cutoffs(cut, cut)
+    S.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+    rounds, rc = S.solve(tol=tol, max_rounds=4000)
+    assert rc == 0, rounds
+    return S
+
+
+def test_adjoint_with_a_distant_third_molecule_equals_the_pair(gpu, orc):
+    """a third molecule beyond the cutoff of both others: per-molecule solves whose right-hand
+    sides reduce to the pair's shared one; its own force and torque are zero"""
+    gs, _, pos, quat = pair(gpu, orc, 4, 10, 400)
+    S = system(gpu, gs, pos, quat)
+    fa, ta, _ = S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+    pos3 = np.vstack([pos, pos[:1] + np.array([0.0, 400.0, 0.0])])
+    quat3 = np.vstack([quat, quat[:1]])
+    S3 = system(gpu, gs + gs[:1], pos3, quat3)
+    f3, t3, _ = S3.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+    assert relerr(f3[:2], fa) < 1e-10
+    assert relerr(t3[:2], ta) < 1e-10
+    assert np.all(f3[2] == 0.0) and np.all(t3[2] == 0.0)
+
+
+def test_adjoint_three_molecules_is_the_energy_gradient(gpu, orc):
+    gs, pos, quat = triangle(gpu, orc, seed=23)
+    S = system(gpu, gs, pos, quat)
+    fa, ta, _ = S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+    h = 1e-4
+    for mol in range(3):
+        fd, td = np.zeros(3), np.zeros(3)
+        for d in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[mol, d] += h
+            pm[mol, d] -= h
+            ep = system(gpu, gs, pp, quat).energy()[mol]
+            em = system(gpu, gs, pm, quat).energy()[mol]
+            fd[d] = -(ep - em) / (2 * h)
+            qp, qm = quat.copy(), quat.copy()
+            qp[mol] = turned(quat[mol], d, h)
+            qm[mol] = turned(quat[mol], d, -h)
+            ep = system(gpu, gs, pos, qp).energy()[mol]
+            em = system(gpu, gs, pos, qm).energy()[mol]
+            td[d] = -(ep - em) / (2 * h)
+        assert np.abs(fa[mol] - fd).max() < 1e-6 * np.linalg.norm(fd), (mol, fa[mol], fd)
+        assert np.abs(ta[mol] - td).max() < 1e-6 * np.linalg.norm(td), (mol, ta[mol], td)
+
+
+def test_adjoint_mixed_group_sizes_equal_single_systems(gpu, orc):
+    """a group of three molecules and a group of two in one handle: the three-molecule group's
+    per-molecule solves run beside the pair's (zero right-hand side after the first pass)"""
+    import pbsam_amd
+    gs3, pos3, quat3 = triangle(gpu, orc, seed=25)
+    gs2, _, pos2, quat2 = pair(gpu, orc, 4, 10, 400)
+    alone = []
+    for gs, pos, quat in ((gs3, pos3, quat3), (gs2, pos2, quat2)):
+        S = system(gpu, gs, pos, quat)
+        fa, ta, _ = S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+        alone.append((fa, ta))
+    B = pbsam_amd.System(gpu, gs3 + gs2, np.vstack([pos3, pos2]), np.vstack([quat3, quat2]), groups=[3, 2])
+    B.set_cutoffs(100.0, 100.0)
+    B.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+    B.solve(tol=1e-18, max_rounds=4000)
+    fb, tb, _ = B.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+    assert relerr(fb[:3], alone[0][0]) < 1e-9
+    assert relerr(tb[:3], alone[0][1]) < 1e-9
+    assert relerr(fb[3:], alone[1][0]) < 1e-9
+    assert relerr(tb[3:], alone[1][1]) < 1e-9
 
 
 @pytest.mark.parametrize("ns,p,npts", [(4, 10, 400), (3, 30, 1200)])
