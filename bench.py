@@ -372,6 +372,7 @@ def bd_leg(pbsam_amd, synth, ctx, kappa, ntraj, nsteps, cpu_budget):
 
 
 LARGE_MOLS, LARGE_P, LARGE_NPTS, LARGE_SEED = 8, 30, 2000, 7000
+EXACT_TOL = 1e-16  # squared relative residual of the exact path's GMRES solves
 
 
 def device_memory():
@@ -391,7 +392,10 @@ def large_leg(pbsam_amd, synth, ctx, nsph, solves, evals):
     different molecules closer than 20 A are polarized (the reference's 10 A would polarize
     none across 10 A gaps), interactions to 100 A.  Set-up (types, self-polarization, frozen
     values) is timed apart; then `solves` mutual solves (stock tolerance) and `evals` complete
-    force+torque evaluations.  Operators beyond the context's store cap are made per use."""
+    force+torque evaluations.  Operators beyond the context's store cap are made per use.
+    Then the exact-derivative path on the same poses with interactions cut at 20 A: the fixed
+    point by GMRES and the adjoint gradient of every molecule's energy with respect to all 48
+    rigid-body coordinates ("exact")."""
     import numpy as np
     rng = np.random.default_rng(LARGE_SEED)
     R = synth.ball_radius(nsph)
@@ -440,6 +444,30 @@ def large_leg(pbsam_amd, synth, ctx, nsph, solves, evals):
     st, passes = S.status()[0], S.status()[2]
     stored, limit = ctx.stored()
     free, total = device_memory()
+    S.close()
+    # the exact-derivative path on the same poses, interactions cut at the polarization cutoff:
+    # the fixed point by GMRES and the adjoint gradient of every molecule's energy with respect
+    # to all 48 rigid-body coordinates (one transposed solve per molecule); the second of two
+    # evaluations is timed (the first makes the inter pairs' rotations)
+    E = pbsam_amd.System(ctx, mts, pos, quat)
+    E.set_cutoffs(20.0, 20.0)
+    E.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+    ex = {}
+    for rep in range(2):
+        t0 = time.perf_counter()
+        n_s, rc_s = E.solve_exact(tol=EXACT_TOL, max_rounds=3000)
+        ctx.sync()
+        t1 = time.perf_counter()
+        fe, te, n_a = E.adjoint_forces(tol=EXACT_TOL, max_rounds=5000)
+        ctx.sync()
+        t2 = time.perf_counter()
+        ex = dict(evals_per_s=1.0 / (t2 - t0), solve_s=t1 - t0, adjoint_s=t2 - t1, applications_solve=n_s,
+                  applications_adjoint=n_a, status=int(rc_s), tol=EXACT_TOL, interact_rcutoff_A=20.0,
+                  force_norm_max=float(np.abs(fe).max()), coordinates=6 * LARGE_MOLS)
+    E.close()
+    print("large system, exact path: %.3f evals/s (GMRES %d applications, adjoint %d over %d solves)"
+          % (ex["evals_per_s"], ex["applications_solve"], ex["applications_adjoint"], LARGE_MOLS), file=sys.stderr,
+          flush=True)
     out = dict(molecules=LARGE_MOLS, spheres_per_molecule=nsph, p=LARGE_P, points_per_sphere=LARGE_NPTS,
                grid_spacing_A=a, inter_rcutoff_A=20.0, interact_rcutoff_A=100.0,
                setup_s=t_setup, first_solve_s=t_first, solves_per_s=1.0 / t_solve, mutual_rounds=rounds,
@@ -447,8 +475,7 @@ def large_leg(pbsam_amd, synth, ctx, nsph, solves, evals):
                inter_pairs=n_inter, polarized_spheres=n_pol, status=int(st[0]),
                force_norm_max=float(np.abs(f).max()),
                stored_GB=stored / 1e9, store_limit_GB=limit / 1e9, device_used_GB=(total - free) / 1e9,
-               device_total_GB=total / 1e9)
-    S.close()
+               device_total_GB=total / 1e9, exact=ex)
     for g in mts:
         g.close()
     return out

@@ -242,6 +242,34 @@ def test_adjoint_three_molecules_is_the_energy_gradient(gpu, orc):
         assert np.abs(ta[mol] - td).max() < 1e-6 * np.linalg.norm(td), (mol, ta[mol], td)
 
 
+def test_adjoint_eight_molecules_all_48_coordinates(gpu, orc):
+    """eight molecules on the corners of a cube (10 A gaps between packing balls): the adjoint's
+    force and torque on every molecule (eight transposed solves) against central differences
+    of that molecule's solved energy, all 48 rigid-body coordinates"""
+    from pbsam_amd import synth
+    from test_gpu_configs import eight, self_polarized
+    mols = [synth.molecule(4, seed=900 + i) for i in range(8)]
+    gs, _ = self_polarized(gpu, orc, mols, 10, 300, oracle=False)
+    pos, quat = eight(mols, 10.0, 3)
+    S = system(gpu, gs, pos, quat)
+    fa, ta, sweeps = S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+    print("eight molecules: adjoint applications over eight solves", sweeps, flush=True)
+    h = 1e-4
+    for mol in range(8):
+        fd, td = np.zeros(3), np.zeros(3)
+        for d in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[mol, d] += h
+            pm[mol, d] -= h
+            fd[d] = -(system(gpu, gs, pp, quat).energy()[mol] - system(gpu, gs, pm, quat).energy()[mol]) / (2 * h)
+            qp, qm = quat.copy(), quat.copy()
+            qp[mol] = turned(quat[mol], d, h)
+            qm[mol] = turned(quat[mol], d, -h)
+            td[d] = -(system(gpu, gs, pos, qp).energy()[mol] - system(gpu, gs, pos, qm).energy()[mol]) / (2 * h)
+        assert np.abs(fa[mol] - fd).max() < 1e-6 * np.linalg.norm(fd), (mol, fa[mol], fd)
+        assert np.abs(ta[mol] - td).max() < 1e-6 * np.linalg.norm(td), (mol, ta[mol], td)
+
+
 def test_adjoint_mixed_group_sizes_equal_single_systems(gpu, orc):
     """a group of three molecules and a group of two in one handle: the three-molecule group's
     per-molecule solves run beside the pair's (zero right-hand side after the first pass)"""

@@ -16,7 +16,9 @@
   forces and torques are invariant under a global rotation and translation).
 * The large-system workload at full size: eight distinct 500-sphere molecules at p = 30
   with 2000 points per sphere on a grid with 10 A gaps (about 200 GB of device memory;
-  operators past the store cap are made per use), by the same invariances.
+  operators past the store cap are made per use), by the same invariances; there also the
+  adjoint gradient of all 48 rigid-body coordinates (exact mode), by its covariance under a
+  rigid motion and central differences of one force and one torque component.
 """
 import numpy as np
 import pytest
@@ -206,7 +208,8 @@ def test_eight_by_500_p30_full_size():
     translation matrices and all numerical maps made per use (the store cap of a quarter of
     the device's memory is reached), and a rigid motion of the whole system leaves the
     energies after two simultaneous rounds unchanged to 1e-9 and the returned forces (stock
-    schedule) to 1e-3.  Most spheres here interact without being polarized; their forces are
+    schedule) to 1e-3; the adjoint forces and torques of all eight molecules turn with it
+    and match central differences of the solved energy.  Most spheres here interact without being polarized; their forces are
     rotation covariant with PBSAM_SELF_FRAME (the reference's pairing of the unrotated self
     H with a lab-frame gradient is not, tests/test_oracle_forces.py), which is set."""
     import pbsam_amd
@@ -229,9 +232,10 @@ def test_eight_by_500_p30_full_size():
     quat = np.array([synth.random_quat(rng) for _ in range(8)])
     Q = synth.random_quat(np.random.default_rng(12))
     shift = np.array([13.0, -7.0, 21.0])
+    pos2 = np.array([quat_rot(Q, x) for x in pos]) + shift
+    quat2 = np.array([quat_mul(Q, q) for q in quat])
     res = []
-    for P_, Q_ in ((pos, quat), (np.array([quat_rot(Q, x) for x in pos]) + shift,
-                                 np.array([quat_mul(Q, q) for q in quat]))):
+    for P_, Q_ in ((pos, quat), (pos2, quat2)):
         S = pbsam_amd.System(ctx, gs, P_, Q_)
         S.set_cutoffs(20.0, 100.0)
         S.set_flags(pbsam_amd.SELF_FRAME)
@@ -246,6 +250,52 @@ def test_eight_by_500_p30_full_size():
     assert relerr(res[1][0], res[0][0]) < 1e-9
     assert np.abs(res[0][1]).max() > 0 and np.isfinite(res[0][1]).all()
     assert relerr(res[1][1], res[0][1]) < 1e-3
+    # The adjoint gradient of every molecule's energy with respect to all 48 rigid-body
+    # coordinates (one transposed solve per molecule), interactions cut at the polarization
+    # cutoff: the exact-mode fixed point by GMRES, then the adjoint.  It needs the solution
+    # only (no iterates kept).  The rigid motion turns the lab-frame forces and torques with
+    # it, and central differences of molecule 0's solved energy give one force and one torque
+    # component.
+    from test_gpu_torque import turned
+
+    def exact(P_, Q_):
+        S = pbsam_amd.System(ctx, gs, P_, Q_)
+        S.set_cutoffs(20.0, 20.0)
+        S.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+        n, rc = S.solve_exact(tol=1e-24, max_rounds=3000)
+        assert rc == 0, n
+        return S
+
+    S = exact(pos, quat)
+    f0, t0, napp = S.adjoint_forces(tol=1e-20, max_rounds=5000)
+    S.close()
+    print("8 x 500 adjoint: applications over eight solves", napp, flush=True)
+    assert np.abs(f0).max() > 0 and np.isfinite(f0).all() and np.isfinite(t0).all()
+    S = exact(pos2, quat2)
+    f1, t1, _ = S.adjoint_forces(tol=1e-20, max_rounds=5000)
+    S.close()
+    assert relerr(f1, np.array([quat_rot(Q, x) for x in f0])) < 1e-6, (f1, f0)
+    assert relerr(t1, np.array([quat_rot(Q, x) for x in t0])) < 1e-6, (t1, t0)
+    h = 1e-4
+
+    def e0(P_, Q_):
+        S = exact(P_, Q_)
+        e = S.energy()[0]
+        S.close()
+        return e
+
+    pp, pm = pos.copy(), pos.copy()
+    pp[0, 0] += h
+    pm[0, 0] -= h
+    fd = -(e0(pp, quat) - e0(pm, quat)) / (2 * h)
+    qp, qm = quat.copy(), quat.copy()
+    qp[0] = turned(quat[0], 2, h)
+    qm[0] = turned(quat[0], 2, -h)
+    td = -(e0(pos, qp) - e0(pos, qm)) / (2 * h)
+    print("8 x 500 molecule 0: force x adjoint %.9e fd %.9e, torque z adjoint %.9e fd %.9e"
+          % (f0[0, 0], fd, t0[0, 2], td), flush=True)
+    assert abs(f0[0, 0] - fd) < 1e-5 * np.linalg.norm(f0[0])
+    assert abs(t0[0, 2] - td) < 1e-5 * np.linalg.norm(t0[0])
     for g in gs:
         g.close()
     ctx.close()
