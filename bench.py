@@ -454,6 +454,9 @@ def large_leg(pbsam_amd, synth, ctx, nsph, solves, evals):
     E.set_flags(pbsam_amd.EXACT_DERIVATIVE)
     ex = {}
     for rep in range(2):
+        mark = roctx() if rep else None
+        if mark:
+            mark.roctxRangePushA(b"large_exact_region")
         t0 = time.perf_counter()
         n_s, rc_s = E.solve_exact(tol=EXACT_TOL, max_rounds=3000)
         ctx.sync()
@@ -461,6 +464,8 @@ def large_leg(pbsam_amd, synth, ctx, nsph, solves, evals):
         fe, te, n_a = E.adjoint_forces(tol=EXACT_TOL, max_rounds=5000)
         ctx.sync()
         t2 = time.perf_counter()
+        if mark:
+            mark.roctxRangePop()
         ex = dict(evals_per_s=1.0 / (t2 - t0), solve_s=t1 - t0, adjoint_s=t2 - t1, applications_solve=n_s,
                   applications_adjoint=n_a, status=int(rc_s), tol=EXACT_TOL, interact_rcutoff_A=20.0,
                   force_norm_max=float(np.abs(fe).max()), coordinates=6 * LARGE_MOLS)

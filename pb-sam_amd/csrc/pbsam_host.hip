@@ -1479,6 +1479,9 @@ struct AdjState {
   DBuf<GVJob> gnum, gsurf;                        // stored-matrix products: numerical pairs, surfaces
   DBuf<int> ptgt, joff, jlist, tsph, town, ext;
   DBuf<int> mode, sel;                            // per pass: right-hand side kinds, source spheres
+  DBuf<int> zslot;                                // slots of the jobs the operator leaves out
+  DBuf<PJob> pjk;                                 // numerical jobs the operator keeps
+  DBuf<GVJob> gnumk;                              // their stored-map products
   DBuf<long long> qoff;
 };
 

@@ -55,6 +55,15 @@ __global__ void __launch_bounds__(256) k_adj_zero_slots(const PJob *jobs, double
   slotF[o] = 0.0;
 }
 
+// Zero the listed output slots (H and F).  Grid (slots, chunks of 256 coefficients).
+__global__ void __launch_bounds__(256) k_adj_zero_slotlist(const int *slots, double *slotH, double *slotF, int P2) {
+  const int c = blockIdx.y * 256 + threadIdx.x;
+  if (c >= P2) return;
+  const size_t o = (size_t)slots[blockIdx.x] * P2 + c;
+  slotH[o] = 0.0;
+  slotF[o] = 0.0;
+}
+
 // Transposed numerical re-expansion (CXFormNIntra::xformFH, xforms.cpp:528-550).  The forward
 // job adds to the target's LF/LH the local expansions of the source's surface charges; its
 // transpose gives, at every exposed point h of the source, the weighted contraction

@@ -4,7 +4,7 @@
 
 Prints a Markdown table of per-kernel dispatch statistics over the whole trace and, when
 the marker trace holds bench.py's "timed_region" (and "batch_region", "mixed_region", "bd_region",
-"large_region", "tight_region", "exact_region", "kernel_timing_region") range, over the dispatches that ran inside it; with bench.py's JSON line it also
+"large_region", "tight_region", "exact_region", "large_exact_region", "kernel_timing_region") range, over the dispatches that ran inside it; with bench.py's JSON line it also
 lists bench.py's own HIP-event averages of the instrumented kernels (timed in the instrumented repeat of
 the timed evaluations) beside rocprofv3's over the same dispatches.
 
@@ -74,7 +74,9 @@ def main(argv):
                         ("bd_region", "bench.py's Brownian-dynamics leg"),
                         ("large_region", "bench.py's large-system force evaluations"),
                         ("tight_region", "bench.py's tight-tolerance mutual solve"),
-                        ("exact_region", "bench.py's exact evaluations (GMRES solve + adjoint)")):
+                        ("exact_region", "bench.py's exact evaluations (GMRES solve + adjoint)"),
+                        ("large_exact_region", "bench.py's large-system exact evaluation (GMRES solve + "
+                                               "adjoint of all 48 coordinates)")):
         brng = find_range(argv[2], label)
         if brng is not None:
             inb = [r for r in rows if r[1] >= brng[0] and r[2] <= brng[1]]
