@@ -1480,6 +1480,7 @@ struct AdjState {
   DBuf<int> ptgt, joff, jlist, tsph, town, ext;
   DBuf<int> mode, sel;                            // per pass: right-hand side kinds, source spheres
   DBuf<int> zslot;                                // slots of the jobs the operator leaves out
+  DBuf<int> loff, llst;                           // per target, the slots the operator's sums read
   DBuf<PJob> pjk;                                 // numerical jobs the operator keeps
   DBuf<GVJob> gnumk;                              // their stored-map products
   DBuf<long long> qoff;
