@@ -1475,6 +1475,8 @@ struct AdjState {
   DBuf<double> tq;                                // surface-charge totals of the exact solve's rounds
   DBuf<double> rP, rst;                           // inter pairs' directions and stored rotations
   DBuf<RJob> rj;
+  DBuf<double> rP2, rst2;                         // the same for the operator's intra jobs
+  DBuf<RJob> rj2;
   DBuf<XJob> xinter, xintra;                      // inter jobs applying them; intra jobs, stored first
   DBuf<GVJob> gnum, gsurf;                        // stored-matrix products: numerical pairs, surfaces
   DBuf<int> ptgt, joff, jlist, tsph, town, ext;
@@ -1483,6 +1485,7 @@ struct AdjState {
   DBuf<int> loff, llst;                           // per target, the slots the operator's sums read
   DBuf<PJob> pjk;                                 // numerical jobs the operator keeps
   DBuf<GVJob> gnumk;                              // their stored-map products
+  DBuf<double> gvpart;                            // split stored-map products' partial sums
   DBuf<long long> qoff;
 };
 

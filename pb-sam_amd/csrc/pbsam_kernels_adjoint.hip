@@ -162,33 +162,72 @@ struct GVJob {
   const double *M1, *M2;
   int npts, pt_off, slot, pad;
 };
+// Split launches (gridDim.z > 1, for few jobs): workgroup z takes points [z spl, (z+1) spl)
+// and writes its sums to part[((job nz + z) 2 + set) P2 + c]; k_mapgemv_sum adds them in z order.
+// When M1 == M2 (a sphere's harmonics) each matrix element is loaded once.
 __global__ void __launch_bounds__(256) k_mapgemv(const GVJob *jobs, const double *q1, const double *q2, double *out1,
-                                                 double *out2, int P2) {
+                                                 double *out2, int P2, int spl, double *part) {
   const GVJob J = jobs[blockIdx.x];
   const int c = blockIdx.y * 256 + threadIdx.x;
   if (c >= P2) return;
+  const int nz = gridDim.z, lo = nz > 1 ? min(J.npts, (int)blockIdx.z * spl) : 0;
+  const int hi = nz > 1 ? min(J.npts, lo + spl) : J.npts;
   const double *m1 = J.M1 + c, *m2 = J.M2 + c, *a = q1 + J.pt_off, *b = q2 + J.pt_off;
   double s1 = 0.0, s2 = 0.0;
-  int h = 0;
-  for (; h + 4 <= J.npts; h += 4) {  // eight loads in flight, added in point order
-    double x[4], y[4];
+  int h = lo;
+  if (J.M1 == J.M2) {
+    for (; h + 8 <= hi; h += 8) {  // eight loads in flight, added in point order
+      double x[8];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      x[u] = m1[(size_t)(h + u) * P2];
-      y[u] = m2[(size_t)(h + u) * P2];
+      for (int u = 0; u < 8; u++) x[u] = m1[(size_t)(h + u) * P2];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        s1 += x[u] * a[h + u];
+        s2 += x[u] * b[h + u];
+      }
     }
+  } else {
+    for (; h + 4 <= hi; h += 4) {  // eight loads in flight, added in point order
+      double x[4], y[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      s1 += x[u] * a[h + u];
-      s2 += y[u] * b[h + u];
+      for (int u = 0; u < 4; u++) {
+        x[u] = m1[(size_t)(h + u) * P2];
+        y[u] = m2[(size_t)(h + u) * P2];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        s1 += x[u] * a[h + u];
+        s2 += y[u] * b[h + u];
+      }
     }
   }
-  for (; h < J.npts; h++) {
+  for (; h < hi; h++) {
     s1 += m1[(size_t)h * P2] * a[h];
     s2 += m2[(size_t)h * P2] * b[h];
   }
-  out1[(size_t)J.slot * P2 + c] = s1;
-  out2[(size_t)J.slot * P2 + c] = s2;
+  if (nz > 1) {
+    double *o = part + ((size_t)blockIdx.x * nz + blockIdx.z) * 2 * P2 + c;
+    o[0] = s1;
+    o[P2] = s2;
+  } else {
+    out1[(size_t)J.slot * P2 + c] = s1;
+    out2[(size_t)J.slot * P2 + c] = s2;
+  }
+}
+// the split sums of k_mapgemv added in range order.  Grid (jobs, chunks of 256 coefficients).
+__global__ void __launch_bounds__(256) k_mapgemv_sum(const GVJob *jobs, int nz, const double *part, double *out1,
+                                                     double *out2, int P2) {
+  const int c = blockIdx.y * 256 + threadIdx.x;
+  if (c >= P2) return;
+  const double *o = part + (size_t)blockIdx.x * nz * 2 * P2 + c;
+  double s1 = 0.0, s2 = 0.0;
+  for (int z = 0; z < nz; z++) {
+    s1 += o[(size_t)z * 2 * P2];
+    s2 += o[(size_t)z * 2 * P2 + P2];
+  }
+  const int slot = jobs[blockIdx.x].slot;
+  out1[(size_t)slot * P2 + c] = s1;
+  out2[(size_t)slot * P2 + c] = s2;
 }
 
 // Adjoint surface charges of each listed sphere: at each of its exposed points, the sum over
