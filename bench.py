@@ -260,7 +260,9 @@ def bd_leg(pbsam_amd, synth, ctx, kappa, ntraj, nsteps, cpu_budget):
     interaction), starting near contact (centre distance 2R+6 A + U[0,10] A, random
     orientations).  One warm-up step, then `nsteps` timed steps of every trajectory.  The
     oracle runs trajectory 0 from the same start and seed on one host core for about
-    `cpu_budget` seconds; its poses after the first steps are compared with the device's."""
+    `cpu_budget` seconds; its poses after the first steps are compared with the device's.
+    Then the same trajectories with every step's forces and torques from the adjoint
+    (PBSAM_BD_ADJOINT, cutoffs 10/10 A), timed the same way ("adjoint")."""
     import numpy as np
     rng = np.random.default_rng(BATCH_SEED + 7)
     mols = [synth.molecule(BD_SPH, seed=s) for s in BD_SEEDS]
@@ -300,6 +302,34 @@ def bd_leg(pbsam_amd, synth, ctx, kappa, ntraj, nsteps, cpu_budget):
                workload="%d BD trajectories x 2 molecules x %d spheres, p=%d, %d points/sphere, D_t=0.01, "
                         "D_r=1e-4, dt=2 ps, forces every step, cutoffs 10/100 A, start 2R+6+U[0,10] A" %
                         (ntraj, BD_SPH, BD_P, BD_NPTS))
+    # the same trajectories with every step's forces and torques from the adjoint
+    # (PBSAM_BD_ADJOINT: the exact-mode fixed point by GMRES, then the adjoint), interactions cut
+    # at the 10 A polarization cutoff (which the adjoint needs)
+    # (the reference's schedule with the same cutoffs beside it: the 10-100 A interaction-only
+    # pairs, most of the work of the 10/100 A run, are not part of either)
+    def timed_bd(flags):
+        b = pbsam_amd.System(ctx, [mts[0], mts[1]] * ntraj, P, Q, groups=[2] * ntraj)
+        b.set_cutoffs(10.0, 10.0)
+        b.set_flags(flags)
+        b.bd_start(seeds, place=False, params=prm)
+        b.bd_run(1)
+        ctx.sync()
+        t0 = time.perf_counter()
+        rc_b, st_b, _ = b.bd_run(nsteps)
+        ctx.sync()
+        el_b = time.perf_counter() - t0
+        b.close()
+        return dict(traj_steps_per_s=ntraj * nsteps / el_b, s_per_step=el_b / nsteps, status=int(rc_b),
+                    running_after=int((st_b == pbsam_amd.BD_RUNNING).sum()))
+
+    out["adjoint"] = timed_bd(pbsam_amd.EXACT_DERIVATIVE | pbsam_amd.BD_ADJOINT)
+    out["adjoint"]["workload"] = ("the same trajectories, forces and torques of every step from the adjoint (exact "
+                                  "fixed point by GMRES + adjoint, squared relative residual 1e-16), cutoffs 10/10 A")
+    out["reference_10_10"] = timed_bd(0)
+    out["reference_10_10"]["workload"] = "the same trajectories, the reference's force evaluation, cutoffs 10/10 A"
+    print("Brownian dynamics, cutoffs 10/10 A: adjoint forces %.1f, reference schedule %.1f trajectory-steps/s"
+          % (out["adjoint"]["traj_steps_per_s"], out["reference_10_10"]["traj_steps_per_s"]), file=sys.stderr,
+          flush=True)
     if cpu_budget > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle

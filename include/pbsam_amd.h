@@ -192,6 +192,14 @@ int pbsam_system_set_cutoffs(pbsam_system *sys, double inter_rcutoff, double int
 #define PBSAM_EXT_SIGN 8
 #define PBSAM_SELF_FRAME 16
 #define PBSAM_EXACT_DERIVATIVE 31
+/* With PBSAM_EXACT_DERIVATIVE: pbsam_bd_run takes each step's forces and torques from
+ * pbsam_solve_exact and pbsam_adjoint_forces (squared relative residual 1e-16) instead of the
+ * reference's force evaluation, and moves the molecules by those lab-frame vectors (-dE/dr,
+ * -dE/domega).  The reference's CBDnam moves them by its returned force conj(q) F and torque,
+ * which pbsam_compute_forces reproduces.  No counterpart in the reference; needs identity
+ * surface matrices and the interaction cutoff at the polarization cutoff
+ * (PBSAM_ERR_UNSUPPORTED from pbsam_bd_run otherwise). */
+#define PBSAM_BD_ADJOINT 32
 int pbsam_system_set_flags(pbsam_system *sys, unsigned flags);
 int pbsam_solve_mutual(pbsam_system *sys, double tol, int max_rounds, unsigned flags, int *rounds_out);
 /* round 1 from self values, then a fixed number of simultaneous (Jacobi) rounds */

@@ -2618,7 +2618,7 @@ int pbsam_system_set_cutoffs(pbsam_system *S, double inter_rc, double interact_r
 }
 
 int pbsam_system_set_flags(pbsam_system *S, unsigned flags) {
-  if (!S || (flags & ~(unsigned)PBSAM_EXACT_DERIVATIVE)) return PBSAM_ERR_BADARG;
+  if (!S || (flags & ~(unsigned)(PBSAM_EXACT_DERIVATIVE | PBSAM_BD_ADJOINT))) return PBSAM_ERR_BADARG;
   for (auto &P : S->parts) pbsam_system_set_flags(P.get(), flags);
   S->flags = flags;
   S->lists_ok = false;

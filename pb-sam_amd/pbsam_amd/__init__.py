@@ -19,6 +19,7 @@ OK, ERR_BADARG, ERR_OVERLAP, ERR_NOCONV, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED, ERR_
 HROT_REFRESH = 1
 FULL_ORDERS, NO_GRAD_SKIP, EXT_SIGN, SELF_FRAME = 2, 4, 8, 16
 EXACT_DERIVATIVE = HROT_REFRESH | FULL_ORDERS | NO_GRAD_SKIP | EXT_SIGN | SELF_FRAME
+BD_ADJOINT = 32  # with EXACT_DERIVATIVE: Brownian-dynamics steps move by the adjoint's forces
 PMAX = 32
 
 D = ctypes.c_double
