@@ -1466,7 +1466,7 @@ struct GradState {
 struct BdState;  // Brownian-dynamics state of a batch (pbsam_host_bd.inc)
 
 // Buffers of the adjoint solve (pbsam_adjoint_forces)
-constexpr int KRYLOV_RESTART = 30;  // GMRES basis size before a restart
+constexpr int KRYLOV_RESTART = 60;  // GMRES basis size before a restart
 struct AdjState {
   DBuf<double> aF, aH, wF, wH, zeta, rot, inter;  // [sphere][p^2]: adjoint blocks, w = M^T a, R w, sums
   DBuf<double> qTF, qTH, lqF, lqH, sT;            // numerical part: per job point, per point, harmonic sums
