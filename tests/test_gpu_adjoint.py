@@ -485,3 +485,51 @@ def test_exact_path_without_stored_maps(gpu):
     (e0, f0, t0, st0), (e1, f1, t1, st1) = res
     assert st0 > 0 and st1 == 0
     assert relerr(e1, e0) < 1e-11 and relerr(f1, f0) < 1e-10 and relerr(t1, t0) < 1e-10
+
+
+def test_adjoint_mixed_pole_orders_equal_single_systems(gpu):
+    """a batch handle with systems at p = 10 and p = 30 (split into one part per order): a pair
+    and a group of three at p = 10, a pair at p = 30; the exact solve and the adjoint of every
+    group equal the same systems solved alone"""
+    import pbsam_amd
+    from pbsam_amd import synth
+    types = {}
+    for t, (p, n, npts, seed) in enumerate([(10, 4, 400, 951), (10, 5, 400, 952), (30, 3, 1200, 953),
+                                            (30, 3, 1200, 954)]):
+        m = synth.molecule(n, seed=seed)
+        g = pbsam_amd.MolType(gpu, m["cen"], m["rad"], m["chg_xyz"], m["chg_q"], p, eps_in=4.0, rcen=m["rcen"],
+                              npts=[npts] * n)
+        g.selfpol(tol=0, max_rounds=0)
+        g.values()
+        types[t] = (m, g)
+    rng = np.random.default_rng(955)
+    systems = []
+    for a, b in ((0, 1), (2, 3)):
+        d = synth.ball_radius(len(types[a][0]["rad"])) + synth.ball_radius(len(types[b][0]["rad"])) + 6.0
+        pos, quat = synth.place_pair(types[a][0], types[b][0], d, seed=956 + a)
+        systems.append(([a, b], pos, quat))
+    d = 2 * synth.ball_radius(5) + 6.0
+    pos = np.array([[0.0, 0.0, 0.0], [d, 0.0, 0.0], [0.5 * d, 0.5 * np.sqrt(3.0) * d, -0.3]])
+    systems.insert(1, ([0, 1, 1], pos, np.array([synth.random_quat(rng) for _ in range(3)])))
+
+    def solved(mts, pos, quat, groups=None):
+        S = pbsam_amd.System(gpu, mts, pos, quat, groups=groups)
+        S.set_cutoffs(100.0, 100.0)
+        S.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+        n, rc = S.solve_exact(tol=1e-24, max_rounds=3000)
+        assert rc == 0, n
+        f, t, _ = S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)
+        e = S.energy()
+        S.close()
+        return f, t, e
+
+    alone = [solved([types[k][1] for k in ks], pos, quat) for ks, pos, quat in systems]
+    fb, tb, eb = solved([types[k][1] for ks, _, _ in systems for k in ks], np.vstack([p for _, p, _ in systems]),
+                        np.vstack([q for _, _, q in systems]), groups=[len(ks) for ks, _, _ in systems])
+    o = 0
+    for (ks, _, _), (f, t, e) in zip(systems, alone):
+        n = len(ks)
+        assert relerr(fb[o:o + n], f) < 1e-9, (ks, fb[o:o + n], f)
+        assert relerr(tb[o:o + n], t) < 1e-9
+        assert relerr(eb[o:o + n], e) < 1e-10
+        o += n
