@@ -149,7 +149,8 @@ def exact_leg(pbsam_amd, ctx, mts, pos, quat, reps):
     (pbsam_solve_exact) and every molecule's force and torque from the adjoint
     (pbsam_adjoint_forces), both to a squared relative residual of 1e-16, timed over `reps`
     evaluations after a 1e-24 evaluation (warm-up and the reference the timed forces are
-    compared with).  The reference's own rounds do not converge on these molecules (nearly
+    compared with); the poses are set again before each, so every timed evaluation remakes its
+    pair lists, rotations and translation matrices as at new poses.  The reference's own rounds do not converge on these molecules (nearly
     touching spheres), so no forward-mode exact evaluation exists to time beside it."""
     import numpy as np
     S = pbsam_amd.System(ctx, mts, pos, quat)
@@ -163,6 +164,8 @@ def exact_leg(pbsam_amd, ctx, mts, pos, quat, reps):
         mark.roctxRangePushA(b"exact_region")
     ts = ta = 0.0
     for _ in range(reps):
+        for i in range(len(mts)):  # poses set again: pair lists, rotations and matrices are remade
+            S.set_pose(i, pos[i], quat[i])
         t0 = time.perf_counter()
         napp, rc = S.solve_exact(tol=1e-16, max_rounds=1000)
         t1 = time.perf_counter()
@@ -484,6 +487,8 @@ def large_leg(pbsam_amd, synth, ctx, nsph, solves, evals):
     E.set_flags(pbsam_amd.EXACT_DERIVATIVE)
     ex = {}
     for rep in range(2):
+        for i in range(LARGE_MOLS):  # poses set again: pair lists, rotations and matrices are remade
+            E.set_pose(i, pos[i], quat[i])
         mark = roctx() if rep else None
         if mark:
             mark.roctxRangePushA(b"large_exact_region")

@@ -1477,7 +1477,7 @@ struct AdjState {
   DBuf<RJob> rj;
   DBuf<double> rP2, rst2;                         // the same for the operator's intra jobs
   DBuf<RJob> rj2;
-  DBuf<XJob> xinter, xintra;                      // inter jobs applying them; intra jobs, stored first
+  DBuf<XJob> xinter, xintra, xall;                // inter jobs applying them; intra jobs, stored first
   DBuf<GVJob> gnum, gsurf;                        // stored-matrix products: numerical pairs, surfaces
   DBuf<int> ptgt, joff, jlist, tsph, town, ext;
   DBuf<int> mode, sel;                            // per pass: right-hand side kinds, source spheres
@@ -1514,6 +1514,9 @@ struct pbsam_system {
   DBuf<int> d_pol, d_inter_off, d_inter_lst, d_intra_off, d_intra_lst, d_ones_off;
   int n_inter = 0, n_xintra = 0, n_pintra = 0, n_intra_slots = 0;
   bool xintra_allrot = false;  // every analytical intra job carries stored rotations (k_xapply)
+  bool tinter_full = false;    // d_Tinter holds every inter pair's matrices at full order
+  bool inter_rot_ok = false;   // adj.xinter / adj.rst hold the inter pairs' stored rotations
+  int inter_rot_n = 0;         // how many of adj.xinter's jobs carry them (those first)
   DBuf<XJob> d_xinter, d_xintra;
   DBuf<PJob> d_pintra;
   DBuf<TJob> d_tinter;
@@ -1854,6 +1857,8 @@ static int system_lists(pbsam_system *S) {
   S->n_pintra = (int)pa.size();
   S->n_intra_slots = aslot;
   S->d_Tinter.alloc(std::max<size_t>(ti.size() * TS, 1));
+  S->tinter_full = false;
+  S->inter_rot_ok = false;
   for (size_t k = 0; k < xi.size(); k++) xi[k].TH = S->d_Tinter.p + (size_t)k * TS;
   S->d_xinter.upload(xi, s);
   S->d_xintra.upload(xa, s);
@@ -1920,6 +1925,7 @@ static double system_round(pbsam_system *S, bool round1, bool refresh) {
   xa.slotH = S->d_slotI.p;
   xa.p = p;
   launch_trans(c, S->n_inter, S->d_tinter.p, p, S->d_Tinter.p, nullptr, xa.TQH);
+  S->tinter_full = false;
   launch_xform(c, S->n_inter, xa);
   LArgs la{S->d_pol.p, S->d_inter_off.p, S->d_inter_lst.p, S->d_slotI.p, nullptr, S->d_LSrot.p, nullptr, p};
   launch_sumslots(c, np, la);
@@ -2114,6 +2120,7 @@ static void system_gs_round(pbsam_system *S, const std::vector<double> &dev, con
     const std::array<int, 12> &r = rng[s0];
     if (r[1]) {
       launch_trans(c, r[1], S->d_tinter.p, p, S->d_Tinter.p, sb + r[0], S->tq(S->d_TQH.p));
+      S->tinter_full = false;
       xa.sel = sb + r[0];
       launch_xform(c, r[1], xa);
     }
