@@ -533,3 +533,38 @@ def test_adjoint_mixed_pole_orders_equal_single_systems(gpu):
         assert relerr(tb[o:o + n], t) < 1e-9
         assert relerr(eb[o:o + n], e) < 1e-10
         o += n
+
+
+def test_adjoint_after_a_move_equals_a_fresh_system(gpu, orc):
+    """the inter rotations and full-order translation matrices the exact solve leaves for the
+    adjoint are made again after a move (set_pose) and after mutual rounds that write the
+    matrices: the results equal a fresh handle's at the same poses"""
+    import pbsam_amd
+    gs, _, pos, quat = pair(gpu, orc, 4, 10, 400)
+
+    def exact(S):
+        n, rc = S.solve_exact(tol=1e-24, max_rounds=3000)
+        assert rc == 0, n
+        return S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)[:2]
+
+    def fresh(pos, quat):
+        S = pbsam_amd.System(gpu, gs, pos, quat)
+        S.set_cutoffs(100.0, 100.0)
+        S.set_flags(pbsam_amd.EXACT_DERIVATIVE)
+        return S
+
+    S = fresh(pos, quat)
+    exact(S)
+    pos2, quat2 = pos.copy(), quat.copy()
+    pos2[1] += np.array([0.7, -0.4, 0.3])
+    quat2[1] = turned(quat[1], 2, 0.2)
+    S.set_pose(1, pos2[1], quat2[1])
+    f2, t2 = exact(S)
+    f2r, t2r = exact(fresh(pos2, quat2))
+    assert relerr(f2, f2r) < 1e-10 and relerr(t2, t2r) < 1e-10
+    assert relerr(f2, exact(fresh(pos, quat))[0]) > 1e-3  # the move changes the forces
+    # mutual rounds between the exact solve and the adjoint write the translation matrices
+    S.solve_exact(tol=1e-24, max_rounds=3000)
+    S.solve(tol=1e-18, max_rounds=4000)
+    f3, t3 = S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)[:2]
+    assert relerr(f3, f2r) < 1e-8 and relerr(t3, t2r) < 1e-8
