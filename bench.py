@@ -540,8 +540,8 @@ def cpu_info():
 
 
 def host_workers():
-    """worker threads for the parallel CPU legs: the usable cores, at most 16 (the GPU box's
-    CPU share for one GPU)"""
+    """worker threads for the parallel CPU legs: the usable cores, at most 16 (the host's CPU
+    share for one GPU)"""
     return max(1, min(16, cpu_info()["usable"]))
 
 
