@@ -9,7 +9,8 @@ which the reference's forward-mode gradient phase is the derivative of that ener
 `RigidEnergy.apply(x, ev)` is the same energy as a function of x = (displacement of molecule
 j's centre, rotation vector of a lab-frame turn of j about its centre), six float64 values;
 its backward at x is -(force, torque) of pbsam_adjoint_forces at the displaced pose (exact at
-x = 0, where the turn's derivative is the torque), from the adjoint solve.
+x = 0, where the turn's derivative is the torque), from the adjoint solve (any number of
+molecules: the adjoint solves per molecule when there are more than two).
 
 Both exist so that torch.autograd.gradcheck can compare the device's analytic derivatives
 with central differences of the device's own energy; nothing in the library depends on torch.

@@ -568,3 +568,22 @@ def test_adjoint_after_a_move_equals_a_fresh_system(gpu, orc):
     S.solve(tol=1e-18, max_rounds=4000)
     f3, t3 = S.adjoint_forces(tol=ADJ_TOL, max_rounds=20000)[:2]
     assert relerr(f3, f2r) < 1e-8 and relerr(t3, t2r) < 1e-8
+
+
+def test_gradcheck_rigid_energy_three_molecules(gpu, orc):
+    """torch.autograd.gradcheck of every molecule's energy in the triangle of three molecules
+    over its six rigid-body coordinates (the adjoint's per-molecule solves in backward)"""
+    import pbsam_amd
+    pbsam_amd.load()   # the library's HIP runtime first, then torch
+    import torch
+    from pbsam_amd import autograd
+    gs, pos, quat = triangle(gpu, orc, seed=27)
+    for j in range(3):
+        ev = autograd.Evaluator(gpu, gs, pos, quat, j, tol=1e-18, max_rounds=4000, adj_tol=ADJ_TOL,
+                                adj_sweeps=20000)
+        x = torch.zeros(6, dtype=torch.float64, requires_grad=True)
+        _, g = autograd.solve_rigid(ev, np.zeros(6), True)
+        scale = float(np.linalg.norm(g))
+        assert scale > 0
+        assert torch.autograd.gradcheck(lambda y: autograd.RigidEnergy.apply(y, ev), (x,), eps=1e-4,
+                                        atol=1e-6 * scale, rtol=1e-6)
